@@ -1,0 +1,199 @@
+/*
+ * atmo_hip.h -- C ABI of libatmo_hip.so, an MI355X (gfx950) implementation of
+ * the DGSEM semidiscretization `rhs!` used by TrixiAtmo.jl on P4est meshes.
+ *
+ * All entry points are `extern "C"`, take plain pointers and sizes, and return
+ * an int status (ATMO_OK = 0, negative on error; `atmo_last_error` gives the
+ * message).  Arrays use Julia's column-major layout (first index fastest), so
+ * a Julia caller passes `pointer(u_ode)` etc. without copies or transposes.
+ * One handle owns its device buffers and one HIP stream; calls on one handle
+ * must be serialised by the caller (as OrdinaryDiffEq does for `rhs!`).
+ * Device-pointer entry points (`*_device`) take pointers into device memory
+ * that the caller owns.
+ */
+#ifndef ATMO_HIP_H
+#define ATMO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define ATMO_OK 0
+#define ATMO_ERR_INVALID (-1)     /* bad argument / unsupported combination */
+#define ATMO_ERR_DEVICE (-2)      /* HIP runtime error */
+#define ATMO_ERR_NONFINITE (-3)   /* non-finite or non-physical state found */
+#define ATMO_ERR_NEWTON (-4)      /* saturation Newton: too many failures */
+#define ATMO_ERR_MESH (-5)        /* mesh construction failed */
+
+/* -------------------------------------------------------- enumerations */
+/* equation systems */
+#define ATMO_EQ_EULER_ENERGY_GRAVITY_3D 1 /* CompressibleEulerEnergyEquationsWithGravity3D */
+
+/* volume (symmetric two-point) fluxes */
+#define ATMO_VOL_RANOCHA 0        /* flux_ranocha */
+#define ATMO_VOL_KENNEDY_GRUBER 1 /* flux_kennedy_gruber */
+#define ATMO_VOL_SHIMA 2          /* flux_shima_etal */
+#define ATMO_VOL_CENTRAL 3        /* flux_central */
+
+/* nonconservative gravity fluxes */
+#define ATMO_NC_ZERO 0            /* flux_zero */
+#define ATMO_NC_WARUSZEWSKI 1     /* flux_nonconservative_waruszewski_etal */
+#define ATMO_NC_SOUZA 2           /* flux_nonconservative_souza_etal */
+
+/* surface fluxes */
+#define ATMO_SURF_LAX_FRIEDRICHS 0 /* flux_lax_friedrichs (max_abs_speed) */
+#define ATMO_SURF_RANOCHA 1        /* flux_ranocha (entropy conservative) */
+#define ATMO_SURF_KENNEDY_GRUBER 2 /* flux_kennedy_gruber */
+#define ATMO_SURF_LMARS 3          /* FluxLMARS(c) */
+
+/* arithmetic of the two-point flux evaluation and volume sums */
+#define ATMO_PREC_FP64 0          /* everything in FP64 */
+#define ATMO_PREC_FP32_KAHAN 1    /* FP32 fluxes, Kahan-compensated FP32 sums */
+#define ATMO_PREC_FP32 2          /* FP32 fluxes, plain FP32 sums */
+
+/* node-local source terms (added after the Jacobian, as calc_sources!) */
+#define ATMO_SRC_NONE 0
+#define ATMO_SRC_CORIOLIS 1       /* source_terms_coriolis: -2 Omega e_z x rho v */
+
+/* boundary conditions (per boundary face) */
+#define ATMO_BC_SLIP_WALL 0       /* boundary_condition_slip_wall */
+
+/* node-index codes of Trixi's P4estInterfaceContainer.node_indices */
+#define ATMO_IDX_BEGIN 0
+#define ATMO_IDX_END 1
+#define ATMO_IDX_I_FORWARD 2
+#define ATMO_IDX_I_BACKWARD 3
+#define ATMO_IDX_J_FORWARD 4
+#define ATMO_IDX_J_BACKWARD 5
+
+/* ------------------------------------------------------------ basis */
+/* LobattoLegendreBasis(polydeg): nodes[n], weights[n], D[n*n] and
+ * Dsplit[n*n] (row-major, D[i*n+j] = d l_j/dx (x_i)); any pointer may be NULL.
+ * Replaces Trixi's LobattoLegendreBasis as constructed by DGSEM(polydeg=...)
+ * (examples/euler/dry_air/global_circulation/elixir_energy_baroclinic_instability.jl:27). */
+int atmo_lgl_basis(int n, double* nodes, double* weights, double* D, double* Dsplit);
+
+/* ------------------------------------------------------------ meshes */
+typedef struct atmo_mesh3d atmo_mesh3d;
+
+/* P4estMesh(trees_per_dimension; polydeg, coordinates_min, coordinates_max,
+ * periodicity) with initial_refinement_level = 0 (one element per tree). */
+int atmo_mesh3d_cartesian(int polydeg, const int32_t trees_per_dimension[3],
+                          const double coordinates_min[3],
+                          const double coordinates_max[3],
+                          const int32_t periodicity[3], atmo_mesh3d** mesh);
+
+/* P4estMeshCubedSphereTopography(trees_per_face_dimension, layers,
+ * inner_radius, thickness; polydeg) with the perfect-sphere topography
+ * (src/meshes/p4est_cubed_sphere_topography.jl:114-163). */
+int atmo_mesh3d_cubed_sphere_shell(int polydeg, int trees_per_face_dimension,
+                                   int layers, double inner_radius,
+                                   double thickness, atmo_mesh3d** mesh);
+
+/* sizes of a built mesh */
+int atmo_mesh3d_sizes(const atmo_mesh3d* mesh, int32_t* n_elements,
+                      int32_t* n_interfaces, int32_t* n_boundaries);
+
+/* Borrowed pointers into the mesh (valid until atmo_mesh3d_destroy):
+ *   node_coordinates [3,n,n,n,E], contravariant_vectors [3,3,n,n,n,E],
+ *   inverse_jacobian [n,n,n,E], interface_neighbor_ids [2,I] (0-based),
+ *   interface_node_indices [3,2,I], boundary_element [B],
+ *   boundary_node_indices [3,B], boundary_kind [B] (0 = :inside, 1 = :outside,
+ *   2 = other).  These are the arrays of Trixi's P4estElementContainer,
+ *   P4estInterfaceContainer and P4estBoundaryContainer. */
+int atmo_mesh3d_arrays(const atmo_mesh3d* mesh, const double** node_coordinates,
+                       const double** contravariant_vectors,
+                       const double** inverse_jacobian,
+                       const int32_t** interface_neighbor_ids,
+                       const int32_t** interface_node_indices,
+                       const int32_t** boundary_element,
+                       const int32_t** boundary_node_indices,
+                       const int32_t** boundary_kind);
+void atmo_mesh3d_destroy(atmo_mesh3d* mesh);
+
+/* ---------------------------------------------- semidiscretization */
+typedef struct atmo_handle atmo_handle;
+
+typedef struct {
+  int32_t equations;        /* ATMO_EQ_* */
+  int32_t polydeg;          /* 1..7 */
+  int32_t n_elements;
+  int32_t volume_flux;      /* ATMO_VOL_* */
+  int32_t volume_noncons;   /* ATMO_NC_* */
+  int32_t surface_flux;     /* ATMO_SURF_* */
+  int32_t surface_noncons;  /* ATMO_NC_* */
+  int32_t precision;        /* ATMO_PREC_* */
+  double c_p, c_v, gravity; /* CompressibleEulerEnergyEquationsWithGravity3D(; c_p, c_v, gravity) */
+  double lmars_c;           /* FluxLMARS speed of sound */
+  int32_t source_terms;     /* ATMO_SRC_* */
+  double rotation_rate;     /* Omega of source_terms_coriolis */
+} atmo_config;
+
+typedef struct {
+  const double* contravariant_vectors;   /* [3,3,n,n,n,E] */
+  const double* inverse_jacobian;        /* [n,n,n,E] */
+  int32_t n_interfaces;
+  const int32_t* interface_neighbor_ids; /* [2,I] 0-based element ids */
+  const int32_t* interface_node_indices; /* [3,2,I] ATMO_IDX_* codes */
+  int32_t n_boundaries;
+  const int32_t* boundary_element;       /* [B] */
+  const int32_t* boundary_node_indices;  /* [3,B] */
+  const int32_t* boundary_condition;     /* [B] ATMO_BC_* */
+} atmo_geometry3d;
+
+/* SemidiscretizationHyperbolic(mesh, equations, initial_condition, solver;
+ * boundary_conditions) for P4estMesh{3}: copies geometry and connectivity to
+ * the device; replaces Trixi.create_cache (src/solvers/dgsem_p4est/dg.jl:5-32). */
+int atmo_create(const atmo_config* config, const atmo_geometry3d* geometry,
+                atmo_handle** handle);
+void atmo_destroy(atmo_handle* handle);
+const char* atmo_last_error(const atmo_handle* handle); /* NULL handle: global */
+
+/* number of doubles in one state vector (n_vars * n^3 * n_elements) */
+int64_t atmo_state_length(const atmo_handle* handle);
+
+/* Trixi.rhs!(du_ode, u_ode, semi, t) with host arrays in Trixi layout
+ * [v,i,j,k,e]; replaces rhs_hyperbolic!
+ * (src/solvers/dgsem_p4est/dg_2d_manifold_in_3d_covariant.jl:8-71 for the
+ * covariant 2D form; the 3D P4est form of Trixi [ext]). */
+int atmo_rhs(atmo_handle* handle, double* du, const double* u, double t);
+/* same with device pointers (no copies) */
+int atmo_rhs_device(atmo_handle* handle, double* du, const double* u, double t);
+
+/* resident state: upload / download the handle's u */
+int atmo_set_state(atmo_handle* handle, const double* u_host);
+int atmo_get_state(atmo_handle* handle, double* u_host);
+
+/* n_steps CarpenterKennedy2N54 steps of size dt on the resident state, every
+ * stage one fused launch (OrdinaryDiffEq low-storage 2N scheme as used by
+ * examples/shallow_water/covariant/elixir_barotropic_instability.jl:79). */
+int atmo_step(atmo_handle* handle, double dt, int32_t n_steps);
+
+/* StepsizeCallback max_dt on the resident state: dt = cfl * 2 / (n * max
+ * scaled speed) (src/callbacks_step/stepsize_dg2d.jl:5-43 for the 2D form). */
+int atmo_max_dt(atmo_handle* handle, double cfl, double* dt);
+
+/* Integrals on the resident state (analysis_covariant.jl:53-105 analogues):
+ * out[0] mass, out[1] entropy, out[2] dS/dt = int w(u).du of the last
+ * computed rhs (or of a fresh one when `fresh_rhs`), out[3..] integral of each
+ * conserved variable. */
+int atmo_integrals(atmo_handle* handle, int32_t fresh_rhs, double* out);
+
+/* wait for all work of the handle's stream */
+int atmo_synchronize(atmo_handle* handle);
+
+/* Kernel timing: when enabled, every launch is bracketed by HIP events on the
+ * handle's stream; atmo_kernel_times returns per kernel class the number of
+ * launches and the total milliseconds (classes: 0 fused stage/rhs kernel,
+ * 1 reductions). */
+int atmo_enable_timing(atmo_handle* handle, int32_t enable);
+int atmo_kernel_times(atmo_handle* handle, int32_t kernel_class,
+                      int64_t* launches, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

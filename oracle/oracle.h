@@ -1,0 +1,123 @@
+/*
+ * CPU ORACLE -- test infrastructure only.
+ *
+ * A from-scratch FP64 C++ restatement of the TrixiAtmo.jl / Trixi.jl DGSEM
+ * right-hand side that the HIP library accelerates.  Only the test suite,
+ * the smoke check and the CPU-baseline leg of bench.py load this library;
+ * the product library never links or calls it.
+ *
+ * Every function names the reference file:line it restates.  Functions whose
+ * per-call behaviour cannot be pinned by a reference fixture are pinned by the
+ * closed forms and invariants listed in the tests (free-stream preservation,
+ * conservation, entropy conservation, SBP identities, exactness on
+ * polynomials, 4th-order convergence of the Runge-Kutta scheme).
+ *
+ * Layouts are Julia column-major (first index fastest), as Trixi's
+ * `wrap_array` produces:
+ *   u, du           [v, i, j, k, e]         v fastest
+ *   contravariant   [c, d, i, j, k, e]      c = Cartesian component, d = direction
+ *   inverse_jac     [i, j, k, e]
+ *   node_coords     [c, i, j, k, e]
+ * Interface / boundary arrays follow Trixi's P4estInterfaceContainer and
+ * P4estBoundaryContainer with 0-based element ids and node-index codes
+ *   0 = :begin, 1 = :end, 2 = :i_forward, 3 = :i_backward,
+ *   4 = :j_forward, 5 = :j_backward.
+ */
+#ifndef ATMO_ORACLE_H
+#define ATMO_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- basis (Trixi LobattoLegendreBasis, restated from Kopriva 2009) ---- */
+void ora_lgl_nodes_weights(int n, double* nodes, double* weights);
+void ora_derivative_matrix(int n, const double* nodes, double* D); /* D[i*n+j] */
+void ora_derivative_split(int n, const double* nodes, const double* weights,
+                          double* Dsplit);
+
+/* ---- mean values (Ranocha, Schlottke-Lakemper, Chan, Rueda-Ramirez,
+ *      Winters, Hindenlang, Gassner 2023, ACM TOMS; imported by
+ *      src/TrixiAtmo.jl:37) ---- */
+double ora_ln_mean(double x, double y);
+double ora_inv_ln_mean(double x, double y);
+
+/* ---- dry Euler, energy form, with gravity (6 variables) ---- */
+enum {
+  ORA_VOL_RANOCHA = 0,
+  ORA_VOL_KENNEDY_GRUBER = 1,
+  ORA_VOL_SHIMA = 2,
+  ORA_VOL_CENTRAL = 3
+};
+enum { ORA_NC_ZERO = 0, ORA_NC_WARUSZEWSKI = 1, ORA_NC_SOUZA = 2 };
+enum {
+  ORA_SURF_LAX_FRIEDRICHS = 0,
+  ORA_SURF_RANOCHA = 1,
+  ORA_SURF_KENNEDY_GRUBER = 2,
+  ORA_SURF_LMARS = 3
+};
+
+typedef struct {
+  int n;          /* nodes per direction (polydeg + 1) */
+  int n_elements;
+  double c_p, c_v, gravity;
+  int volume_flux, volume_noncons, surface_flux, surface_noncons;
+  double lmars_c; /* speed of sound estimate for FluxLMARS */
+  double coriolis_rate; /* > 0: source_terms_coriolis with this rotation rate */
+} ora_e3_params;
+
+void ora_e3_flux(const ora_e3_params* p, const double* u, const double* normal,
+                 double* f);
+void ora_e3_volume_flux(const ora_e3_params* p, int which, const double* ull,
+                        const double* urr, const double* normal, double* f);
+void ora_e3_noncons_flux(const ora_e3_params* p, int which, const double* ull,
+                         const double* urr, const double* normal, double* f);
+void ora_e3_surface_flux(const ora_e3_params* p, int which, const double* ull,
+                         const double* urr, const double* normal, double* f);
+double ora_e3_max_abs_speed(const ora_e3_params* p, const double* ull,
+                            const double* urr, const double* normal);
+void ora_e3_cons2entropy(const ora_e3_params* p, const double* u, double* w);
+double ora_e3_entropy(const ora_e3_params* p, const double* u);
+
+typedef struct {
+  int n_interfaces;
+  const int* neighbor_ids;  /* [2, I] */
+  const int* node_indices;  /* [3, 2, I] codes */
+  int n_boundaries;
+  const int* bnd_element;   /* [B] */
+  const int* bnd_node_indices; /* [3, B] */
+  const int* bnd_kind;      /* [B], 0 = slip wall */
+} ora_connectivity;
+
+/* full rhs!: du = -(1/J) (volume + surface) */
+int ora_e3_rhs(const ora_e3_params* p, const ora_connectivity* c,
+               const double* contravariant, const double* inverse_jacobian,
+               const double* u, double* du);
+
+/* sum over all contributions of |contribution| for every entry of du: the
+ * scale that bounds the rounding error of the tendency */
+int ora_e3_rhs_magnitude(const ora_e3_params* p, const ora_connectivity* c,
+                         const double* contravariant, const double* inverse_jacobian,
+                         const double* u, double* du_abs);
+
+/* CarpenterKennedy2N54 steps with fixed dt (OrdinaryDiffEq 2N scheme) */
+int ora_e3_ck2n54(const ora_e3_params* p, const ora_connectivity* c,
+                  const double* contravariant, const double* inverse_jacobian,
+                  double* u, double dt, int n_steps);
+
+/* StepsizeCallback max_dt for P4estMesh{3} (without the CFL factor) */
+double ora_e3_max_dt(const ora_e3_params* p, const double* contravariant,
+                     const double* inverse_jacobian, const double* u);
+
+/* integrals: out[0] = mass, out[1] = entropy, out[2] = dS/dt = int w(u).du,
+ * out[3..8] = integral of every conserved variable */
+void ora_e3_integrals(const ora_e3_params* p, const double* inverse_jacobian,
+                      const double* u, const double* du, double* out);
+
+/* CarpenterKennedy2N54 coefficients A[5], B[5], c[5] */
+void ora_ck2n54_coefficients(double* A, double* B, double* c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

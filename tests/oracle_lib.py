@@ -1,0 +1,188 @@
+"""ctypes access to the CPU oracle (oracle/build/liboracle.so) for the tests.
+
+The oracle is the checker: tests compare the HIP library against it, never the
+other way round, and nothing in the product imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "build", "liboracle.so")
+
+dp = ctypes.POINTER(ctypes.c_double)
+ip = ctypes.POINTER(ctypes.c_int)
+
+
+class E3Params(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("n_elements", ctypes.c_int), ("c_p", ctypes.c_double),
+                ("c_v", ctypes.c_double), ("gravity", ctypes.c_double),
+                ("volume_flux", ctypes.c_int), ("volume_noncons", ctypes.c_int),
+                ("surface_flux", ctypes.c_int), ("surface_noncons", ctypes.c_int),
+                ("lmars_c", ctypes.c_double), ("coriolis_rate", ctypes.c_double)]
+
+
+class Connectivity(ctypes.Structure):
+    _fields_ = [("n_interfaces", ctypes.c_int), ("neighbor_ids", ip), ("node_indices", ip),
+                ("n_boundaries", ctypes.c_int), ("bnd_element", ip), ("bnd_node_indices", ip),
+                ("bnd_kind", ip)]
+
+
+def _stale():
+    t = os.path.getmtime(ORACLE_LIB)
+    return any(os.path.getmtime(os.path.join(ORACLE_DIR, f)) > t
+               for f in os.listdir(ORACLE_DIR) if f.endswith((".cpp", ".h")))
+
+
+def _d(a):
+    return a.ctypes.data_as(dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(ip)
+
+
+class Oracle:
+    def __init__(self):
+        if not os.path.exists(ORACLE_LIB) or _stale():
+            subprocess.run(["make", "-C", ORACLE_DIR, "-j8"], check=True, capture_output=True)
+        L = ctypes.CDLL(ORACLE_LIB)
+        L.ora_ln_mean.restype = ctypes.c_double
+        L.ora_inv_ln_mean.restype = ctypes.c_double
+        L.ora_ln_mean.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.ora_inv_ln_mean.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.ora_e3_max_abs_speed.restype = ctypes.c_double
+        L.ora_e3_entropy.restype = ctypes.c_double
+        L.ora_e3_max_dt.restype = ctypes.c_double
+        self.L = L
+
+    # ---- basis
+    def lgl(self, n):
+        x = np.zeros(n)
+        w = np.zeros(n)
+        self.L.ora_lgl_nodes_weights(n, _d(x), _d(w))
+        return x, w
+
+    def dmat(self, n):
+        x, w = self.lgl(n)
+        D = np.zeros(n * n)
+        self.L.ora_derivative_matrix(n, _d(x), _d(D))
+        return D.reshape(n, n)
+
+    def dsplit(self, n):
+        x, w = self.lgl(n)
+        D = np.zeros(n * n)
+        self.L.ora_derivative_split(n, _d(x), _d(w), _d(D))
+        return D.reshape(n, n)
+
+    def ln_mean(self, x, y):
+        return self.L.ora_ln_mean(x, y)
+
+    def inv_ln_mean(self, x, y):
+        return self.L.ora_inv_ln_mean(x, y)
+
+    def ck2n54(self):
+        A, B, c = np.zeros(5), np.zeros(5), np.zeros(5)
+        self.L.ora_ck2n54_coefficients(_d(A), _d(B), _d(c))
+        return A, B, c
+
+    # ---- 3D Euler
+    @staticmethod
+    def params(n, n_elements, c_p=1004.0, c_v=717.0, gravity=9.81, volume_flux=0,
+               volume_noncons=0, surface_flux=0, surface_noncons=0, lmars_c=0.0,
+               coriolis_rate=0.0):
+        return E3Params(n, n_elements, c_p, c_v, gravity, volume_flux, volume_noncons,
+                        surface_flux, surface_noncons, lmars_c, coriolis_rate)
+
+    def _vec(self, fn, p, *args):
+        out = np.zeros(6)
+        getattr(self.L, fn)(ctypes.byref(p), *args, _d(out))
+        return out
+
+    def flux(self, p, u, n):
+        return self._vec("ora_e3_flux", p, _d(np.asarray(u, float)), _d(np.asarray(n, float)))
+
+    def volume_flux(self, p, which, ul, ur, n):
+        return self._vec("ora_e3_volume_flux", p, which, _d(np.asarray(ul, float)),
+                         _d(np.asarray(ur, float)), _d(np.asarray(n, float)))
+
+    def noncons_flux(self, p, which, ul, ur, n):
+        return self._vec("ora_e3_noncons_flux", p, which, _d(np.asarray(ul, float)),
+                         _d(np.asarray(ur, float)), _d(np.asarray(n, float)))
+
+    def surface_flux(self, p, which, ul, ur, n):
+        return self._vec("ora_e3_surface_flux", p, which, _d(np.asarray(ul, float)),
+                         _d(np.asarray(ur, float)), _d(np.asarray(n, float)))
+
+    def cons2entropy(self, p, u):
+        return self._vec("ora_e3_cons2entropy", p, _d(np.asarray(u, float)))
+
+    def entropy(self, p, u):
+        return self.L.ora_e3_entropy(ctypes.byref(p), _d(np.asarray(u, float)))
+
+    def _conn(self, mesh, bc_kinds=None):
+        keep = [np.ascontiguousarray(mesh.interface_neighbor_ids.ravel(order="F"), dtype=np.int32),
+                np.ascontiguousarray(mesh.interface_node_indices.ravel(order="F"), dtype=np.int32),
+                np.ascontiguousarray(mesh.boundary_element, dtype=np.int32),
+                np.ascontiguousarray(mesh.boundary_node_indices.ravel(order="F"), dtype=np.int32),
+                np.zeros(mesh.n_boundaries, dtype=np.int32)]
+        c = Connectivity(mesh.n_interfaces, _i(keep[0]), _i(keep[1]), mesh.n_boundaries,
+                         _i(keep[2]), _i(keep[3]), _i(keep[4]))
+        return c, keep
+
+    def _geo(self, mesh):
+        cv = np.ascontiguousarray(mesh.contravariant_vectors.ravel(order="F"))
+        ij = np.ascontiguousarray(mesh.inverse_jacobian.ravel(order="F"))
+        return cv, ij
+
+    def rhs(self, p, mesh, u):
+        c, keep = self._conn(mesh)
+        cv, ij = self._geo(mesh)
+        uf = np.ascontiguousarray(np.asarray(u).ravel(order="F"), dtype=float)
+        du = np.zeros_like(uf)
+        rc = self.L.ora_e3_rhs(ctypes.byref(p), ctypes.byref(c), _d(cv), _d(ij), _d(uf), _d(du))
+        assert rc == 0
+        return du
+
+    def rhs_magnitude(self, p, mesh, u):
+        c, keep = self._conn(mesh)
+        cv, ij = self._geo(mesh)
+        uf = np.ascontiguousarray(np.asarray(u).ravel(order="F"), dtype=float)
+        du = np.zeros_like(uf)
+        assert self.L.ora_e3_rhs_magnitude(ctypes.byref(p), ctypes.byref(c), _d(cv), _d(ij),
+                                           _d(uf), _d(du)) == 0
+        return du
+
+    def ck2n54_steps(self, p, mesh, u, dt, n_steps):
+        c, keep = self._conn(mesh)
+        cv, ij = self._geo(mesh)
+        uf = np.ascontiguousarray(np.asarray(u).ravel(order="F"), dtype=float).copy()
+        rc = self.L.ora_e3_ck2n54(ctypes.byref(p), ctypes.byref(c), _d(cv), _d(ij), _d(uf),
+                                  ctypes.c_double(dt), n_steps)
+        assert rc == 0
+        return uf
+
+    def max_dt(self, p, mesh, u):
+        cv, ij = self._geo(mesh)
+        uf = np.ascontiguousarray(np.asarray(u).ravel(order="F"), dtype=float)
+        return self.L.ora_e3_max_dt(ctypes.byref(p), _d(cv), _d(ij), _d(uf))
+
+    def integrals(self, p, mesh, u, du=None):
+        _, ij = self._geo(mesh)
+        uf = np.ascontiguousarray(np.asarray(u).ravel(order="F"), dtype=float)
+        out = np.zeros(9)
+        dptr = _d(np.ascontiguousarray(du)) if du is not None else None
+        self.L.ora_e3_integrals(ctypes.byref(p), _d(ij), _d(uf), dptr, _d(out))
+        return out
+
+
+def params_for(semi_args):
+    """oracle parameters for (mesh, equations, solver, source omega)"""
+    mesh, eq, solver, omega = semi_args
+    codes = solver.codes()
+    return Oracle.params(mesh.nnodes, mesh.n_elements, eq.c_p, eq.c_v, eq.gravity,
+                         codes["volume_flux"], codes["volume_noncons"], codes["surface_flux"],
+                         codes["surface_noncons"], codes["lmars_c"], omega)

@@ -1,0 +1,151 @@
+"""GPU parity: the HIP rhs!, RK stages and reductions of libatmo_hip against
+the FP64 oracle on identical seeded states, the reference's regression values
+on the GPU, and size-independent properties at the full shell size.
+
+Tolerance: FP64 path <= 1e-12 relative per conserved variable, where the
+error max |gpu - oracle| is divided by max over nodes of sum_terms |term| (the
+oracle's magnitude mode; equal to max |du| when no terms cancel), and in
+addition <= 1e-9 relative to max |du| itself."""
+import numpy as np
+import pytest
+
+import trixiatmo_hip as T
+from oracle_lib import Oracle, params_for
+
+pytestmark = pytest.mark.gpu
+
+RHS_RTOL = 1e-12
+
+
+def rel_err_per_var(a, b, nv=6):
+    a = np.asarray(a).reshape(nv, -1, order="F")
+    b = np.asarray(b).reshape(nv, -1, order="F")
+    scale = np.maximum(np.abs(b).max(axis=1), 1e-300)
+    return np.abs(a - b).max(axis=1) / scale
+
+
+def make(workload, **kw):
+    mesh, eq, solver, bcs, u0 = workload(**kw)
+    return mesh, eq, solver, bcs, u0
+
+
+def check_rhs(oracle, mesh, eq, solver, bcs, u0, omega=0.0, source=None):
+    print(f"case: E={mesh.n_elements} n={mesh.nnodes} codes={solver.codes()}", flush=True)
+    semi = T.SemidiscretizationHyperbolic(mesh, eq, solver, bcs, source_terms=source)
+    du_gpu = semi.rhs(None, u0)
+    p = params_for((mesh, eq, solver, omega))
+    du_ref = oracle.rhs(p, mesh, u0)
+    # Near hydrostatic or geostrophic balance a tendency is a small residual of
+    # large cancelling terms and rounding scales with those terms: the error is
+    # measured against max over nodes of sum_terms |term| (the oracle's
+    # magnitude mode), which equals max |du| where nothing cancels.
+    du_mag = oracle.rhs_magnitude(p, mesh, u0)
+    scale = np.abs(du_mag.reshape(6, -1, order="F")).max(axis=1)
+    diff = np.abs((du_gpu - du_ref).reshape(6, -1, order="F")).max(axis=1)
+    err = diff / np.maximum(scale, 1e-300)
+    plain = diff / np.maximum(np.abs(du_ref.reshape(6, -1, order="F")).max(axis=1), 1e-300)
+    print("  rel err per variable (term scale):", err, flush=True)
+    print("  rel err per variable (|du| scale):", plain, flush=True)
+    assert np.all(err[:5] <= RHS_RTOL), err
+    assert np.all(plain[:5] <= 1e-9), plain
+    assert np.abs(du_gpu.reshape(6, -1, order="F")[5] -
+                  du_ref.reshape(6, -1, order="F")[5]).max() <= 1e-12 * max(1.0, np.abs(du_ref).max())
+    return semi, du_gpu, du_ref
+
+
+def test_rhs_periodic_cube_c1(oracle):
+    check_rhs(oracle, *make(T.workloads.periodic_cube, trees=4, polydeg=3, seed=0))
+
+
+@pytest.mark.parametrize("polydeg", [1, 2, 4, 5, 6, 7])
+def test_rhs_periodic_cube_polydeg(oracle, polydeg):
+    check_rhs(oracle, *make(T.workloads.periodic_cube, trees=2, polydeg=polydeg, seed=polydeg))
+
+
+@pytest.mark.parametrize("vol,surf,snc", [
+    ("flux_kennedy_gruber", "flux_lax_friedrichs", "flux_nonconservative_waruszewski_etal"),
+    ("flux_ranocha", "flux_ranocha", "flux_nonconservative_waruszewski_etal"),
+    ("flux_shima_etal", "flux_kennedy_gruber", "flux_zero"),
+])
+def test_rhs_shell(oracle, vol, surf, snc):
+    mesh, eq, solver, bcs, u0 = T.workloads.shell(3, 2, polydeg=4, seed=1, volume_flux=vol,
+                                                  surface_flux=surf, surface_noncons=snc)
+    u0 = u0.copy()
+    u0[1:4] += 10.0 * np.sin(mesh.node_coordinates / 1e6) * u0[0]
+    check_rhs(oracle, mesh, eq, solver, bcs, u0)
+
+
+def test_rhs_shell_polydeg7_ec(oracle):
+    mesh, eq, solver, bcs, u0 = T.workloads.shell(2, 2, polydeg=7, seed=2,
+                                                  volume_flux="flux_ranocha",
+                                                  surface_flux="flux_ranocha")
+    check_rhs(oracle, mesh, eq, solver, bcs, u0)
+
+
+def test_rhs_baroclinic_small(oracle):
+    mesh, eq, solver, bcs, u0 = T.workloads.baroclinic_instability(trees_per_face=3, layers=2)
+    check_rhs(oracle, mesh, eq, solver, bcs, u0, omega=T.EARTH_ROTATION_RATE,
+              source="source_terms_coriolis")
+
+
+def test_steps_max_dt_integrals_c1(oracle):
+    """config C1: one rhs! plus 50 CarpenterKennedy2N54 steps at CFL 0.5"""
+    mesh, eq, solver, bcs, u0 = T.workloads.periodic_cube(trees=4, polydeg=3, seed=0)
+    semi = T.SemidiscretizationHyperbolic(mesh, eq, solver, bcs)
+    p = params_for((mesh, eq, solver, 0.0))
+    semi.set_state(u0)
+    dt = semi.max_dt(0.5)
+    dt_ref = 0.5 * oracle.max_dt(p, mesh, u0)
+    assert abs(dt / dt_ref - 1) < 1e-14
+    ints = semi.integrals(fresh_rhs=True)
+    ints_ref = oracle.integrals(p, mesh, u0, oracle.rhs(p, mesh, u0))
+    np.testing.assert_allclose(ints[[0, 1, 3, 4, 5, 6, 7]], ints_ref[[0, 1, 3, 4, 5, 6, 7]],
+                               rtol=1e-13, atol=1e-15)
+    assert abs(ints[2] - ints_ref[2]) < 1e-12 * np.abs(ints_ref).max()
+    semi.step(dt, 50)
+    u_gpu = semi.get_state()
+    u_ref = oracle.ck2n54_steps(p, mesh, u0, dt, 50)
+    err = rel_err_per_var(u_gpu - u0.ravel(order="F"), u_ref - u0.ravel(order="F"))
+    print("50-step increment rel err:", err)
+    assert np.all(err[:5] < 1e-10), err
+
+
+# test/test_3d_euler_energy.jl:7-29 pins (tspan = (0, 0.01), tol = 1e-15, atol = 1e-8)
+BAROCLINIC_L2 = [6.540827547514395e-7, 0.0001972937313811121, 0.00042458497371703434,
+                 0.00018720446122477726, 0.06698234918918748, 0.0408936452021324]
+BAROCLINIC_LINF = [4.42523425103758e-6, 0.024276205576123866, 0.04570236938186356,
+                   0.022711920085996914, 0.9490993925137445, 0.3118668645620346]
+
+
+def test_baroclinic_regression_on_gpu():
+    mesh, eq, solver, bcs, u0 = T.workloads.baroclinic_instability()
+    semi = T.SemidiscretizationHyperbolic(mesh, eq, solver, bcs, source_terms="source_terms_coriolis")
+    semi.set_state(u0)
+    semi.step(0.0025, 4)
+    u1 = semi.get_state().reshape(u0.shape, order="F")
+    l2, linf = T.analysis.calc_error_norms(
+        u1, mesh, lambda x: T.initial_condition_baroclinic_instability(x, eq))
+    np.testing.assert_allclose(l2, BAROCLINIC_L2, rtol=1.5e-8, atol=1e-8)
+    np.testing.assert_allclose(linf, BAROCLINIC_LINF, rtol=1.5e-8, atol=1e-8)
+
+
+def test_full_shell_properties():
+    """C3 at full size (6 x 16^2 x 8, polydeg 4): mass conservation of the rhs,
+    well-balance of the unperturbed background, finite 100-step run."""
+    mesh, eq, solver, bcs, u0 = T.workloads.shell(16, 8, polydeg=4, seed=0)
+    semi = T.SemidiscretizationHyperbolic(mesh, eq, solver, bcs)
+    du = semi.rhs(None, u0).reshape(u0.shape, order="F")
+    _, w, _, _ = T.lgl_basis(4)
+    W = (w[:, None, None] * w[None, :, None] * w[None, None, :])[..., None] / mesh.inverse_jacobian
+    assert abs((W * du[0]).sum()) < 1e-12 * (W * np.abs(du[0])).sum()
+    ub = T.initial_condition_isothermal_perturbed(mesh, eq, n_bumps=0, inner_radius=6.371e6)
+    dub = semi.rhs(None, ub).reshape(u0.shape, order="F")
+    assert np.abs(dub[1:4]).max() < 1e-8 * np.abs(du[1:4]).max() + 1e-7
+    semi.set_state(u0)
+    dt = semi.max_dt(0.5)
+    semi.step(dt, 100)
+    u = semi.get_state()
+    assert np.all(np.isfinite(u))
+    mass0 = (W * u0[0]).sum()
+    mass1 = (W * u.reshape(u0.shape, order="F")[0]).sum()
+    assert abs(mass1 / mass0 - 1) < 1e-12

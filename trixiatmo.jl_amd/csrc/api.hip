@@ -1,0 +1,551 @@
+// C ABI of libatmo_hip (declared in include/atmo_hip.h): handle management,
+// host <-> device transfers, the RK driver and the mesh generators.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/atmo_hip.h"
+#include "atmo_internal.hpp"
+#include "euler3d_kernels.hpp"
+
+namespace atmo {
+void build_cartesian(Mesh3D& m, const Basis& b, const int trees[3], const double xmin[3],
+                     const double xmax[3], const int periodic[3]);
+void build_cubed_sphere_shell(Mesh3D& m, const Basis& b, int trees_per_face, int layers,
+                              double inner_radius, double thickness);
+}  // namespace atmo
+
+using namespace atmo;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(std::string* slot, int code, const std::string& msg) {
+  if (slot) *slot = msg;
+  g_error = msg;
+  return code;
+}
+
+struct TimerPair {
+  hipEvent_t a, b;
+  int cls;
+};
+
+}  // namespace
+
+struct atmo_mesh3d {
+  Mesh3D m;
+};
+
+struct atmo_handle {
+  atmo_config cfg;
+  int n = 0, n3 = 0, nv = 6;
+  int64_t len = 0;  // doubles per state
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double *d_u = nullptr, *d_u2 = nullptr, *d_k = nullptr, *d_du = nullptr;
+  double *d_uio = nullptr, *d_duio = nullptr;  // staging for atmo_rhs with host arrays
+  double *d_cv = nullptr, *d_invJ = nullptr;
+  int* d_nbr = nullptr;
+  int* d_face_info = nullptr;
+  double *d_partial = nullptr, *d_result = nullptr;
+  bool du_valid = false;
+  bool timing = false;
+  std::vector<TimerPair> timers;
+  int64_t launches[2] = {0, 0};
+  double total_ms[2] = {0, 0};
+  std::string err;
+};
+
+namespace {
+
+std::once_flag g_tables_once[64];
+hipError_t g_tables_status[64];
+
+hipError_t ensure_tables(int device) {
+  if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+  std::call_once(g_tables_once[device], [device]() {
+    double ds[9][64], w[9][8];
+    std::memset(ds, 0, sizeof(ds));
+    std::memset(w, 0, sizeof(w));
+    for (int n = 2; n <= 8; ++n) {
+      Basis b(n);
+      for (int q = 0; q < n * n; ++q) ds[n][q] = b.Dsplit[q];
+      for (int q = 0; q < n; ++q) w[n][q] = b.w[q];
+    }
+    g_tables_status[device] = e3_upload_basis_tables(&ds[0][0], &w[0][0]);
+  });
+  return g_tables_status[device];
+}
+
+inline int code_index(int code, int a, int b, int n) {
+  switch (code) {
+    case kBegin: return 0;
+    case kEnd: return n - 1;
+    case kIFwd: return a;
+    case kIBwd: return n - 1 - a;
+    case kJFwd: return b;
+    default: return n - 1 - b;
+  }
+}
+
+// direction 0..5 from Trixi node-index codes; -1 if malformed
+int indices2direction(const int32_t* idx) {
+  int dir = -1, nfix = 0, nsurf = 0;
+  for (int d = 0; d < 3; ++d) {
+    if (idx[d] == kBegin || idx[d] == kEnd) { dir = 2 * d + (idx[d] == kEnd); ++nfix; }
+    else if (idx[d] >= kIFwd && idx[d] <= kJBwd) ++nsurf;
+    else return -1;
+  }
+  return (nfix == 1 && nsurf == 2) ? dir : -1;
+}
+
+// own-face canonical surface index (a, b) of volume node (i, j, k) on face dir
+inline void face_ab(int dir, int i, int j, int k, int* a, int* b) {
+  const int d = dir / 2;
+  *a = d == 0 ? j : i;
+  *b = d == 2 ? j : k;
+}
+
+#define HIP_TRY(h, expr)                                                         \
+  do {                                                                           \
+    hipError_t e__ = (expr);                                                     \
+    if (e__ != hipSuccess)                                                       \
+      return fail((h) ? &(h)->err : nullptr, ATMO_ERR_DEVICE,                    \
+                  std::string(#expr) + ": " + hipGetErrorString(e__));           \
+  } while (0)
+
+void timer_begin(atmo_handle* h, int cls, TimerPair* tp) {
+  tp->cls = cls;
+  hipEventCreate(&tp->a);
+  hipEventCreate(&tp->b);
+  hipEventRecord(tp->a, h->stream);
+}
+
+void timer_end(atmo_handle* h, TimerPair* tp) {
+  hipEventRecord(tp->b, h->stream);
+  h->timers.push_back(*tp);
+}
+
+void collect_timers(atmo_handle* h) {
+  for (auto& tp : h->timers) {
+    hipEventSynchronize(tp.b);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, tp.a, tp.b);
+    h->launches[tp.cls] += 1;
+    h->total_ms[tp.cls] += ms;
+    hipEventDestroy(tp.a);
+    hipEventDestroy(tp.b);
+  }
+  h->timers.clear();
+}
+
+E3Launch base_launch(atmo_handle* h) {
+  E3Launch L{};
+  const double gamma = h->cfg.c_p / h->cfg.c_v;
+  L.q.gamma = gamma;
+  L.q.gm1 = gamma - 1.0;
+  L.q.inv_gm1 = 1.0 / (gamma - 1.0);
+  L.q.lmars_c = h->cfg.lmars_c;
+  L.n = h->n;
+  L.n_elements = h->cfg.n_elements;
+  L.volume_flux = h->cfg.volume_flux;
+  L.volume_noncons = h->cfg.volume_noncons;
+  L.surface_flux = h->cfg.surface_flux;
+  L.surface_noncons = h->cfg.surface_noncons;
+  L.cv = h->d_cv;
+  L.invJ = h->d_invJ;
+  L.nbr = h->d_nbr;
+  L.face_info = h->d_face_info;
+  L.coriolis = h->cfg.source_terms == ATMO_SRC_CORIOLIS ? 2.0 * h->cfg.rotation_rate : 0.0;
+  return L;
+}
+
+int launch_stage(atmo_handle* h, const E3Launch& L) {
+  TimerPair tp;
+  if (h->timing) timer_begin(h, 0, &tp);
+  const bool ok = e3_launch_stage_fp64(L, h->stream);
+  if (h->timing) timer_end(h, &tp);
+  if (!ok) return fail(&h->err, ATMO_ERR_INVALID, "no kernel for this flux/polydeg combination");
+  HIP_TRY(h, hipGetLastError());
+  return ATMO_OK;
+}
+
+int run_rhs(atmo_handle* h, double* du, const double* u) {
+  E3Launch L = base_launch(h);
+  L.mode = 0;
+  L.u = u;
+  L.out = du;
+  return launch_stage(h, L);
+}
+
+}  // namespace
+
+extern "C" {
+
+int atmo_lgl_basis(int n, double* nodes, double* weights, double* D, double* Dsplit) {
+  if (n < 2 || n > 16) return fail(nullptr, ATMO_ERR_INVALID, "n must be in 2..16");
+  Basis b(n);
+  if (nodes) std::memcpy(nodes, b.x.data(), sizeof(double) * n);
+  if (weights) std::memcpy(weights, b.w.data(), sizeof(double) * n);
+  if (D) std::memcpy(D, b.D.data(), sizeof(double) * n * n);
+  if (Dsplit) std::memcpy(Dsplit, b.Dsplit.data(), sizeof(double) * n * n);
+  return ATMO_OK;
+}
+
+int atmo_mesh3d_cartesian(int polydeg, const int32_t trees[3], const double xmin[3],
+                          const double xmax[3], const int32_t periodic[3], atmo_mesh3d** out) {
+  if (!out || !trees || !xmin || !xmax || !periodic)
+    return fail(nullptr, ATMO_ERR_INVALID, "null argument");
+  if (polydeg < 1 || polydeg > 15) return fail(nullptr, ATMO_ERR_INVALID, "polydeg out of range");
+  for (int c = 0; c < 3; ++c)
+    if (trees[c] < 1 || !(xmax[c] > xmin[c]))
+      return fail(nullptr, ATMO_ERR_INVALID, "bad brick dimensions");
+  auto* mh = new atmo_mesh3d;
+  mh->m.polydeg = polydeg;
+  mh->m.n = polydeg + 1;
+  Basis b(polydeg + 1);
+  const int t[3] = {trees[0], trees[1], trees[2]};
+  const int p[3] = {periodic[0], periodic[1], periodic[2]};
+  build_cartesian(mh->m, b, t, xmin, xmax, p);
+  compute_metric_terms(mh->m, b);
+  std::string err;
+  if (!build_connectivity(mh->m, &err)) {
+    delete mh;
+    return fail(nullptr, ATMO_ERR_MESH, err);
+  }
+  *out = mh;
+  return ATMO_OK;
+}
+
+int atmo_mesh3d_cubed_sphere_shell(int polydeg, int trees_per_face_dimension, int layers,
+                                   double inner_radius, double thickness, atmo_mesh3d** out) {
+  if (!out) return fail(nullptr, ATMO_ERR_INVALID, "null argument");
+  if (polydeg < 1 || polydeg > 15 || trees_per_face_dimension < 1 || layers < 1 ||
+      !(inner_radius > 0) || !(thickness > 0))
+    return fail(nullptr, ATMO_ERR_INVALID, "bad shell parameters");
+  auto* mh = new atmo_mesh3d;
+  mh->m.polydeg = polydeg;
+  mh->m.n = polydeg + 1;
+  Basis b(polydeg + 1);
+  build_cubed_sphere_shell(mh->m, b, trees_per_face_dimension, layers, inner_radius, thickness);
+  compute_metric_terms(mh->m, b);
+  std::string err;
+  if (!build_connectivity(mh->m, &err)) {
+    delete mh;
+    return fail(nullptr, ATMO_ERR_MESH, err);
+  }
+  *out = mh;
+  return ATMO_OK;
+}
+
+int atmo_mesh3d_sizes(const atmo_mesh3d* mh, int32_t* ne, int32_t* ni, int32_t* nb) {
+  if (!mh) return fail(nullptr, ATMO_ERR_INVALID, "null mesh");
+  if (ne) *ne = mh->m.n_elements;
+  if (ni) *ni = mh->m.n_interfaces;
+  if (nb) *nb = mh->m.n_boundaries;
+  return ATMO_OK;
+}
+
+int atmo_mesh3d_arrays(const atmo_mesh3d* mh, const double** xyz, const double** cv,
+                       const double** invJ, const int32_t** iids, const int32_t** iidx,
+                       const int32_t** be, const int32_t** bidx, const int32_t** bk) {
+  if (!mh) return fail(nullptr, ATMO_ERR_INVALID, "null mesh");
+  if (xyz) *xyz = mh->m.node_coordinates.data();
+  if (cv) *cv = mh->m.contravariant.data();
+  if (invJ) *invJ = mh->m.inverse_jacobian.data();
+  if (iids) *iids = mh->m.interface_neighbor_ids.data();
+  if (iidx) *iidx = mh->m.interface_node_indices.data();
+  if (be) *be = mh->m.boundary_element.data();
+  if (bidx) *bidx = mh->m.boundary_node_indices.data();
+  if (bk) *bk = mh->m.boundary_kind.data();
+  return ATMO_OK;
+}
+
+void atmo_mesh3d_destroy(atmo_mesh3d* mh) { delete mh; }
+
+const char* atmo_last_error(const atmo_handle* h) {
+  return h ? h->err.c_str() : g_error.c_str();
+}
+
+int atmo_create(const atmo_config* cfg, const atmo_geometry3d* g, atmo_handle** out) {
+  if (!cfg || !g || !out) return fail(nullptr, ATMO_ERR_INVALID, "null argument");
+  if (cfg->equations != ATMO_EQ_EULER_ENERGY_GRAVITY_3D)
+    return fail(nullptr, ATMO_ERR_INVALID, "unsupported equations");
+  if (cfg->polydeg < 1 || cfg->polydeg > 7)
+    return fail(nullptr, ATMO_ERR_INVALID, "polydeg must be in 1..7");
+  if (cfg->n_elements < 1) return fail(nullptr, ATMO_ERR_INVALID, "n_elements must be positive");
+  if (cfg->volume_flux < 0 || cfg->volume_flux > 2)
+    return fail(nullptr, ATMO_ERR_INVALID, "unsupported volume flux");
+  if (cfg->volume_noncons < 0 || cfg->volume_noncons > 2 || cfg->surface_noncons < 0 ||
+      cfg->surface_noncons > 2)
+    return fail(nullptr, ATMO_ERR_INVALID, "unsupported nonconservative flux");
+  if (cfg->surface_flux < 0 || cfg->surface_flux > 3)
+    return fail(nullptr, ATMO_ERR_INVALID, "unsupported surface flux");
+  if (cfg->precision != ATMO_PREC_FP64)
+    return fail(nullptr, ATMO_ERR_INVALID, "unsupported precision");
+  if (!(cfg->c_p > cfg->c_v) || !(cfg->c_v > 0))
+    return fail(nullptr, ATMO_ERR_INVALID, "need c_p > c_v > 0");
+  if (cfg->surface_flux == ATMO_SURF_LMARS && !(cfg->lmars_c > 0))
+    return fail(nullptr, ATMO_ERR_INVALID, "FluxLMARS needs a positive speed of sound");
+  if (cfg->source_terms != ATMO_SRC_NONE && cfg->source_terms != ATMO_SRC_CORIOLIS)
+    return fail(nullptr, ATMO_ERR_INVALID, "unsupported source terms");
+  if (!g->contravariant_vectors || !g->inverse_jacobian)
+    return fail(nullptr, ATMO_ERR_INVALID, "missing geometry arrays");
+  if (g->n_interfaces < 0 || g->n_boundaries < 0 ||
+      (g->n_interfaces > 0 && (!g->interface_neighbor_ids || !g->interface_node_indices)) ||
+      (g->n_boundaries > 0 &&
+       (!g->boundary_element || !g->boundary_node_indices || !g->boundary_condition)))
+    return fail(nullptr, ATMO_ERR_INVALID, "missing connectivity arrays");
+
+  const int n = cfg->polydeg + 1, n2 = n * n, n3 = n2 * n, E = cfg->n_elements;
+  // face-node gather map: neighbour global node, or -1 for a slip wall
+  std::vector<int32_t> nbr((size_t)E * 6 * n2, INT32_MIN);
+  std::vector<int32_t> face_info((size_t)E * 6, 0);
+  auto set_link = [&](int e, int dir, int a, int b, int32_t v) -> bool {
+    int32_t& slot = nbr[((size_t)e * 6 + dir) * n2 + a + n * b];
+    if (slot != INT32_MIN) return false;
+    slot = v;
+    return true;
+  };
+  for (int I = 0; I < g->n_interfaces; ++I) {
+    const int ep = g->interface_neighbor_ids[2 * I], es = g->interface_neighbor_ids[2 * I + 1];
+    const int32_t* ip = g->interface_node_indices + 6 * I;
+    const int32_t* is = ip + 3;
+    const int dp = indices2direction(ip), ds = indices2direction(is);
+    if (ep < 0 || ep >= E || es < 0 || es >= E || dp < 0 || ds < 0)
+      return fail(nullptr, ATMO_ERR_INVALID, "malformed interface " + std::to_string(I));
+    face_info[(size_t)es * 6 + ds] = 8 | dp;
+    for (int bb = 0; bb < n; ++bb)
+      for (int aa = 0; aa < n; ++aa) {
+        const int pi = code_index(ip[0], aa, bb, n), pj = code_index(ip[1], aa, bb, n),
+                  pk = code_index(ip[2], aa, bb, n);
+        const int si = code_index(is[0], aa, bb, n), sj = code_index(is[1], aa, bb, n),
+                  sk = code_index(is[2], aa, bb, n);
+        int fa, fb;
+        face_ab(dp, pi, pj, pk, &fa, &fb);
+        const int32_t sglob = es * n3 + si + n * (sj + n * sk);
+        const int32_t pglob = ep * n3 + pi + n * (pj + n * pk);
+        if (!set_link(ep, dp, fa, fb, sglob))
+          return fail(nullptr, ATMO_ERR_INVALID, "face used twice (interface " + std::to_string(I) + ")");
+        face_ab(ds, si, sj, sk, &fa, &fb);
+        if (!set_link(es, ds, fa, fb, pglob))
+          return fail(nullptr, ATMO_ERR_INVALID, "face used twice (interface " + std::to_string(I) + ")");
+      }
+  }
+  for (int B = 0; B < g->n_boundaries; ++B) {
+    const int e = g->boundary_element[B];
+    const int32_t* ib = g->boundary_node_indices + 3 * B;
+    const int dir = indices2direction(ib);
+    if (e < 0 || e >= E || dir < 0 || g->boundary_condition[B] != ATMO_BC_SLIP_WALL)
+      return fail(nullptr, ATMO_ERR_INVALID, "malformed boundary " + std::to_string(B));
+    for (int bb = 0; bb < n; ++bb)
+      for (int aa = 0; aa < n; ++aa) {
+        int fa, fb;
+        face_ab(dir, code_index(ib[0], aa, bb, n), code_index(ib[1], aa, bb, n),
+                code_index(ib[2], aa, bb, n), &fa, &fb);
+        if (!set_link(e, dir, fa, fb, -1))
+          return fail(nullptr, ATMO_ERR_INVALID, "face used twice (boundary " + std::to_string(B) + ")");
+      }
+  }
+  for (auto v : nbr)
+    if (v == INT32_MIN) return fail(nullptr, ATMO_ERR_INVALID, "element face without interface or boundary");
+
+  auto* h = new atmo_handle;
+  h->cfg = *cfg;
+  h->n = n;
+  h->n3 = n3;
+  h->len = (int64_t)6 * n3 * E;
+  auto cleanup_fail = [&](int code) {
+    atmo_destroy(h);
+    return code;
+  };
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    fail(nullptr, ATMO_ERR_DEVICE, "no HIP device");
+    return cleanup_fail(ATMO_ERR_DEVICE);
+  }
+  if (ensure_tables(h->device) != hipSuccess) {
+    fail(nullptr, ATMO_ERR_DEVICE, "uploading basis tables failed");
+    return cleanup_fail(ATMO_ERR_DEVICE);
+  }
+  const size_t sb = sizeof(double) * h->len;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&h->d_u, sb) != hipSuccess || hipMalloc(&h->d_u2, sb) != hipSuccess ||
+      hipMalloc(&h->d_k, sb) != hipSuccess || hipMalloc(&h->d_du, sb) != hipSuccess ||
+      hipMalloc(&h->d_uio, sb) != hipSuccess || hipMalloc(&h->d_duio, sb) != hipSuccess ||
+      hipMalloc(&h->d_cv, sizeof(double) * 9 * n3 * (size_t)E) != hipSuccess ||
+      hipMalloc(&h->d_invJ, sizeof(double) * n3 * (size_t)E) != hipSuccess ||
+      hipMalloc(&h->d_nbr, sizeof(int32_t) * nbr.size()) != hipSuccess ||
+      hipMalloc(&h->d_face_info, sizeof(int32_t) * face_info.size()) != hipSuccess ||
+      hipMalloc(&h->d_partial, sizeof(double) * 9 * (size_t)E) != hipSuccess ||
+      hipMalloc(&h->d_result, sizeof(double) * 16) != hipSuccess) {
+    fail(nullptr, ATMO_ERR_DEVICE, "device allocation failed");
+    return cleanup_fail(ATMO_ERR_DEVICE);
+  }
+  if (hipMemcpy(h->d_cv, g->contravariant_vectors, sizeof(double) * 9 * n3 * (size_t)E,
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->d_invJ, g->inverse_jacobian, sizeof(double) * n3 * (size_t)E,
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->d_nbr, nbr.data(), sizeof(int32_t) * nbr.size(), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(h->d_face_info, face_info.data(), sizeof(int32_t) * face_info.size(),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(h->d_u, 0, sb) != hipSuccess || hipMemset(h->d_k, 0, sb) != hipSuccess) {
+    fail(nullptr, ATMO_ERR_DEVICE, "device upload failed");
+    return cleanup_fail(ATMO_ERR_DEVICE);
+  }
+  *out = h;
+  return ATMO_OK;
+}
+
+void atmo_destroy(atmo_handle* h) {
+  if (!h) return;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (auto& tp : h->timers) {
+    hipEventDestroy(tp.a);
+    hipEventDestroy(tp.b);
+  }
+  double* bufs[] = {h->d_u, h->d_u2, h->d_k, h->d_du, h->d_uio, h->d_duio,
+                    h->d_cv, h->d_invJ, h->d_partial, h->d_result};
+  for (double* p : bufs)
+    if (p) hipFree(p);
+  if (h->d_nbr) hipFree(h->d_nbr);
+  if (h->d_face_info) hipFree(h->d_face_info);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int64_t atmo_state_length(const atmo_handle* h) { return h ? h->len : -1; }
+
+int atmo_rhs(atmo_handle* h, double* du, const double* u, double t) {
+  (void)t;
+  if (!h || !du || !u) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
+  const size_t sb = sizeof(double) * h->len;
+  HIP_TRY(h, hipMemcpyAsync(h->d_uio, u, sb, hipMemcpyHostToDevice, h->stream));
+  int rc = run_rhs(h, h->d_duio, h->d_uio);
+  if (rc != ATMO_OK) return rc;
+  HIP_TRY(h, hipMemcpyAsync(du, h->d_duio, sb, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return ATMO_OK;
+}
+
+int atmo_rhs_device(atmo_handle* h, double* du, const double* u, double t) {
+  (void)t;
+  if (!h || !du || !u) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
+  return run_rhs(h, du, u);
+}
+
+int atmo_set_state(atmo_handle* h, const double* u) {
+  if (!h || !u) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
+  HIP_TRY(h, hipMemcpyAsync(h->d_u, u, sizeof(double) * h->len, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(h, hipMemsetAsync(h->d_k, 0, sizeof(double) * h->len, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->du_valid = false;
+  return ATMO_OK;
+}
+
+int atmo_get_state(atmo_handle* h, double* u) {
+  if (!h || !u) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
+  HIP_TRY(h, hipMemcpyAsync(u, h->d_u, sizeof(double) * h->len, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return ATMO_OK;
+}
+
+int atmo_step(atmo_handle* h, double dt, int32_t n_steps) {
+  if (!h || n_steps < 0 || !(dt > 0) || !std::isfinite(dt))
+    return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "bad step arguments");
+  // Carpenter & Kennedy (1994) five-stage fourth-order 2N coefficients
+  static const double A[5] = {0.0, -567301805773.0 / 1357537059087.0,
+                              -2404267990393.0 / 2016746695238.0,
+                              -3550918686646.0 / 2091501179385.0,
+                              -1275806237668.0 / 842570457699.0};
+  static const double B[5] = {1432997174477.0 / 9575080441755.0,
+                              5161836677717.0 / 13612068292357.0,
+                              1720146321549.0 / 2090206949498.0,
+                              3134564353537.0 / 4481467310338.0,
+                              2277821191437.0 / 14882151754819.0};
+  E3Launch L = base_launch(h);
+  L.mode = 1;
+  L.k = h->d_k;
+  L.rk_dt = dt;
+  for (int s = 0; s < n_steps; ++s)
+    for (int st = 0; st < 5; ++st) {
+      L.u = h->d_u;
+      L.out = h->d_u2;
+      L.rk_a = A[st];
+      L.rk_b = B[st];
+      int rc = launch_stage(h, L);
+      if (rc != ATMO_OK) return rc;
+      std::swap(h->d_u, h->d_u2);
+    }
+  h->du_valid = false;
+  return ATMO_OK;
+}
+
+int atmo_max_dt(atmo_handle* h, double cfl, double* dt) {
+  if (!h || !dt) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
+  E3Launch L = base_launch(h);
+  L.u = h->d_u;
+  TimerPair tp;
+  if (h->timing) timer_begin(h, 1, &tp);
+  e3_launch_reduction(L, 0, nullptr, h->d_partial, h->d_result, h->stream);
+  if (h->timing) timer_end(h, &tp);
+  HIP_TRY(h, hipGetLastError());
+  double mx = 0;
+  HIP_TRY(h, hipMemcpyAsync(&mx, h->d_result, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  if (!std::isfinite(mx)) return fail(&h->err, ATMO_ERR_NONFINITE, "non-finite wave speed in state");
+  if (mx < 5e-324) mx = 5e-324;
+  *dt = cfl * 2.0 / (h->n * mx);
+  return ATMO_OK;
+}
+
+int atmo_integrals(atmo_handle* h, int32_t fresh_rhs, double* out) {
+  if (!h || !out) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
+  if (fresh_rhs) {
+    int rc = run_rhs(h, h->d_du, h->d_u);
+    if (rc != ATMO_OK) return rc;
+    h->du_valid = true;
+  }
+  E3Launch L = base_launch(h);
+  L.u = h->d_u;
+  TimerPair tp;
+  if (h->timing) timer_begin(h, 1, &tp);
+  e3_launch_reduction(L, 1, h->du_valid ? h->d_du : nullptr, h->d_partial, h->d_result, h->stream);
+  if (h->timing) timer_end(h, &tp);
+  HIP_TRY(h, hipGetLastError());
+  HIP_TRY(h, hipMemcpyAsync(out, h->d_result, sizeof(double) * 9, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return ATMO_OK;
+}
+
+int atmo_synchronize(atmo_handle* h) {
+  if (!h) return fail(nullptr, ATMO_ERR_INVALID, "null handle");
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return ATMO_OK;
+}
+
+int atmo_enable_timing(atmo_handle* h, int32_t enable) {
+  if (!h) return fail(nullptr, ATMO_ERR_INVALID, "null handle");
+  collect_timers(h);
+  h->timing = enable != 0;
+  h->launches[0] = h->launches[1] = 0;
+  h->total_ms[0] = h->total_ms[1] = 0;
+  return ATMO_OK;
+}
+
+int atmo_kernel_times(atmo_handle* h, int32_t cls, int64_t* launches, double* total_ms) {
+  if (!h || cls < 0 || cls > 1) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "bad argument");
+  collect_timers(h);
+  if (launches) *launches = h->launches[cls];
+  if (total_ms) *total_ms = h->total_ms[cls];
+  return ATMO_OK;
+}
+
+}  // extern "C"

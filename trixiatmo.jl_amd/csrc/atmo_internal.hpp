@@ -1,0 +1,48 @@
+// Internal (non-ABI) declarations shared by the host-side C++ of libatmo_hip.
+#ifndef ATMO_INTERNAL_HPP
+#define ATMO_INTERNAL_HPP
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace atmo {
+
+// Lobatto-Legendre basis: nodes, weights, derivative matrix D[i*n+j] and the
+// flux-differencing matrix Dsplit = 2D + diag(1/w_1, 0, ..., -1/w_n).
+struct Basis {
+  int n;
+  std::vector<double> x, w, D, Dsplit, bary;
+  explicit Basis(int n);
+};
+
+// Trixi-style node-index codes of P4estInterfaceContainer / BoundaryContainer
+enum NodeCode : int32_t {
+  kBegin = 0, kEnd = 1, kIFwd = 2, kIBwd = 3, kJFwd = 4, kJBwd = 5
+};
+
+// A curved hexahedral mesh with one element per p4est tree (level 0), its
+// geometry and its Trixi-style face connectivity.
+struct Mesh3D {
+  int polydeg = 0, n = 0, n_elements = 0;
+  std::vector<double> node_coordinates;  // [3, n, n, n, E]
+  std::vector<double> jacobian_matrix;   // [3, 3, n, n, n, E]  (dx_c / dxi_d)
+  std::vector<double> contravariant;     // [3, 3, n, n, n, E]  (Ja^d)_c
+  std::vector<double> inverse_jacobian;  // [n, n, n, E]
+  int n_interfaces = 0, n_boundaries = 0;
+  std::vector<int32_t> interface_neighbor_ids;  // [2, I]
+  std::vector<int32_t> interface_node_indices;  // [3, 2, I]
+  std::vector<int32_t> boundary_element;        // [B]
+  std::vector<int32_t> boundary_node_indices;   // [3, B]
+  std::vector<int32_t> boundary_kind;           // [B]  0 inside, 1 outside, ...
+  double period[3] = {0, 0, 0};                 // > 0: periodic length
+};
+
+// node coordinates -> Jacobian matrix, invariant-curl metric terms, 1/J
+void compute_metric_terms(Mesh3D& m, const Basis& b);
+// face matching by node coordinates -> interfaces + boundaries
+bool build_connectivity(Mesh3D& m, std::string* err);
+
+}  // namespace atmo
+
+#endif

@@ -1,0 +1,187 @@
+// Pointwise physics of CompressibleEulerEnergyEquationsWithGravity3D
+// (src/equations/compressible_euler_energy_with_gravity_3d.jl) as device
+// functions.  State order (rho, rho_v1, rho_v2, rho_v3, rho_e, phi); rho_e is
+// total energy without the geopotential, p = (gamma-1)(rho_e - rho |v|^2 / 2).
+#ifndef ATMO_EULER3D_PHYSICS_HPP
+#define ATMO_EULER3D_PHYSICS_HPP
+
+#include <hip/hip_runtime.h>
+
+namespace atmo {
+
+struct E3Params {
+  double gamma, gm1, inv_gm1, lmars_c;
+};
+
+// Logarithmic mean and its inverse with the f^2 < 1e-4 series branch of
+// Ranocha et al. (2023, ACM TOMS), the form of Trixi's ln_mean/inv_ln_mean
+// imported at src/TrixiAtmo.jl:37.
+template <typename T>
+__device__ __forceinline__ T ln_mean(T x, T y) {
+  const T f2 = (x * (x - T(2) * y) + y * y) / (x * (x + T(2) * y) + y * y);
+  if (f2 < T(1.0e-4)) {
+    return (x + y) / (T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0))));
+  }
+  return (y - x) / log(y / x);
+}
+
+template <typename T>
+__device__ __forceinline__ T inv_ln_mean(T x, T y) {
+  const T f2 = (x * (x - T(2) * y) + y * y) / (x * (x + T(2) * y) + y * y);
+  if (f2 < T(1.0e-4)) {
+    return (T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0)))) / (x + y);
+  }
+  return log(y / x) / (y - x);
+}
+
+// primitive quantities a two-point flux needs from one node
+template <typename T>
+struct Prim {
+  T rho, v1, v2, v3, p, rho_e, phi;
+};
+
+template <typename T>
+__device__ __forceinline__ Prim<T> cons2prim(const E3Params& q, const T* u) {
+  Prim<T> w;
+  w.rho = u[0];
+  const T inv = T(1) / w.rho;
+  w.v1 = u[1] * inv;
+  w.v2 = u[2] * inv;
+  w.v3 = u[3] * inv;
+  w.p = T(q.gm1) * (u[4] - T(0.5) * (u[1] * w.v1 + u[2] * w.v2 + u[3] * w.v3));
+  w.rho_e = u[4];
+  w.phi = u[5];
+  return w;
+}
+
+// flux_ranocha, :382-415
+template <typename T>
+__device__ __forceinline__ void flux_ranocha(const E3Params& q, const Prim<T>& l, const Prim<T>& r,
+                                             const T n0, const T n1, const T n2, T f[5]) {
+  const T vnl = l.v1 * n0 + l.v2 * n1 + l.v3 * n2;
+  const T vnr = r.v1 * n0 + r.v2 * n1 + r.v3 * n2;
+  const T rho_mean = ln_mean(l.rho, r.rho);
+  const T inv_rho_p_mean = l.p * r.p * inv_ln_mean(l.rho * r.p, r.rho * l.p);
+  const T v1a = T(0.5) * (l.v1 + r.v1), v2a = T(0.5) * (l.v2 + r.v2), v3a = T(0.5) * (l.v3 + r.v3);
+  const T pa = T(0.5) * (l.p + r.p);
+  const T vsq = T(0.5) * (l.v1 * r.v1 + l.v2 * r.v2 + l.v3 * r.v3);
+  const T f1 = rho_mean * T(0.5) * (vnl + vnr);
+  f[0] = f1;
+  f[1] = f1 * v1a + pa * n0;
+  f[2] = f1 * v2a + pa * n1;
+  f[3] = f1 * v3a + pa * n2;
+  f[4] = f1 * (vsq + inv_rho_p_mean * T(q.inv_gm1)) + T(0.5) * (l.p * vnr + r.p * vnl);
+}
+
+// flux_kennedy_gruber, :289-318
+template <typename T>
+__device__ __forceinline__ void flux_kennedy_gruber(const Prim<T>& l, const Prim<T>& r, const T n0,
+                                                    const T n1, const T n2, T f[5]) {
+  const T ra = T(0.5) * (l.rho + r.rho);
+  const T v1a = T(0.5) * (l.v1 + r.v1), v2a = T(0.5) * (l.v2 + r.v2), v3a = T(0.5) * (l.v3 + r.v3);
+  const T pa = T(0.5) * (l.p + r.p);
+  const T ea = T(0.5) * (l.rho_e / l.rho + r.rho_e / r.rho);
+  const T vna = v1a * n0 + v2a * n1 + v3a * n2;
+  const T f1 = ra * vna;
+  f[0] = f1;
+  f[1] = f1 * v1a + pa * n0;
+  f[2] = f1 * v2a + pa * n1;
+  f[3] = f1 * v3a + pa * n2;
+  f[4] = f1 * ea + pa * vna;
+}
+
+// flux_shima_etal, :248-276
+template <typename T>
+__device__ __forceinline__ void flux_shima(const E3Params& q, const Prim<T>& l, const Prim<T>& r,
+                                           const T n0, const T n1, const T n2, T f[5]) {
+  const T vnl = l.v1 * n0 + l.v2 * n1 + l.v3 * n2;
+  const T vnr = r.v1 * n0 + r.v2 * n1 + r.v3 * n2;
+  const T ra = T(0.5) * (l.rho + r.rho);
+  const T v1a = T(0.5) * (l.v1 + r.v1), v2a = T(0.5) * (l.v2 + r.v2), v3a = T(0.5) * (l.v3 + r.v3);
+  const T vna = T(0.5) * (vnl + vnr);
+  const T pa = T(0.5) * (l.p + r.p);
+  const T vsq = T(0.5) * (l.v1 * r.v1 + l.v2 * r.v2 + l.v3 * r.v3);
+  const T f1 = ra * vna;
+  f[0] = f1;
+  f[1] = f1 * v1a + pa * n0;
+  f[2] = f1 * v2a + pa * n1;
+  f[3] = f1 * v3a + pa * n2;
+  f[4] = f1 * vsq + pa * vna * T(q.inv_gm1) + T(0.5) * (l.p * vnr + r.p * vnl);
+}
+
+// pointwise flux, :77-91.  The reference takes `rho_e = last(u)`, the sixth
+// entry (phi), for the energy flux; kept as the reference has it.
+template <typename T>
+__device__ __forceinline__ void flux_point(const Prim<T>& w, const T n0, const T n1, const T n2,
+                                           T f[5]) {
+  const T vn = w.v1 * n0 + w.v2 * n1 + w.v3 * n2;
+  const T rvn = w.rho * vn;
+  f[0] = rvn;
+  f[1] = rvn * w.v1 + w.p * n0;
+  f[2] = rvn * w.v2 + w.p * n1;
+  f[3] = rvn * w.v3 + w.p * n2;
+  f[4] = (w.phi + w.p) * vn;
+}
+
+template <typename T>
+__device__ __forceinline__ void flux_central(const Prim<T>& l, const Prim<T>& r, const T n0,
+                                             const T n1, const T n2, T f[5]) {
+  T fl[5], fr[5];
+  flux_point(l, n0, n1, n2, fl);
+  flux_point(r, n0, n1, n2, fr);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) f[v] = T(0.5) * (fl[v] + fr[v]);
+}
+
+// max_abs_speed with a normal direction, :538-557
+template <typename T>
+__device__ __forceinline__ T max_abs_speed(const E3Params& q, const Prim<T>& l, const Prim<T>& r,
+                                           const T n0, const T n1, const T n2) {
+  const T vl = l.v1 * n0 + l.v2 * n1 + l.v3 * n2;
+  const T vr = r.v1 * n0 + r.v2 * n1 + r.v3 * n2;
+  const T cl = sqrt(T(q.gamma) * l.p / l.rho);
+  const T cr = sqrt(T(q.gamma) * r.p / r.rho);
+  const T nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+  return fmax(fabs(vl) + cl * nn, fabs(vr) + cr * nn);
+}
+
+// FluxLMARS, :430-466
+template <typename T>
+__device__ __forceinline__ void flux_lmars(const E3Params& q, const T* ul, const T* ur,
+                                           const Prim<T>& l, const Prim<T>& r, const T n0,
+                                           const T n1, const T n2, T f[6]) {
+  const T c = T(q.lmars_c);
+  const T vl = l.v1 * n0 + l.v2 * n1 + l.v3 * n2;
+  const T vr = r.v1 * n0 + r.v2 * n1 + r.v3 * n2;
+  const T nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+  const T rho = T(0.5) * (l.rho + r.rho);
+  const T p = T(0.5) * (l.p + r.p) - T(0.5) * c * rho * (vr - vl) / nn;
+  const T v = T(0.5) * (vl + vr) - T(1) / (T(2) * c * rho) * (r.p - l.p) * nn;
+  const T* us = v >= T(0) ? ul : ur;
+  const T ps = v >= T(0) ? l.p : r.p;
+  f[0] = v * us[0];
+  f[1] = v * us[1] + p * n0;
+  f[2] = v * us[2] + p * n1;
+  f[3] = v * us[3] + p * n2;
+  f[4] = v * us[4] + ps * v;
+  f[5] = T(0);
+}
+
+// flux_nonconservative_waruszewski_etal (:135-159) / souza_etal (:206-230):
+// g = rho_avg [phi] (0, n, n.v_avg, 0); only entries 1..4 are nonzero.
+template <int NC, typename T>
+__device__ __forceinline__ void noncons_gravity(const Prim<T>& l, const Prim<T>& r, const T n0,
+                                                const T n1, const T n2, T g[4]) {
+  const T ra = (NC == 1) ? ln_mean(l.rho, r.rho) : T(0.5) * (l.rho + r.rho);
+  const T jump = r.phi - l.phi;
+  const T s = ra * jump;
+  const T v1a = T(0.5) * (l.v1 + r.v1), v2a = T(0.5) * (l.v2 + r.v2), v3a = T(0.5) * (l.v3 + r.v3);
+  g[0] = n0 * s;
+  g[1] = n1 * s;
+  g[2] = n2 * s;
+  g[3] = n0 * s * v1a + n1 * s * v2a + n2 * s * v3a;
+}
+
+}  // namespace atmo
+
+#endif

@@ -1,0 +1,48 @@
+"""Equation systems and flux names (the Julia function names of TrixiAtmo/Trixi)."""
+from dataclasses import dataclass
+
+# src/equations/reference_data.jl:11-61
+EARTH_RADIUS = 6.37122e6
+EARTH_GRAVITATIONAL_ACCELERATION = 9.80616
+EARTH_ROTATION_RATE = 7.292e-5
+
+# two-point volume fluxes -> ATMO_VOL_*
+VOLUME_FLUXES = {"flux_ranocha": 0, "flux_kennedy_gruber": 1, "flux_shima_etal": 2}
+# nonconservative gravity terms -> ATMO_NC_*
+NONCONSERVATIVE_FLUXES = {"flux_zero": 0, "flux_nonconservative_waruszewski_etal": 1,
+                          "flux_nonconservative_souza_etal": 2}
+# surface fluxes -> ATMO_SURF_*
+SURFACE_FLUXES = {"flux_lax_friedrichs": 0, "flux_ranocha": 1, "flux_kennedy_gruber": 2,
+                  "FluxLMARS": 3}
+
+
+@dataclass(frozen=True)
+class CompressibleEulerEnergyEquationsWithGravity3D:
+    """src/equations/compressible_euler_energy_with_gravity_3d.jl:45-68:
+    variables (rho, rho_v1, rho_v2, rho_v3, rho_e, phi)."""
+    c_p: float
+    c_v: float
+    gravity: float
+    p_0: float = 100_000.0
+
+    @property
+    def gamma(self):
+        return self.c_p / self.c_v
+
+    @property
+    def R(self):
+        return self.c_p - self.c_v
+
+    nvariables = 6
+    varnames = ("rho", "rho_v1", "rho_v2", "rho_v3", "rho_e", "phi")
+
+    def prim2cons(self, rho, v1, v2, v3, p, phi):
+        """:626-636"""
+        e = p / (self.gamma - 1.0) + 0.5 * rho * (v1 * v1 + v2 * v2 + v3 * v3)
+        return rho, rho * v1, rho * v2, rho * v3, e, phi
+
+
+@dataclass(frozen=True)
+class FluxLMARS:
+    """FluxLMARS(c): low-Mach approximate Riemann solver with sound speed c."""
+    speed_of_sound: float

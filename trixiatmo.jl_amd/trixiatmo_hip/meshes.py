@@ -1,0 +1,103 @@
+"""Mesh constructors mirroring TrixiAtmo/Trixi names (P4estMesh{3} family).
+
+Each mesh carries the arrays Trixi keeps in its element, interface and
+boundary containers, as numpy arrays in Julia column-major order (first index
+fastest), so they can be handed to the semidiscretization unchanged.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import check, c_double_p, c_int32_p, lib
+
+# Trixi node-index symbols and their integer codes in the C ABI
+NODE_INDEX_CODES = {"begin": 0, "end": 1, "i_forward": 2, "i_backward": 3,
+                    "j_forward": 4, "j_backward": 5}
+
+
+class P4estMesh3D:
+    """A level-0 P4estMesh{3}: one element per tree, geometry and connectivity.
+
+    Attributes (numpy, Julia layout): node_coordinates [3,n,n,n,E],
+    contravariant_vectors [3,3,n,n,n,E], inverse_jacobian [n,n,n,E],
+    interface_neighbor_ids [2,I], interface_node_indices [3,2,I],
+    boundary_element [B], boundary_node_indices [3,B], boundary_kind [B]
+    (0 = :inside, 1 = :outside, 2 = other).
+    """
+
+    def __init__(self, handle, polydeg):
+        L = lib()
+        self.polydeg = polydeg
+        n = polydeg + 1
+        ne, ni, nb = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(L.atmo_mesh3d_sizes(handle, ctypes.byref(ne), ctypes.byref(ni), ctypes.byref(nb)))
+        self.n_elements, self.n_interfaces, self.n_boundaries = ne.value, ni.value, nb.value
+        ptrs_d = [c_double_p() for _ in range(3)]
+        ptrs_i = [c_int32_p() for _ in range(5)]
+        check(L.atmo_mesh3d_arrays(handle, *[ctypes.byref(p) for p in ptrs_d],
+                                   *[ctypes.byref(p) for p in ptrs_i]))
+        E, I, B = self.n_elements, self.n_interfaces, self.n_boundaries
+
+        def dcopy(p, shape):
+            cnt = int(np.prod(shape))
+            if cnt == 0:
+                return np.zeros(shape, dtype=np.float64, order="F")
+            return np.ctypeslib.as_array(p, shape=(cnt,)).copy().reshape(shape, order="F")
+
+        def icopy(p, shape):
+            cnt = int(np.prod(shape))
+            if cnt == 0:
+                return np.zeros(shape, dtype=np.int32, order="F")
+            return np.ctypeslib.as_array(p, shape=(cnt,)).copy().reshape(shape, order="F")
+
+        self.node_coordinates = dcopy(ptrs_d[0], (3, n, n, n, E))
+        self.contravariant_vectors = dcopy(ptrs_d[1], (3, 3, n, n, n, E))
+        self.inverse_jacobian = dcopy(ptrs_d[2], (n, n, n, E))
+        self.interface_neighbor_ids = icopy(ptrs_i[0], (2, I))
+        self.interface_node_indices = icopy(ptrs_i[1], (3, 2, I))
+        self.boundary_element = icopy(ptrs_i[2], (B,))
+        self.boundary_node_indices = icopy(ptrs_i[3], (3, B))
+        self.boundary_kind = icopy(ptrs_i[4], (B,))
+        L.atmo_mesh3d_destroy(handle)
+
+    @property
+    def nnodes(self):
+        return self.polydeg + 1
+
+
+def P4estMesh(trees_per_dimension, *, polydeg, coordinates_min, coordinates_max,
+              periodicity=(True, True, True)):
+    """Trixi `P4estMesh(trees_per_dimension; polydeg, coordinates_min,
+    coordinates_max, periodicity)` in 3D with initial_refinement_level = 0."""
+    t = np.asarray(trees_per_dimension, dtype=np.int32)
+    lo = np.asarray(coordinates_min, dtype=np.float64)
+    hi = np.asarray(coordinates_max, dtype=np.float64)
+    per = np.asarray([1 if p else 0 for p in periodicity], dtype=np.int32)
+    h = ctypes.c_void_p()
+    check(lib().atmo_mesh3d_cartesian(polydeg, t.ctypes.data_as(c_int32_p),
+                                      lo.ctypes.data_as(c_double_p), hi.ctypes.data_as(c_double_p),
+                                      per.ctypes.data_as(c_int32_p), ctypes.byref(h)))
+    return P4estMesh3D(h, polydeg)
+
+
+def P4estMeshCubedSphereTopography(trees_per_face_dimension, layers, inner_radius, thickness,
+                                   *, polydeg):
+    """src/meshes/p4est_cubed_sphere_topography.jl:114-163 with the perfect
+    sphere topography and initial_refinement_level = 0."""
+    h = ctypes.c_void_p()
+    check(lib().atmo_mesh3d_cubed_sphere_shell(polydeg, trees_per_face_dimension, layers,
+                                               float(inner_radius), float(thickness),
+                                               ctypes.byref(h)))
+    return P4estMesh3D(h, polydeg)
+
+
+def lgl_basis(polydeg):
+    """LobattoLegendreBasis(polydeg): (nodes, weights, D, Dsplit)."""
+    n = polydeg + 1
+    x = np.zeros(n)
+    w = np.zeros(n)
+    D = np.zeros(n * n)
+    Ds = np.zeros(n * n)
+    check(lib().atmo_lgl_basis(n, x.ctypes.data_as(c_double_p), w.ctypes.data_as(c_double_p),
+                               D.ctypes.data_as(c_double_p), Ds.ctypes.data_as(c_double_p)))
+    return x, w, D.reshape(n, n), Ds.reshape(n, n)

@@ -1,0 +1,37 @@
+"""DGSEM solver description (Trixi `DGSEM(polydeg, surface_flux, volume_integral)`)."""
+from dataclasses import dataclass
+from typing import Tuple, Union
+
+from .equations import (FluxLMARS, NONCONSERVATIVE_FLUXES, SURFACE_FLUXES, VOLUME_FLUXES)
+
+PRECISIONS = {"fp64": 0, "fp32_kahan": 1, "fp32": 2}
+
+
+@dataclass(frozen=True)
+class VolumeIntegralFluxDifferencing:
+    """`VolumeIntegralFluxDifferencing((symmetric_flux, nonconservative_flux))`"""
+    volume_flux: Tuple[str, str]
+
+
+@dataclass(frozen=True)
+class DGSEM:
+    """`DGSEM(polydeg = p, surface_flux = (flux, noncons), volume_integral = ...)`
+
+    `precision` selects the arithmetic of the flux evaluations and volume sums
+    ("fp64", "fp32_kahan" or "fp32"); RK registers stay FP64 in every form.
+    """
+    polydeg: int
+    surface_flux: Tuple[Union[str, FluxLMARS], str]
+    volume_integral: VolumeIntegralFluxDifferencing
+    precision: str = "fp64"
+
+    def codes(self):
+        vol, vnc = self.volume_integral.volume_flux
+        sf, snc = self.surface_flux
+        lmars_c = 0.0
+        if isinstance(sf, FluxLMARS):
+            lmars_c = float(sf.speed_of_sound)
+            sf = "FluxLMARS"
+        return dict(volume_flux=VOLUME_FLUXES[vol], volume_noncons=NONCONSERVATIVE_FLUXES[vnc],
+                    surface_flux=SURFACE_FLUXES[sf], surface_noncons=NONCONSERVATIVE_FLUXES[snc],
+                    precision=PRECISIONS[self.precision], lmars_c=lmars_c)
