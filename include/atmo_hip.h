@@ -48,6 +48,8 @@ extern "C" {
 #define ATMO_SURF_RANOCHA 1        /* flux_ranocha (entropy conservative) */
 #define ATMO_SURF_KENNEDY_GRUBER 2 /* flux_kennedy_gruber */
 #define ATMO_SURF_LMARS 3          /* FluxLMARS(c) */
+#define ATMO_SURF_KG_PLUS_LLF 4    /* FluxPlusDissipation(flux_kennedy_gruber, DissipationLocalLaxFriedrichs()) */
+#define ATMO_SURF_RANOCHA_PLUS_LLF 5 /* FluxPlusDissipation(flux_ranocha, DissipationLocalLaxFriedrichs()) */
 
 /* arithmetic of the two-point flux evaluation and volume sums */
 #define ATMO_PREC_FP64 0          /* everything in FP64 */

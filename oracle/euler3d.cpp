@@ -213,7 +213,16 @@ void surface_flux(const Eq& eq, const ora_e3_params* p, const double* ul,
     case ORA_SURF_LAX_FRIEDRICHS: flux_llf(eq, ul, ur, n, f); break;
     case ORA_SURF_RANOCHA: flux_ranocha(eq, ul, ur, n, f); break;
     case ORA_SURF_KENNEDY_GRUBER: flux_kennedy_gruber(eq, ul, ur, n, f); break;
-    default: flux_lmars(eq, p->lmars_c, ul, ur, n, f); break;
+    case ORA_SURF_LMARS: flux_lmars(eq, p->lmars_c, ul, ur, n, f); break;
+    default: {
+      // FluxPlusDissipation(f#, DissipationLocalLaxFriedrichs(max_abs_speed)):
+      // equations.jl:256-290 builds the same for the covariant equations
+      if (p->surface_flux == ORA_SURF_KG_PLUS_LLF) flux_kennedy_gruber(eq, ul, ur, n, f);
+      else flux_ranocha(eq, ul, ur, n, f);
+      const double lam = max_abs_speed(eq, ul, ur, n);
+      for (int v = 0; v < NV; ++v) f[v] += -0.5 * lam * (ur[v] - ul[v]);
+      break;
+    }
   }
 }
 

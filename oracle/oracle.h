@@ -54,7 +54,9 @@ enum {
   ORA_SURF_LAX_FRIEDRICHS = 0,
   ORA_SURF_RANOCHA = 1,
   ORA_SURF_KENNEDY_GRUBER = 2,
-  ORA_SURF_LMARS = 3
+  ORA_SURF_LMARS = 3,
+  ORA_SURF_KG_PLUS_LLF = 4,      /* FluxPlusDissipation(flux_kennedy_gruber, DissipationLocalLaxFriedrichs()) */
+  ORA_SURF_RANOCHA_PLUS_LLF = 5  /* FluxPlusDissipation(flux_ranocha, DissipationLocalLaxFriedrichs()) */
 };
 
 typedef struct {

@@ -66,6 +66,9 @@ def test_rhs_periodic_cube_polydeg(oracle, polydeg):
     ("flux_kennedy_gruber", "flux_lax_friedrichs", "flux_nonconservative_waruszewski_etal"),
     ("flux_ranocha", "flux_ranocha", "flux_nonconservative_waruszewski_etal"),
     ("flux_shima_etal", "flux_kennedy_gruber", "flux_zero"),
+    ("flux_kennedy_gruber", T.FluxPlusDissipation("flux_kennedy_gruber"),
+     "flux_nonconservative_waruszewski_etal"),
+    ("flux_ranocha", T.FluxPlusDissipation("flux_ranocha"), "flux_nonconservative_souza_etal"),
 ])
 def test_rhs_shell(oracle, vol, surf, snc):
     mesh, eq, solver, bcs, u0 = T.workloads.shell(3, 2, polydeg=4, seed=1, volume_flux=vol,

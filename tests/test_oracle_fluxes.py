@@ -2,6 +2,7 @@
 flux, symmetry, Tadmor's entropy-conservation condition and antisymmetry of
 the gravity term (the properties the reference's own test_unit.jl style
 checks)."""
+import ctypes
 from decimal import Decimal, getcontext
 
 import numpy as np
@@ -78,13 +79,13 @@ def test_pointwise_flux_uses_phi_for_energy(oracle):
         assert abs(oracle.flux(P, w, n)[4] - (euler_flux(u, n)[4] - u[4] * (u[1:4] / u[0]) @ n)) < 1e-12
 
 
-@pytest.mark.parametrize("which", [0, 1, 2, 3])
+@pytest.mark.parametrize("which", [0, 1, 2, 3, 4, 5])
 def test_surface_flux_consistency_and_conservation(oracle, which):
     for (ul, n), (ur, _) in zip(states(7), states(8)):
         f_lr = oracle.surface_flux(P, which, ul, ur, n)
         f_rl = oracle.surface_flux(P, which, ur, ul, -n)
         np.testing.assert_allclose(f_lr, -f_rl, rtol=1e-13, atol=1e-12)
-        if which in (1, 2, 3):  # two-point fluxes reduce to the textbook flux
+        if which in (1, 2, 3, 4, 5):  # two-point fluxes reduce to the textbook flux
             np.testing.assert_allclose(oracle.surface_flux(P, which, ul, ul, n)[:5],
                                        euler_flux(ul, n)[:5], rtol=1e-13, atol=1e-12)
         else:  # LLF reduces to the (phi-reading) pointwise flux
@@ -102,3 +103,24 @@ def test_gravity_term_antisymmetric(oracle, which):
         # rho_avg [phi] n in the momentum, rho_avg [phi] n.v_avg in the energy
         ra = oracle.ln_mean(ul[0], ur[0]) if which == 1 else 0.5 * (ul[0] + ur[0])
         np.testing.assert_allclose(g1[1:4], ra * (ur[5] - ul[5]) * n, rtol=1e-13)
+
+
+@pytest.mark.parametrize("which,base", [(4, 2), (5, 1)])
+def test_flux_plus_dissipation(oracle, which, base):
+    """FluxPlusDissipation(f#, LLF) = f# - lambda_max/2 [u] with the
+    max_abs_speed of the normal direction"""
+    for (ul, n), (ur, _) in zip(states(11), states(12)):
+        lam = oracle.L.ora_e3_max_abs_speed(ctypes.byref(P), ul.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                            ur.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                            np.asarray(n, float).ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        rho_l, rho_r = ul[0], ur[0]
+        vl, vr = ul[1:4] / rho_l, ur[1:4] / rho_r
+        pl = (GAMMA - 1) * (ul[4] - 0.5 * rho_l * vl @ vl)
+        pr = (GAMMA - 1) * (ur[4] - 0.5 * rho_r * vr @ vr)
+        nn = np.linalg.norm(n)
+        lam_ref = max(abs(vl @ n) + np.sqrt(GAMMA * pl / rho_l) * nn,
+                      abs(vr @ n) + np.sqrt(GAMMA * pr / rho_r) * nn)
+        assert abs(lam / lam_ref - 1) < 1e-14
+        np.testing.assert_allclose(oracle.surface_flux(P, which, ul, ur, n),
+                                   oracle.surface_flux(P, base, ul, ur, n) - 0.5 * lam_ref * (ur - ul),
+                                   rtol=1e-13, atol=1e-12)

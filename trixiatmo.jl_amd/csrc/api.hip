@@ -286,7 +286,7 @@ int atmo_create(const atmo_config* cfg, const atmo_geometry3d* g, atmo_handle** 
   if (cfg->volume_noncons < 0 || cfg->volume_noncons > 2 || cfg->surface_noncons < 0 ||
       cfg->surface_noncons > 2)
     return fail(nullptr, ATMO_ERR_INVALID, "unsupported nonconservative flux");
-  if (cfg->surface_flux < 0 || cfg->surface_flux > 3)
+  if (cfg->surface_flux < 0 || cfg->surface_flux > 5)
     return fail(nullptr, ATMO_ERR_INVALID, "unsupported surface flux");
   if (cfg->precision != ATMO_PREC_FP64)
     return fail(nullptr, ATMO_ERR_INVALID, "unsupported precision");

@@ -49,8 +49,16 @@ __device__ __forceinline__ void surface_flux(const E3Params& q, int surf, const 
   } else if (surf == 2) {
     flux_kennedy_gruber(l, r, n0, n1, n2, f);
     f[5] = T(0);
-  } else {
+  } else if (surf == 3) {
     flux_lmars(q, ul, ur, l, r, n0, n1, n2, f);
+  } else {
+    // FluxPlusDissipation(f#, DissipationLocalLaxFriedrichs(max_abs_speed))
+    if (surf == 4) flux_kennedy_gruber(l, r, n0, n1, n2, f);
+    else flux_ranocha(q, l, r, n0, n1, n2, f);
+    const T lam = max_abs_speed(q, l, r, n0, n1, n2);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) f[v] += T(-0.5) * lam * (ur[v] - ul[v]);
+    f[5] = T(-0.5) * lam * (ur[5] - ul[5]);
   }
 }
 

@@ -5,7 +5,8 @@ hand-written gfx950 HIP kernels of libatmo_hip.so (C ABI: include/atmo_hip.h).
 """
 from ._lib import AtmoError, lib
 from .equations import (EARTH_GRAVITATIONAL_ACCELERATION, EARTH_RADIUS, EARTH_ROTATION_RATE,
-                        CompressibleEulerEnergyEquationsWithGravity3D, FluxLMARS)
+                        CompressibleEulerEnergyEquationsWithGravity3D, FluxLMARS,
+                        FluxPlusDissipation)
 from .initial_conditions import (initial_condition_baroclinic_instability,
                                  initial_condition_isothermal_perturbed,
                                  initial_condition_periodic_modes)
@@ -16,7 +17,7 @@ from . import analysis, workloads
 
 __all__ = [
     "AtmoError", "lib", "EARTH_GRAVITATIONAL_ACCELERATION", "EARTH_RADIUS", "EARTH_ROTATION_RATE",
-    "CompressibleEulerEnergyEquationsWithGravity3D", "FluxLMARS",
+    "CompressibleEulerEnergyEquationsWithGravity3D", "FluxLMARS", "FluxPlusDissipation",
     "initial_condition_baroclinic_instability", "initial_condition_isothermal_perturbed", "initial_condition_periodic_modes",
     "P4estMesh", "P4estMeshCubedSphereTopography", "lgl_basis", "SemidiscretizationHyperbolic",
     "DGSEM", "VolumeIntegralFluxDifferencing", "analysis", "workloads",

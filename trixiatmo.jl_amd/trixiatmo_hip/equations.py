@@ -13,7 +13,19 @@ NONCONSERVATIVE_FLUXES = {"flux_zero": 0, "flux_nonconservative_waruszewski_etal
                           "flux_nonconservative_souza_etal": 2}
 # surface fluxes -> ATMO_SURF_*
 SURFACE_FLUXES = {"flux_lax_friedrichs": 0, "flux_ranocha": 1, "flux_kennedy_gruber": 2,
-                  "FluxLMARS": 3}
+                  "FluxLMARS": 3,
+                  "FluxPlusDissipation(flux_kennedy_gruber, DissipationLocalLaxFriedrichs())": 4,
+                  "FluxPlusDissipation(flux_ranocha, DissipationLocalLaxFriedrichs())": 5}
+
+
+def FluxPlusDissipation(numerical_flux, dissipation="DissipationLocalLaxFriedrichs()"):
+    """`FluxPlusDissipation(flux, DissipationLocalLaxFriedrichs())`: a two-point
+    flux plus local Lax-Friedrichs dissipation -lambda_max/2 [u] (the
+    construction of src/equations/equations.jl:256-290)."""
+    name = f"FluxPlusDissipation({numerical_flux}, {dissipation})"
+    if name not in SURFACE_FLUXES:
+        raise ValueError(f"unsupported surface flux {name}")
+    return name
 
 
 @dataclass(frozen=True)

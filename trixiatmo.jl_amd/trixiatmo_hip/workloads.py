@@ -21,12 +21,22 @@ def periodic_cube(trees=4, polydeg=3, seed=0, precision="fp64"):
     return mesh, eq, solver, {}, u0
 
 
+SHELL_SURFACE_FLUX = "FluxPlusDissipation(flux_kennedy_gruber, DissipationLocalLaxFriedrichs())"
+
+
 def shell(trees_per_face=16, layers=8, polydeg=4, seed=0, precision="fp64",
-          volume_flux="flux_kennedy_gruber", surface_flux="flux_lax_friedrichs",
+          volume_flux="flux_kennedy_gruber", surface_flux=SHELL_SURFACE_FLUX,
           surface_noncons="flux_nonconservative_waruszewski_etal"):
     """Dry Euler (energy form) with gravity on the cubed-sphere shell of radius
     6.371e6 m and 30 km height, isothermal 250 K background plus 10 seeded
-    1 K bumps of 500 km radius, slip walls inside and outside."""
+    1 K bumps of 500 km radius, slip walls inside and outside.
+
+    The local Lax-Friedrichs surface flux is built as
+    FluxPlusDissipation(flux_kennedy_gruber, DissipationLocalLaxFriedrichs()):
+    the reference's `flux_lax_friedrichs` averages the pointwise `flux`, whose
+    energy entry reads phi instead of rho_e
+    (src/equations/compressible_euler_energy_with_gravity_3d.jl:80), and with
+    phi ~ 3e5 J/kg at the model top that run diverges within 10 steps."""
     eq = CompressibleEulerEnergyEquationsWithGravity3D(c_p=1004.0, c_v=717.0, gravity=9.81)
     mesh = P4estMeshCubedSphereTopography(trees_per_face, layers, SHELL_RADIUS, 30000.0,
                                           polydeg=polydeg)
