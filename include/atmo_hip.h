@@ -132,6 +132,9 @@ typedef struct {
   double lmars_c;           /* FluxLMARS speed of sound */
   int32_t source_terms;     /* ATMO_SRC_* */
   double rotation_rate;     /* Omega of source_terms_coriolis */
+  /* Mixed-precision paths only: hydrostatic background split off in FP64,
+   * p_bg = background_p0 exp(-phi / (R background_T0)); <= 0 disables. */
+  double background_p0, background_T0;
 } atmo_config;
 
 typedef struct {

@@ -152,3 +152,44 @@ def test_full_shell_properties():
     mass0 = (W * u0[0]).sum()
     mass1 = (W * u.reshape(u0.shape, order="F")[0]).sum()
     assert abs(mass1 / mass0 - 1) < 1e-12
+
+
+MIXED_RTOL = 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp32_kahan", "fp32"])
+@pytest.mark.parametrize("case", ["cube", "shell_flow", "shell_rest", "shell7_ec"])
+def test_mixed_precision_rhs_error(oracle, precision, case):
+    """Mixed path (FP64 state, pressure recovery, geopotential jumps,
+    hydrostatic-background split and RK registers; FP32 fluxes, geometry and
+    volume sums) against the FP64 oracle: max |du_mixed - du| / max |du| <= 1e-5
+    per conserved variable.  Variables whose tendency is round-off only (mass
+    and energy of a state at rest: max |du| below 1e-10 max |u|) must agree to
+    1e-10 max |u| instead."""
+    if case == "cube":
+        mesh, eq, solver, bcs, u0 = T.workloads.periodic_cube(trees=3, polydeg=3, seed=4,
+                                                              precision=precision)
+    elif case.startswith("shell7"):
+        mesh, eq, solver, bcs, u0 = T.workloads.shell(2, 2, polydeg=7, seed=6, precision=precision,
+                                                      volume_flux="flux_ranocha",
+                                                      surface_flux="flux_ranocha")
+    else:
+        mesh, eq, solver, bcs, u0 = T.workloads.shell(3, 2, polydeg=4, seed=5, precision=precision)
+    if case in ("shell_flow", "shell7_ec"):
+        u0 = u0.copy()
+        u0[1:4] += 10.0 * np.sin(mesh.node_coordinates / 1e6) * u0[0]
+    semi = T.SemidiscretizationHyperbolic(mesh, eq, solver, bcs)
+    du = semi.rhs(None, u0)
+    p = params_for((mesh, eq, solver, 0.0))
+    du_ref = oracle.rhs(p, mesh, u0)
+    mag = np.abs(oracle.rhs_magnitude(p, mesh, u0).reshape(6, -1, order="F")).max(axis=1)
+    ref = np.abs(du_ref.reshape(6, -1, order="F")).max(axis=1)
+    diff = np.abs((du - du_ref).reshape(6, -1, order="F")).max(axis=1)
+    rel = diff / np.maximum(ref, 1e-300)
+    print(precision, case, "rel err", rel, "term-scale err", diff / np.maximum(mag, 1e-300))
+    umax = np.abs(u0.reshape(6, -1, order="F")).max(axis=1)
+    for v in range(5):
+        if ref[v] > 1e-10 * umax[v]:
+            assert rel[v] <= MIXED_RTOL, (v, rel)
+        else:  # round-off-only tendency (state at rest)
+            assert diff[v] <= 1e-10 * umax[v], (v, diff, umax)

@@ -52,6 +52,11 @@ struct atmo_handle {
   double *d_u = nullptr, *d_u2 = nullptr, *d_k = nullptr, *d_du = nullptr;
   double *d_uio = nullptr, *d_duio = nullptr;  // staging for atmo_rhs with host arrays
   double *d_cv = nullptr, *d_invJ = nullptr;
+  float* d_cv32 = nullptr;  // FP32 geometry of the mixed-precision path
+  bool bg_on = false;       // hydrostatic background split (mixed path)
+  double *d_ubg = nullptr, *d_dubg = nullptr;
+  float* d_B = nullptr;
+  std::vector<double> dsplit, weights;
   int* d_nbr = nullptr;
   int* d_face_info = nullptr;
   double *d_partial = nullptr, *d_result = nullptr;
@@ -64,25 +69,6 @@ struct atmo_handle {
 };
 
 namespace {
-
-std::once_flag g_tables_once[64];
-hipError_t g_tables_status[64];
-
-hipError_t ensure_tables(int device) {
-  if (device < 0 || device >= 64) return hipErrorInvalidDevice;
-  std::call_once(g_tables_once[device], [device]() {
-    double ds[9][64], w[9][8];
-    std::memset(ds, 0, sizeof(ds));
-    std::memset(w, 0, sizeof(w));
-    for (int n = 2; n <= 8; ++n) {
-      Basis b(n);
-      for (int q = 0; q < n * n; ++q) ds[n][q] = b.Dsplit[q];
-      for (int q = 0; q < n; ++q) w[n][q] = b.w[q];
-    }
-    g_tables_status[device] = e3_upload_basis_tables(&ds[0][0], &w[0][0]);
-  });
-  return g_tables_status[device];
-}
 
 inline int code_index(int code, int a, int b, int n) {
   switch (code) {
@@ -160,19 +146,48 @@ E3Launch base_launch(atmo_handle* h) {
   L.surface_flux = h->cfg.surface_flux;
   L.surface_noncons = h->cfg.surface_noncons;
   L.cv = h->d_cv;
+  L.cv_t = h->cfg.precision == ATMO_PREC_FP64 ? (const void*)h->d_cv : (const void*)h->d_cv32;
   L.invJ = h->d_invJ;
+  L.dsplit = h->dsplit.data();
+  L.weights = h->weights.data();
+  L.inv_w = 1.0 / h->weights[0];
   L.nbr = h->d_nbr;
   L.face_info = h->d_face_info;
   L.coriolis = h->cfg.source_terms == ATMO_SRC_CORIOLIS ? 2.0 * h->cfg.rotation_rate : 0.0;
+  L.bg.on = h->bg_on ? 1 : 0;
+  L.bg.p0 = h->cfg.background_p0;
+  L.bg.inv_RT0 = h->bg_on ? 1.0 / ((h->cfg.c_p - h->cfg.c_v) * h->cfg.background_T0) : 0.0;
+  L.bg.B = h->d_B;
   return L;
 }
 
 int launch_stage(atmo_handle* h, const E3Launch& L) {
   TimerPair tp;
   if (h->timing) timer_begin(h, 0, &tp);
-  const bool ok = e3_launch_stage_fp64(L, h->stream);
+  const int prec = h->cfg.precision;
+  const bool ok = prec == ATMO_PREC_FP64         ? e3_launch_stage_fp64(L, h->stream)
+                  : prec == ATMO_PREC_FP32_KAHAN ? e3_launch_stage_fp32_kahan(L, h->stream)
+                                                 : e3_launch_stage_fp32(L, h->stream);
   if (h->timing) timer_end(h, &tp);
   if (!ok) return fail(&h->err, ATMO_ERR_INVALID, "no kernel for this flux/polydeg combination");
+  HIP_TRY(h, hipGetLastError());
+  return ATMO_OK;
+}
+
+// FP64 tendency of the hydrostatic background built from the geopotential of
+// u (device pointer), stored as FP32 momentum rows for the mixed kernels
+int compute_background(atmo_handle* h, const double* u) {
+  if (!h->bg_on) return ATMO_OK;
+  E3Launch L = base_launch(h);
+  e3_launch_background_state(L, u, h->d_ubg, h->stream);
+  L.bg.on = 0;
+  L.mode = 0;
+  L.cv_t = h->d_cv;
+  L.u = h->d_ubg;
+  L.out = h->d_dubg;
+  if (!e3_launch_stage_fp64(L, h->stream))
+    return fail(&h->err, ATMO_ERR_INVALID, "no kernel for this flux/polydeg combination");
+  e3_launch_extract_momentum(h->d_dubg, h->d_B, (long)h->n3 * h->cfg.n_elements, h->stream);
   HIP_TRY(h, hipGetLastError());
   return ATMO_OK;
 }
@@ -288,7 +303,7 @@ int atmo_create(const atmo_config* cfg, const atmo_geometry3d* g, atmo_handle** 
     return fail(nullptr, ATMO_ERR_INVALID, "unsupported nonconservative flux");
   if (cfg->surface_flux < 0 || cfg->surface_flux > 5)
     return fail(nullptr, ATMO_ERR_INVALID, "unsupported surface flux");
-  if (cfg->precision != ATMO_PREC_FP64)
+  if (cfg->precision < ATMO_PREC_FP64 || cfg->precision > ATMO_PREC_FP32)
     return fail(nullptr, ATMO_ERR_INVALID, "unsupported precision");
   if (!(cfg->c_p > cfg->c_v) || !(cfg->c_v > 0))
     return fail(nullptr, ATMO_ERR_INVALID, "need c_p > c_v > 0");
@@ -362,6 +377,7 @@ int atmo_create(const atmo_config* cfg, const atmo_geometry3d* g, atmo_handle** 
   h->n = n;
   h->n3 = n3;
   h->len = (int64_t)6 * n3 * E;
+  h->bg_on = cfg->precision != ATMO_PREC_FP64 && cfg->background_p0 > 0 && cfg->background_T0 > 0;
   auto cleanup_fail = [&](int code) {
     atmo_destroy(h);
     return code;
@@ -370,9 +386,10 @@ int atmo_create(const atmo_config* cfg, const atmo_geometry3d* g, atmo_handle** 
     fail(nullptr, ATMO_ERR_DEVICE, "no HIP device");
     return cleanup_fail(ATMO_ERR_DEVICE);
   }
-  if (ensure_tables(h->device) != hipSuccess) {
-    fail(nullptr, ATMO_ERR_DEVICE, "uploading basis tables failed");
-    return cleanup_fail(ATMO_ERR_DEVICE);
+  {
+    Basis b(n);
+    h->dsplit = b.Dsplit;
+    h->weights = b.w;
   }
   const size_t sb = sizeof(double) * h->len;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -384,7 +401,11 @@ int atmo_create(const atmo_config* cfg, const atmo_geometry3d* g, atmo_handle** 
       hipMalloc(&h->d_nbr, sizeof(int32_t) * nbr.size()) != hipSuccess ||
       hipMalloc(&h->d_face_info, sizeof(int32_t) * face_info.size()) != hipSuccess ||
       hipMalloc(&h->d_partial, sizeof(double) * 9 * (size_t)E) != hipSuccess ||
-      hipMalloc(&h->d_result, sizeof(double) * 16) != hipSuccess) {
+      hipMalloc(&h->d_result, sizeof(double) * 16) != hipSuccess ||
+      (cfg->precision != ATMO_PREC_FP64 &&
+       hipMalloc(&h->d_cv32, sizeof(float) * 9 * n3 * (size_t)E) != hipSuccess) ||
+      (h->bg_on && (hipMalloc(&h->d_ubg, sb) != hipSuccess || hipMalloc(&h->d_dubg, sb) != hipSuccess ||
+                    hipMalloc(&h->d_B, sizeof(float) * 3 * n3 * (size_t)E) != hipSuccess))) {
     fail(nullptr, ATMO_ERR_DEVICE, "device allocation failed");
     return cleanup_fail(ATMO_ERR_DEVICE);
   }
@@ -399,6 +420,15 @@ int atmo_create(const atmo_config* cfg, const atmo_geometry3d* g, atmo_handle** 
       hipMemset(h->d_u, 0, sb) != hipSuccess || hipMemset(h->d_k, 0, sb) != hipSuccess) {
     fail(nullptr, ATMO_ERR_DEVICE, "device upload failed");
     return cleanup_fail(ATMO_ERR_DEVICE);
+  }
+  if (h->d_cv32) {
+    std::vector<float> cv32((size_t)9 * n3 * E);
+    for (size_t q = 0; q < cv32.size(); ++q) cv32[q] = (float)g->contravariant_vectors[q];
+    if (hipMemcpy(h->d_cv32, cv32.data(), sizeof(float) * cv32.size(), hipMemcpyHostToDevice) !=
+        hipSuccess) {
+      fail(nullptr, ATMO_ERR_DEVICE, "device upload failed");
+      return cleanup_fail(ATMO_ERR_DEVICE);
+    }
   }
   *out = h;
   return ATMO_OK;
@@ -416,6 +446,10 @@ void atmo_destroy(atmo_handle* h) {
   for (double* p : bufs)
     if (p) hipFree(p);
   if (h->d_nbr) hipFree(h->d_nbr);
+  if (h->d_cv32) hipFree(h->d_cv32);
+  if (h->d_ubg) hipFree(h->d_ubg);
+  if (h->d_dubg) hipFree(h->d_dubg);
+  if (h->d_B) hipFree(h->d_B);
   if (h->d_face_info) hipFree(h->d_face_info);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -428,7 +462,9 @@ int atmo_rhs(atmo_handle* h, double* du, const double* u, double t) {
   if (!h || !du || !u) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
   const size_t sb = sizeof(double) * h->len;
   HIP_TRY(h, hipMemcpyAsync(h->d_uio, u, sb, hipMemcpyHostToDevice, h->stream));
-  int rc = run_rhs(h, h->d_duio, h->d_uio);
+  int rc = compute_background(h, h->d_uio);
+  if (rc != ATMO_OK) return rc;
+  rc = run_rhs(h, h->d_duio, h->d_uio);
   if (rc != ATMO_OK) return rc;
   HIP_TRY(h, hipMemcpyAsync(du, h->d_duio, sb, hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
@@ -438,6 +474,8 @@ int atmo_rhs(atmo_handle* h, double* du, const double* u, double t) {
 int atmo_rhs_device(atmo_handle* h, double* du, const double* u, double t) {
   (void)t;
   if (!h || !du || !u) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
+  int rc = compute_background(h, u);
+  if (rc != ATMO_OK) return rc;
   return run_rhs(h, du, u);
 }
 
@@ -445,6 +483,8 @@ int atmo_set_state(atmo_handle* h, const double* u) {
   if (!h || !u) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
   HIP_TRY(h, hipMemcpyAsync(h->d_u, u, sizeof(double) * h->len, hipMemcpyHostToDevice, h->stream));
   HIP_TRY(h, hipMemsetAsync(h->d_k, 0, sizeof(double) * h->len, h->stream));
+  int rc = compute_background(h, h->d_u);
+  if (rc != ATMO_OK) return rc;
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   h->du_valid = false;
   return ATMO_OK;

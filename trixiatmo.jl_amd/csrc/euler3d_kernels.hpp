@@ -8,6 +8,13 @@
 
 namespace atmo {
 
+// hydrostatic background of the mixed path (see euler3d_stage.hpp)
+struct Background {
+  int on;
+  double p0, inv_RT0;
+  const float* B;  // [n nodes, 3] FP32 copy of the background's momentum tendency
+};
+
 struct E3Launch {
   E3Params q;
   int n, n_elements;
@@ -16,17 +23,27 @@ struct E3Launch {
   const double* u;
   double* out;
   double* k;
-  const double* cv;
+  const double* cv;       // FP64 contravariant vectors (reductions)
+  const void* cv_t;       // contravariant vectors in the stage kernel's compute type
   const double* invJ;
+  const double* dsplit;   // host copy of Dsplit[n*n]
+  const double* weights;  // host copy of the LGL weights[n]
+  double inv_w;           // 1 / w_end
   const int* nbr;  // [E, 6, n*n] neighbour global node id, < 0 boundary
   const int* face_info;  // [E, 6] 0 primary/boundary, 8 | dp secondary of primary direction dp
   double rk_a, rk_dt, rk_b;
   double coriolis;  // 2 Omega, 0 = no source
+  Background bg;
 };
 
-// tables: dsplit_all[9][64] (entry [n][i*n+m]), weights_all[9][8]
-hipError_t e3_upload_basis_tables(const double* dsplit_all, const double* weights_all);
+// background state from the geopotential of u: (rho_bg, 0, 0, 0, p_bg/(gamma-1), phi)
+void e3_launch_background_state(const E3Launch& L, const double* u, double* ubg, hipStream_t s);
+// FP32 copy of the momentum rows of du: B[node * 3 + c] = du[node * 6 + 1 + c]
+void e3_launch_extract_momentum(const double* du, float* B, long n_nodes, hipStream_t s);
+
 bool e3_launch_stage_fp64(const E3Launch& L, hipStream_t s);
+bool e3_launch_stage_fp32_kahan(const E3Launch& L, hipStream_t s);
+bool e3_launch_stage_fp32(const E3Launch& L, hipStream_t s);
 // which 0: per-element max scaled speed -> result[0] = max
 // which 1: integrals (9 columns) -> result[0..8]
 bool e3_launch_reduction(const E3Launch& L, int which, const double* du, double* partial,

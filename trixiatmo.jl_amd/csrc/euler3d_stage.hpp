@@ -1,0 +1,616 @@
+// Fused RK-stage kernel of the 3D DGSEM right-hand side for
+// CompressibleEulerEnergyEquationsWithGravity3D on curved hexahedra
+// (the work of Trixi's rhs! for P4estMesh{3}: volume flux differencing,
+// prolong2interfaces + calc_interface_flux!, boundary fluxes, surface
+// integral, apply_jacobian!, calc_sources!, then the 2N RK update).
+//
+// One workgroup owns one element.  Per direction, lanes evaluate every
+// unordered node pair of every line once (two-point volume flux f# and the
+// gravity term g with the averaged contravariant vector) into LDS; then every
+// node gathers its n-1 pair terms with Dsplit[own, m] (g enters with the sign
+// of (own - m), being antisymmetric).  Lines are processed in chunks so the
+// pair buffer fits LDS for every polydeg.  Then one lane per face node
+// evaluates the surface flux (neighbour state read from HBM, primary normal
+// as Trixi's calc_interface_flux! uses it) into LDS, and every node adds its
+// lifted face terms, multiplies by -1/J, adds the node-local source and
+// either stores du or applies k = a k + dt du, u_out = u + b k (ping-pong
+// state buffers, since neighbours still read the old state's face nodes).
+//
+// Compute type T: double (FP64 path, IEEE ops, compared with the oracle to
+// 1e-12) or float (mixed path: node states, pressure recovery, geopotential
+// jumps, the hydrostatic-background split and the RK registers stay FP64;
+// fluxes, geometry and volume sums are FP32, with Kahan-compensated sums when
+// KAHAN is set).  With the background split the momentum rows carry only
+// p' = p - p_bg and (rho_avg - rho_avg,bg) [phi]; the FP64 tendency of the
+// background is added back, so the discretisation is unchanged.
+#ifndef ATMO_EULER3D_STAGE_HPP
+#define ATMO_EULER3D_STAGE_HPP
+
+#include <hip/hip_runtime.h>
+
+#include "euler3d_kernels.hpp"
+#include "euler3d_physics.hpp"
+
+namespace atmo {
+
+template <int N>
+struct Shape {
+  static constexpr int N2 = N * N;
+  static constexpr int N3 = N2 * N;
+  static constexpr int BLOCK = ((N3 + 63) / 64) * 64;
+  static constexpr int P = N * (N - 1) / 2;  // unordered pairs per line
+};
+
+// pair-buffer budget per workgroup (bytes); lines are chunked to fit
+template <int N, typename T>
+struct Chunking {
+  static constexpr int BUDGET = 40 * 1024;
+  static constexpr int PAIR_BYTES = 9 * (int)sizeof(T);
+  static constexpr int LINES_MAX = BUDGET / (PAIR_BYTES * Shape<N>::P);
+  static constexpr int LINES = LINES_MAX >= Shape<N>::N2 ? Shape<N>::N2 : LINES_MAX;
+  static constexpr int CHUNKS = (Shape<N>::N2 + LINES - 1) / LINES;
+  static constexpr int NPC = LINES * Shape<N>::P;  // pairs per chunk
+  static constexpr int FACE = 36 * Shape<N>::N2;   // face buffer [6 vars][6 n^2]
+  static constexpr int FLEN = 9 * NPC > FACE ? 9 * NPC : FACE;
+};
+
+// running sum, optionally Kahan-compensated
+template <typename T, bool KAHAN>
+struct Acc {
+  T s = T(0), c = T(0);
+  __device__ __forceinline__ void add(T t) {
+    if constexpr (KAHAN) {
+      const T y = t - c;
+      const T z = s + y;
+      c = (z - s) - y;
+      s = z;
+    } else {
+      s += t;
+    }
+  }
+  __device__ __forceinline__ T value() const { return s; }
+};
+
+template <typename T>
+__device__ __forceinline__ void surface_flux(const E3Params& q, int surf, const T* ul, const T* ur,
+                                             const Prim<T>& l, const Prim<T>& r, T n0, T n1,
+                                             T n2, T f[6]) {
+  if (surf == 0) {  // flux_lax_friedrichs
+    flux_central(l, r, n0, n1, n2, f);
+    const T lam = max_abs_speed(q, l, r, n0, n1, n2);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) f[v] += T(-0.5) * lam * (ur[v] - ul[v]);
+    f[5] = T(-0.5) * lam * (ur[5] - ul[5]);
+  } else if (surf == 1) {
+    flux_ranocha(q, l, r, n0, n1, n2, f);
+    f[5] = T(0);
+  } else if (surf == 2) {
+    flux_kennedy_gruber(l, r, n0, n1, n2, f);
+    f[5] = T(0);
+  } else if (surf == 3) {
+    flux_lmars(q, ul, ur, l, r, n0, n1, n2, f);
+  } else {
+    // FluxPlusDissipation(f#, DissipationLocalLaxFriedrichs(max_abs_speed))
+    if (surf == 4) flux_kennedy_gruber(l, r, n0, n1, n2, f);
+    else flux_ranocha(q, l, r, n0, n1, n2, f);
+    const T lam = max_abs_speed(q, l, r, n0, n1, n2);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) f[v] += T(-0.5) * lam * (ur[v] - ul[v]);
+    f[5] = T(-0.5) * lam * (ur[5] - ul[5]);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void surface_noncons(int snc, const Prim<T>& l, const Prim<T>& r, T n0,
+                                                T n1, T n2, T g[4]) {
+  if (snc == 1) noncons_gravity<1>(l, r, n0, n1, n2, g);
+  else noncons_gravity<2>(l, r, n0, n1, n2, g);
+}
+
+// Per-node quantities of the two-point fluxes, with the divisions and
+// logarithms done once per node instead of once per pair.
+template <typename T>
+struct FastPrim {
+  T rho, v1, v2, v3, p, e, phi, lrho, lrp;  // e = rho_e / rho, lrp = log(rho / p)
+  // hydrostatic-background split (mixed path): p' = p - p_bg, rho' = rho - rho_bg,
+  // lam = log(rho / rho_bg); the background is formed in FP64 from phi
+  T pp, rb, rp, lam;
+};
+
+// Hydrostatic background of the mixed path (struct Background): the isothermal
+// atmosphere p_bg = p0 exp(-phi / (R T0)), rho_bg = p_bg / (R T0), and an FP32
+// copy B of the FP64 rhs of that background state (momentum rows only; mass
+// and energy tendencies of a state at rest vanish).
+
+// delta = ln_mean(x, y) - ln_mean(xb, yb) from perturbation quantities, with
+// the branch of the actual state: series form where (y-x)^2 < 1e-4 (x+y)^2,
+// else the logarithmic form with dLb = log yb - log xb, dlam = log(y/yb) -
+// log(x/xb) (exact identities; no difference of two rounded means is formed).
+template <typename T>
+__device__ __forceinline__ T delta_ln_mean(T A, T s, T Ab, T sb, T Ap, T sp, T dLb, T dlam) {
+  if (A * A < T(1.0e-4) * (s * s)) {
+    const T f = div_(A, s), fb = div_(Ab, sb);
+    const T f2 = f * f, fb2 = fb * fb;
+    const T Q = T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0)));
+    const T Qb = T(2) + fb2 * (T(2.0 / 3.0) + fb2 * (T(2.0 / 5.0) + fb2 * T(2.0 / 7.0)));
+    const T df = div_(Ap * sb - Ab * sp, s * sb);  // f - fb
+    const T dQ = -df * (f + fb) *
+                 (T(2.0 / 3.0) + T(2.0 / 5.0) * (fb2 + f2) + T(2.0 / 7.0) * (fb2 * fb2 + fb2 * f2 + f2 * f2));
+    return div_(sp, Q) + div_(sb * dQ, Q * Qb);  // dQ = Qb - Q
+  }
+  const T dL = dLb + dlam;
+  if (fabs(dLb) > T(1e-6) * fabs(dL)) return div_(Ap * dLb - Ab * dlam, dLb * dL);
+  return div_(A, dL) - T(0.5) * sb;  // flat background: ln_mean(xb, yb) = (xb + yb) / 2
+}
+
+// ln_mean / inv_ln_mean from node logarithms: (y - x) / (log y - log x) away
+// from x = y, the series of Ranocha et al. (2023) where (x-y)^2/(x+y)^2 < 1e-4.
+template <typename T>
+__device__ __forceinline__ T ln_mean_logs(T x, T y, T lx, T ly) {
+  const T d = y - x, s = x + y;
+  if (d * d < T(1.0e-4) * (s * s)) {
+    const T f2 = div_(d * d, s * s);
+    return div_(s, T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0))));
+  }
+  return div_(d, ly - lx);
+}
+
+template <typename T>
+__device__ __forceinline__ T inv_ln_mean_logs(T x, T y, T lx_minus_ly) {
+  const T d = y - x, s = x + y;
+  if (d * d < T(1.0e-4) * (s * s)) {
+    const T f2 = div_(d * d, s * s);
+    return div_(T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0))), s);
+  }
+  return div_(-lx_minus_ly, d);
+}
+
+// the volume flux f (5 entries) and the gravity term g (4 entries) of one pair
+template <int VOL, int NC, typename T>
+__device__ __forceinline__ void pair_flux(const E3Params& q, const FastPrim<T>& l,
+                                          const FastPrim<T>& r, T n0, T n1, T n2, T dphi,
+                                          const Background& bg, T f[5], T g[4]) {
+  const T v1a = T(0.5) * (l.v1 + r.v1), v2a = T(0.5) * (l.v2 + r.v2), v3a = T(0.5) * (l.v3 + r.v3);
+  const T pa = T(0.5) * (l.p + r.p);
+  // momentum pressure term: p' when the hydrostatic background is split off
+  const T pm = bg.on ? T(0.5) * (l.pp + r.pp) : pa;
+  if constexpr (VOL == 0) {  // flux_ranocha, :382-415
+    const T vnl = l.v1 * n0 + l.v2 * n1 + l.v3 * n2;
+    const T vnr = r.v1 * n0 + r.v2 * n1 + r.v3 * n2;
+    const T rho_mean = ln_mean_logs(l.rho, r.rho, l.lrho, r.lrho);
+    // inv_ln_mean(rho_l p_r, rho_r p_l): log of the ratio = lrp_r - lrp_l
+    const T inv_rp = l.p * r.p * inv_ln_mean_logs(l.rho * r.p, r.rho * l.p, l.lrp - r.lrp);
+    const T vsq = T(0.5) * (l.v1 * r.v1 + l.v2 * r.v2 + l.v3 * r.v3);
+    const T f1 = rho_mean * T(0.5) * (vnl + vnr);
+    f[0] = f1;
+    f[1] = f1 * v1a + pm * n0;
+    f[2] = f1 * v2a + pm * n1;
+    f[3] = f1 * v3a + pm * n2;
+    f[4] = f1 * (vsq + inv_rp * T(q.inv_gm1)) + T(0.5) * (l.p * vnr + r.p * vnl);
+  } else if constexpr (VOL == 1) {  // flux_kennedy_gruber, :289-318
+    const T ra = T(0.5) * (l.rho + r.rho);
+    const T ea = T(0.5) * (l.e + r.e);
+    const T vna = v1a * n0 + v2a * n1 + v3a * n2;
+    const T f1 = ra * vna;
+    f[0] = f1;
+    f[1] = f1 * v1a + pm * n0;
+    f[2] = f1 * v2a + pm * n1;
+    f[3] = f1 * v3a + pm * n2;
+    f[4] = f1 * ea + pa * vna;
+  } else {  // flux_shima_etal, :248-276
+    const T vnl = l.v1 * n0 + l.v2 * n1 + l.v3 * n2;
+    const T vnr = r.v1 * n0 + r.v2 * n1 + r.v3 * n2;
+    const T ra = T(0.5) * (l.rho + r.rho);
+    const T vna = T(0.5) * (vnl + vnr);
+    const T vsq = T(0.5) * (l.v1 * r.v1 + l.v2 * r.v2 + l.v3 * r.v3);
+    const T f1 = ra * vna;
+    f[0] = f1;
+    f[1] = f1 * v1a + pm * n0;
+    f[2] = f1 * v2a + pm * n1;
+    f[3] = f1 * v3a + pm * n2;
+    f[4] = f1 * vsq + pa * vna * T(q.inv_gm1) + T(0.5) * (l.p * vnr + r.p * vnl);
+  }
+  if constexpr (NC != 0) {  // waruszewski (:135-159) / souza (:206-230)
+    const T ra = (NC == 1) ? ln_mean_logs(l.rho, r.rho, l.lrho, r.lrho) : T(0.5) * (l.rho + r.rho);
+    const T sj = ra * dphi;
+    T sm = sj;  // momentum part: (rho_avg - rho_avg of the background) [phi]
+    if (bg.on) {
+      const T Ap = r.rp - l.rp, sp = l.rp + r.rp;
+      const T Ab = r.rb - l.rb, sb = l.rb + r.rb;
+      const T dmean = (NC == 1) ? delta_ln_mean(Ap + Ab, sp + sb, Ab, sb, Ap, sp,
+                                                -dphi * T(bg.inv_RT0), r.lam - l.lam)
+                                : T(0.5) * sp;
+      sm = dmean * dphi;
+    }
+    g[0] = n0 * sm;
+    g[1] = n1 * sm;
+    g[2] = n2 * sm;
+    g[3] = n0 * sj * v1a + n1 * sj * v2a + n2 * sj * v3a;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ Prim<T> to_t(const Prim<double>& w) {
+  Prim<T> o;
+  o.rho = T(w.rho); o.v1 = T(w.v1); o.v2 = T(w.v2); o.v3 = T(w.v3);
+  o.p = T(w.p); o.rho_e = T(w.rho_e); o.phi = T(w.phi);
+  return o;
+}
+
+// Surface fluxes of all 6 n^2 face nodes of element e, one lane per face node,
+// lifted by 1/w_end into LDS as s_sf[v * 6 n^2 + f n^2 + a + n b].  Node
+// states and their primitives are formed in FP64 and then rounded to T.
+// Momentum rows of the surface flux with the background pressure split off
+// (p' for the pressure terms; wave speeds and upwinding keep the full state).
+template <typename T>
+__device__ __forceinline__ void surface_momentum_split(const E3Params& q, int surf, const T* ul,
+                                                       const T* ur, const Prim<T>& l,
+                                                       const Prim<T>& r, T ppl, T ppr, T n0, T n1,
+                                                       T n2, T m[3]) {
+  const T nv[3] = {n0, n1, n2};
+  const T vl[3] = {l.v1, l.v2, l.v3}, vr[3] = {r.v1, r.v2, r.v3};
+  const T vnl = l.v1 * n0 + l.v2 * n1 + l.v3 * n2, vnr = r.v1 * n0 + r.v2 * n1 + r.v3 * n2;
+  const T pma = T(0.5) * (ppl + ppr);
+  if (surf == 0) {
+    const T lam = max_abs_speed(q, l, r, n0, n1, n2);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      m[k] = T(0.5) * ((l.rho * vnl * vl[k] + ppl * nv[k]) + (r.rho * vnr * vr[k] + ppr * nv[k])) +
+             T(-0.5) * lam * (ur[k + 1] - ul[k + 1]);
+  } else if (surf == 3) {  // FluxLMARS
+    const T c = T(q.lmars_c);
+    const T nn = sqrt_(n0 * n0 + n1 * n1 + n2 * n2);
+    const T rho = T(0.5) * (l.rho + r.rho);
+    const T pm = pma - div_(T(0.5) * c * rho * (vnr - vnl), nn);
+    const T v = T(0.5) * (vnl + vnr) - div_(T(1), T(2) * c * rho) * (r.p - l.p) * nn;
+    const T* us = v >= T(0) ? ul : ur;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) m[k] = v * us[k + 1] + pm * nv[k];
+  } else {
+    T f1;
+    if (surf == 1 || surf == 5) f1 = ln_mean(l.rho, r.rho) * T(0.5) * (vnl + vnr);
+    else
+      f1 = T(0.5) * (l.rho + r.rho) *
+           (T(0.5) * (l.v1 + r.v1) * n0 + T(0.5) * (l.v2 + r.v2) * n1 + T(0.5) * (l.v3 + r.v3) * n2);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) m[k] = f1 * T(0.5) * (vl[k] + vr[k]) + pma * nv[k];
+    if (surf >= 4) {
+      const T lam = max_abs_speed(q, l, r, n0, n1, n2);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) m[k] += T(-0.5) * lam * (ur[k + 1] - ul[k + 1]);
+    }
+  }
+}
+
+template <int N, typename T>
+__device__ __forceinline__ void face_fluxes_to_lds(
+    const E3Params& q, int e, int t, double inv_w, const double* __restrict__ u,
+    const T* __restrict__ cv, const int* __restrict__ nbr, const int* __restrict__ face_info,
+    int surf, int snc, const Background& bg, const double* s_pb, T* s_sf) {
+  constexpr int N2 = Shape<N>::N2, N3 = Shape<N>::N3, NF = 6 * N2;
+  for (int fid = t; fid < NF; fid += Shape<N>::BLOCK) {
+    const int f = fid / N2, ab = fid - f * N2;
+    const int a = ab % N, b = ab / N;
+    const int d = f / 2, fixed = (f % 2 == 0) ? 0 : N - 1;
+    const int node = d == 0 ? fixed + N * (a + N * b)
+                            : (d == 1 ? a + N * (fixed + N * b) : a + N * (b + N * fixed));
+    const size_t gn = (size_t)e * N3 + node;
+    double uo[6];
+#pragma unroll
+    for (int v = 0; v < 6; ++v) uo[v] = u[gn * 6 + v];
+    const int link = nbr[((size_t)e * 6 + f) * N2 + ab];
+    // face_info: 0 = primary side or boundary (own outward normal);
+    // 8 | dp = secondary side of an interface whose primary face direction is dp
+    const int info = face_info[(size_t)e * 6 + f];
+    T n0, n1, n2;
+    if (info == 0) {
+      const T sgn = (f % 2 == 0) ? T(-1) : T(1);
+      const T* jo = cv + gn * 9 + 3 * d;
+      n0 = sgn * jo[0];
+      n1 = sgn * jo[1];
+      n2 = sgn * jo[2];
+    } else {
+      const int dp = info & 7;
+      const T sgn = (dp % 2 == 0) ? T(1) : T(-1);  // minus the primary's outward normal
+      const T* jp = cv + (size_t)link * 9 + 3 * (dp / 2);
+      n0 = sgn * jp[0];
+      n1 = sgn * jp[1];
+      n2 = sgn * jp[2];
+    }
+    double ur[6];
+    if (link >= 0) {
+#pragma unroll
+      for (int v = 0; v < 6; ++v) ur[v] = u[(size_t)link * 6 + v];
+    } else {
+      // boundary_condition_slip_wall (:93-116): mirror the normal momentum
+      const double m0 = n0, m1 = n1, m2 = n2;
+      const double nn = sqrt(m0 * m0 + m1 * m1 + m2 * m2);
+      const double h0 = m0 / nn, h1 = m1 / nn, h2 = m2 / nn;
+      const double un = h0 * uo[1] + h1 * uo[2] + h2 * uo[3];
+      ur[0] = uo[0];
+      ur[1] = uo[1] - 2.0 * un * h0;
+      ur[2] = uo[2] - 2.0 * un * h1;
+      ur[3] = uo[3] - 2.0 * un * h2;
+      ur[4] = uo[4];
+      ur[5] = uo[5];
+    }
+    const Prim<T> own = to_t<T>(cons2prim(q, uo));
+    const Prim<T> pr = to_t<T>(cons2prim(q, ur));
+    T ut[6], urt[6];
+#pragma unroll
+    for (int v = 0; v < 6; ++v) { ut[v] = T(uo[v]); urt[v] = T(ur[v]); }
+    T fs[6];
+    surface_flux<T>(q, surf, ut, urt, own, pr, n0, n1, n2, fs);
+    double pb_o = 0.0, pb_n = 0.0;
+    if (bg.on) {
+      // own background pressure from the prologue; across a face phi is
+      // continuous up to rounding, so the neighbour's follows to first order
+      pb_o = s_pb[node];
+      const double dph = ur[5] - uo[5];
+      pb_n = fabs(dph * bg.inv_RT0) < 1e-8 ? pb_o * (1.0 - dph * bg.inv_RT0)
+                                           : bg.p0 * exp(-ur[5] * bg.inv_RT0);
+      T m[3];
+      const double pp_o = (q.gm1 * (uo[4] - 0.5 * (uo[1] * uo[1] + uo[2] * uo[2] + uo[3] * uo[3]) / uo[0])) - pb_o;
+      const double pp_n = (q.gm1 * (ur[4] - 0.5 * (ur[1] * ur[1] + ur[2] * ur[2] + ur[3] * ur[3]) / ur[0])) - pb_n;
+      surface_momentum_split<T>(q, surf, ut, urt, own, pr, T(pp_o), T(pp_n), n0, n1, n2, m);
+      fs[1] = m[0];
+      fs[2] = m[1];
+      fs[3] = m[2];
+    }
+    if (snc != 0) {
+      // gravity term with the phi jump formed in FP64
+      const T dphi = T(ur[5] - uo[5]);
+      const T ra = snc == 1 ? ln_mean(own.rho, pr.rho) : T(0.5) * (own.rho + pr.rho);
+      const T sj = ra * dphi;
+      T sm = sj;
+      if (bg.on) {
+        // the face jump of phi is rounding-sized: the background part of the
+        // mean density uses the arithmetic mean of the perturbations
+        const double rb_o = pb_o * bg.inv_RT0, rb_n = pb_n * bg.inv_RT0;
+        sm = T(0.5 * ((uo[0] - rb_o) + (ur[0] - rb_n))) * dphi;
+      }
+      const T v1a = T(0.5) * (own.v1 + pr.v1), v2a = T(0.5) * (own.v2 + pr.v2),
+              v3a = T(0.5) * (own.v3 + pr.v3);
+      fs[1] += T(0.5) * (n0 * sm);
+      fs[2] += T(0.5) * (n1 * sm);
+      fs[3] += T(0.5) * (n2 * sm);
+      fs[4] += T(0.5) * (n0 * sj * v1a + n1 * sj * v2a + n2 * sj * v3a);
+    }
+#pragma unroll
+    for (int v = 0; v < 6; ++v) s_sf[v * NF + fid] = fs[v] * T(inv_w);
+  }
+}
+
+struct DsplitTable {
+  double ds[64];  // Dsplit[i * n + m]
+};
+
+// element handled by workgroup b: workgroups are dispatched round-robin over
+// the 8 XCDs, so XCD x receives b = x, x + 8, ...; mapping those to one
+// contiguous element range keeps neighbouring elements in the same L2
+__device__ __forceinline__ int xcd_element(int b, int n_elements) {
+  const int per = (n_elements + 7) / 8;
+  return (b & 7) * per + (b >> 3);
+}
+
+template <int N, int VOL, int NC, typename T, bool KAHAN>
+__global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
+    E3Params q, DsplitTable dst, double inv_w, int n_elements, const double* __restrict__ u,
+    double* __restrict__ out, double* __restrict__ kreg, const T* __restrict__ cv,
+    const double* __restrict__ invJ, const int* __restrict__ nbr,
+    const int* __restrict__ face_info, int surf, int snc, int mode, double rk_a, double rk_dt,
+    double rk_b, double coriolis, Background bg) {
+  using S = Shape<N>;
+  using C = Chunking<N, T>;
+  constexpr int N2 = S::N2, N3 = S::N3, P = S::P, B = S::BLOCK;
+  constexpr int NV = 6;
+  constexpr bool MIXED = sizeof(T) == 4;
+  constexpr int NW = MIXED ? 13 : 9;  // per-node fields (the last 4: background split)
+  __shared__ T s_w[NW][N3];
+  __shared__ double s_phi[MIXED ? N3 : 1];  // FP64 phi for the jumps of the mixed path
+  __shared__ double s_pb[MIXED ? N3 : 1];   // FP64 background pressure
+  __shared__ T s_ja[3][N3];
+  __shared__ T s_fl[C::FLEN];  // pair terms [9][NPC], later face fluxes [6][6 n^2]
+  __shared__ T s_ds[N * N];
+
+  const int e = xcd_element(blockIdx.x, n_elements);
+  if (e >= n_elements) return;  // whole workgroup: padding of the XCD mapping
+  const int t = threadIdx.x;
+  const bool active = t < N3;
+  const int ti = t % N, tj = (t / N) % N, tk = t / N2;
+  const size_t gnode = (size_t)e * N3 + t;
+  if (t < N * N) s_ds[t] = T(dst.ds[t]);
+
+  double uo[NV];
+  if (active) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) uo[v] = u[gnode * NV + v];
+    // primitives (pressure from total minus kinetic energy) in FP64
+    const Prim<double> w = cons2prim(q, uo);
+    s_w[0][t] = T(w.rho); s_w[1][t] = T(w.v1); s_w[2][t] = T(w.v2); s_w[3][t] = T(w.v3);
+    s_w[4][t] = T(w.p);
+    s_w[5][t] = T(w.rho_e / w.rho);
+    s_w[6][t] = T(w.phi);
+    if constexpr (VOL == 0 || NC == 1) s_w[7][t] = log_(T(w.rho));
+    if constexpr (VOL == 0) s_w[8][t] = log_(T(w.rho / w.p));
+    if constexpr (MIXED) {
+      s_phi[t] = w.phi;
+      if (bg.on) {
+        // hydrostatic background in FP64; FP32 carries only the perturbation
+        const double pb = bg.p0 * exp(-w.phi * bg.inv_RT0);
+        const double rb = pb * bg.inv_RT0;
+        s_pb[t] = pb;
+        s_w[9][t] = T(w.p - pb);
+        s_w[10][t] = T(rb);
+        s_w[11][t] = T(w.rho - rb);
+        s_w[12][t] = log1pf(float((w.rho - rb) / rb));  // log(rho / rho_bg)
+      }
+    }
+  }
+
+  Acc<T, KAHAN> acc[5], acc_nc[4];
+  for (int d = 0; d < 3; ++d) {
+    __syncthreads();
+    if (active) {
+      s_ja[0][t] = cv[gnode * 9 + 3 * d + 0];
+      s_ja[1][t] = cv[gnode * 9 + 3 * d + 1];
+      s_ja[2][t] = cv[gnode * 9 + 3 * d + 2];
+    }
+    const int stride = d == 0 ? 1 : (d == 1 ? N : N2);
+    const int own_d = d == 0 ? ti : (d == 1 ? tj : tk);
+    const int own_line = d == 0 ? tj + N * tk : (d == 1 ? ti + N * tk : ti + N * tj);
+    for (int c = 0; c < C::CHUNKS; ++c) {
+      const int line0 = c * C::LINES;
+      const int n_lines = (N2 - line0) < C::LINES ? (N2 - line0) : C::LINES;
+      __syncthreads();
+      for (int pid = t; pid < n_lines * P; pid += B) {
+        const int lrel = pid / P;
+        const int line = line0 + lrel;
+        int qq = pid - lrel * P, a = 0;
+        while (qq >= N - 1 - a) { qq -= N - 1 - a; ++a; }
+        const int m = a + 1 + qq;
+        const int base = d == 0 ? N * line : (d == 1 ? (line % N) + N2 * (line / N) : line);
+        const int na = base + a * stride, nm = base + m * stride;
+        FastPrim<T> l, r;
+        l.rho = s_w[0][na]; l.v1 = s_w[1][na]; l.v2 = s_w[2][na]; l.v3 = s_w[3][na];
+        l.p = s_w[4][na]; l.e = s_w[5][na]; l.phi = s_w[6][na]; l.lrho = s_w[7][na]; l.lrp = s_w[8][na];
+        r.rho = s_w[0][nm]; r.v1 = s_w[1][nm]; r.v2 = s_w[2][nm]; r.v3 = s_w[3][nm];
+        r.p = s_w[4][nm]; r.e = s_w[5][nm]; r.phi = s_w[6][nm]; r.lrho = s_w[7][nm]; r.lrp = s_w[8][nm];
+        T dphi;
+        if constexpr (MIXED) {
+          dphi = T(s_phi[nm] - s_phi[na]);
+          if (bg.on) {
+            l.pp = s_w[9][na]; l.rb = s_w[10][na]; l.rp = s_w[11][na]; l.lam = s_w[12][na];
+            r.pp = s_w[9][nm]; r.rb = s_w[10][nm]; r.rp = s_w[11][nm]; r.lam = s_w[12][nm];
+          }
+        } else {
+          dphi = r.phi - l.phi;
+        }
+        const T n0 = T(0.5) * (s_ja[0][na] + s_ja[0][nm]);
+        const T n1 = T(0.5) * (s_ja[1][na] + s_ja[1][nm]);
+        const T n2 = T(0.5) * (s_ja[2][na] + s_ja[2][nm]);
+        T f[5], g[4];
+        pair_flux<VOL, NC, T>(q, l, r, n0, n1, n2, dphi, bg, f, g);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) s_fl[v * C::NPC + pid] = f[v];
+        if constexpr (NC != 0) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) s_fl[(5 + v) * C::NPC + pid] = g[v];
+        }
+      }
+      __syncthreads();
+      if (active && own_line >= line0 && own_line < line0 + n_lines) {
+#pragma unroll
+        for (int m = 0; m < N; ++m) {
+          if (m == own_d) continue;
+          const int a = m < own_d ? m : own_d, b = m < own_d ? own_d : m;
+          const int pid = (own_line - line0) * P + a * (2 * N - a - 1) / 2 + (b - a - 1);
+          const T ds = s_ds[own_d * N + m];
+#pragma unroll
+          for (int v = 0; v < 5; ++v) acc[v].add(ds * s_fl[v * C::NPC + pid]);
+          if constexpr (NC != 0) {
+            const T dsg = own_d < m ? ds : -ds;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc_nc[v].add(dsg * s_fl[(5 + v) * C::NPC + pid]);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();  // pair terms consumed; the buffer now takes the face fluxes
+  face_fluxes_to_lds<N, T>(q, e, t, inv_w, u, cv, nbr, face_info, surf, snc, bg, s_pb, s_fl);
+  __syncthreads();
+  if (!active) return;
+
+  double du[NV];
+  du[0] = double(acc[0].value());
+#pragma unroll
+  for (int v = 1; v < 5; ++v)
+    du[v] = double(acc[v].value()) + 0.5 * double(acc_nc[v - 1].value());
+  du[5] = 0.0;
+  constexpr int NF = 6 * N2;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const int d = f / 2;
+    const int od = d == 0 ? ti : (d == 1 ? tj : tk);
+    const bool on_face = (f % 2 == 0) ? (od == 0) : (od == N - 1);
+    if (!on_face) continue;
+    const int a = d == 0 ? tj : ti;
+    const int b = d == 2 ? tj : tk;
+    const int fid = f * N2 + a + N * b;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) du[v] += double(s_fl[v * NF + fid]);
+  }
+  const double fac = -invJ[gnode];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) du[v] *= fac;
+  if (bg.on) {  // tendency of the hydrostatic background, computed in FP64
+    du[1] += double(bg.B[gnode * 3 + 0]);
+    du[2] += double(bg.B[gnode * 3 + 1]);
+    du[3] += double(bg.B[gnode * 3 + 2]);
+  }
+  // calc_sources!: source_terms_coriolis (-2 Omega e_z x rho v), after the Jacobian
+  du[1] += coriolis * uo[2];
+  du[2] += -coriolis * uo[1];
+  if (mode == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) out[gnode * NV + v] = du[v];
+  } else {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double kn = rk_a * kreg[gnode * NV + v] + rk_dt * du[v];
+      kreg[gnode * NV + v] = kn;
+      out[gnode * NV + v] = uo[v] + rk_b * kn;
+    }
+  }
+}
+
+template <int N, int VOL, int NC, typename T, bool KAHAN>
+void launch_stage_inst(const E3Launch& L, hipStream_t s) {
+  DsplitTable dst;
+  for (int q = 0; q < N * N; ++q) dst.ds[q] = L.dsplit[q];
+  const int grid = 8 * ((L.n_elements + 7) / 8);
+  hipLaunchKernelGGL((e3_stage<N, VOL, NC, T, KAHAN>), dim3(grid), dim3(Shape<N>::BLOCK), 0, s,
+                     L.q, dst, L.inv_w, L.n_elements, L.u, L.out, L.k,
+                     reinterpret_cast<const T*>(L.cv_t), L.invJ, L.nbr, L.face_info,
+                     L.surface_flux, L.surface_noncons, L.mode, L.rk_a, L.rk_dt, L.rk_b,
+                     L.coriolis, L.bg);
+}
+
+template <typename T, bool KAHAN, int N, int VOL>
+bool dispatch_nc(const E3Launch& L, hipStream_t s) {
+  switch (L.volume_noncons) {
+    case 0: launch_stage_inst<N, VOL, 0, T, KAHAN>(L, s); return true;
+    case 1: launch_stage_inst<N, VOL, 1, T, KAHAN>(L, s); return true;
+    case 2: launch_stage_inst<N, VOL, 2, T, KAHAN>(L, s); return true;
+  }
+  return false;
+}
+
+template <typename T, bool KAHAN, int N>
+bool dispatch_vol(const E3Launch& L, hipStream_t s) {
+  switch (L.volume_flux) {
+    case 0: return dispatch_nc<T, KAHAN, N, 0>(L, s);
+    case 1: return dispatch_nc<T, KAHAN, N, 1>(L, s);
+    case 2: return dispatch_nc<T, KAHAN, N, 2>(L, s);
+  }
+  return false;
+}
+
+template <typename T, bool KAHAN>
+bool dispatch_stage(const E3Launch& L, hipStream_t s) {
+  switch (L.n) {
+    case 2: return dispatch_vol<T, KAHAN, 2>(L, s);
+    case 3: return dispatch_vol<T, KAHAN, 3>(L, s);
+    case 4: return dispatch_vol<T, KAHAN, 4>(L, s);
+    case 5: return dispatch_vol<T, KAHAN, 5>(L, s);
+    case 6: return dispatch_vol<T, KAHAN, 6>(L, s);
+    case 7: return dispatch_vol<T, KAHAN, 7>(L, s);
+    case 8: return dispatch_vol<T, KAHAN, 8>(L, s);
+  }
+  return false;
+}
+
+}  // namespace atmo
+
+#endif

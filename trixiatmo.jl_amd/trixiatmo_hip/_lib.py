@@ -40,6 +40,8 @@ class AtmoConfig(ctypes.Structure):
         ("lmars_c", ctypes.c_double),
         ("source_terms", ctypes.c_int32),
         ("rotation_rate", ctypes.c_double),
+        ("background_p0", ctypes.c_double),
+        ("background_T0", ctypes.c_double),
     ]
 
 

@@ -1,6 +1,6 @@
 """DGSEM solver description (Trixi `DGSEM(polydeg, surface_flux, volume_integral)`)."""
 from dataclasses import dataclass
-from typing import Tuple, Union
+from typing import Optional, Tuple, Union
 
 from .equations import (FluxLMARS, NONCONSERVATIVE_FLUXES, SURFACE_FLUXES, VOLUME_FLUXES)
 
@@ -24,6 +24,9 @@ class DGSEM:
     surface_flux: Tuple[Union[str, FluxLMARS], str]
     volume_integral: VolumeIntegralFluxDifferencing
     precision: str = "fp64"
+    # mixed paths: (p0, T0) of an isothermal hydrostatic background whose
+    # pressure and density are split off in FP64 (None: no split)
+    hydrostatic_background: Optional[Tuple[float, float]] = None
 
     def codes(self):
         vol, vnc = self.volume_integral.volume_flux
@@ -32,6 +35,8 @@ class DGSEM:
         if isinstance(sf, FluxLMARS):
             lmars_c = float(sf.speed_of_sound)
             sf = "FluxLMARS"
+        p0, T0 = self.hydrostatic_background or (0.0, 0.0)
         return dict(volume_flux=VOLUME_FLUXES[vol], volume_noncons=NONCONSERVATIVE_FLUXES[vnc],
                     surface_flux=SURFACE_FLUXES[sf], surface_noncons=NONCONSERVATIVE_FLUXES[snc],
-                    precision=PRECISIONS[self.precision], lmars_c=lmars_c)
+                    precision=PRECISIONS[self.precision], lmars_c=lmars_c,
+                    background_p0=float(p0), background_T0=float(T0))

@@ -14,9 +14,12 @@ def periodic_cube(trees=4, polydeg=3, seed=0, precision="fp64"):
     eq = CompressibleEulerEnergyEquationsWithGravity3D(c_p=1004.0, c_v=717.0, gravity=9.80616)
     mesh = P4estMesh((trees,) * 3, polydeg=polydeg, coordinates_min=(0.0, 0.0, 0.0),
                      coordinates_max=(1.0, 1.0, 1.0), periodicity=(True, True, True))
+    # mixed paths split off the mean pressure p = 1 (a constant background:
+    # phi = 0); rho_bg = p0 / (R T0) = 1 is unused without gravity
     solver = DGSEM(polydeg=polydeg, surface_flux=("flux_lax_friedrichs", "flux_zero"),
                    volume_integral=VolumeIntegralFluxDifferencing(("flux_ranocha", "flux_zero")),
-                   precision=precision)
+                   precision=precision,
+                   hydrostatic_background=None if precision == "fp64" else (1.0, 1.0 / eq.R))
     u0 = initial_condition_periodic_modes(mesh, eq, seed)
     return mesh, eq, solver, {}, u0
 
@@ -43,7 +46,8 @@ def shell(trees_per_face=16, layers=8, polydeg=4, seed=0, precision="fp64",
     solver = DGSEM(polydeg=polydeg, surface_flux=(surface_flux, surface_noncons),
                    volume_integral=VolumeIntegralFluxDifferencing(
                        (volume_flux, "flux_nonconservative_waruszewski_etal")),
-                   precision=precision)
+                   precision=precision,
+                   hydrostatic_background=None if precision == "fp64" else (eq.p_0, 250.0))
     u0 = initial_condition_isothermal_perturbed(mesh, eq, seed, inner_radius=SHELL_RADIUS)
     bcs = {"inside": "boundary_condition_slip_wall", "outside": "boundary_condition_slip_wall"}
     return mesh, eq, solver, bcs, u0
