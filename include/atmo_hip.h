@@ -31,6 +31,7 @@ extern "C" {
 /* -------------------------------------------------------- enumerations */
 /* equation systems */
 #define ATMO_EQ_EULER_ENERGY_GRAVITY_3D 1 /* CompressibleEulerEnergyEquationsWithGravity3D */
+#define ATMO_EQ_COVARIANT_SWE_2D 2        /* SplitCovariantShallowWaterEquations2D */
 
 /* volume (symmetric two-point) fluxes */
 #define ATMO_VOL_RANOCHA 0        /* flux_ranocha */
@@ -116,6 +117,28 @@ int atmo_mesh3d_arrays(const atmo_mesh3d* mesh, const double** node_coordinates,
                        const int32_t** boundary_kind);
 void atmo_mesh3d_destroy(atmo_mesh3d* mesh);
 
+/* 2D cubed sphere for the covariant form */
+typedef struct atmo_mesh2d atmo_mesh2d;
+
+/* P4estMeshCubedSphere2D(trees_per_face_dimension, radius; polydeg,
+ * element_local_mapping = true) with the 26 auxiliary variables per node of
+ * init_auxiliary_node_variables! (GlobalCartesianCoordinates; Christoffel
+ * symbols from the exact metric derivatives, or from the collocation
+ * derivative when `christoffel_collocation` is nonzero); bottom topography
+ * (aux row 20) is zero and may be set by the caller
+ * (src/meshes/p4est_cubed_sphere.jl:47-79,
+ * src/solvers/dgsem_p4est/containers_2d_manifold_in_3d_covariant.jl:232-489). */
+int atmo_mesh2d_cubed_sphere(int polydeg, int trees_per_face_dimension, double radius,
+                             int32_t christoffel_collocation, atmo_mesh2d** mesh);
+int atmo_mesh2d_sizes(const atmo_mesh2d* mesh, int32_t* n_elements, int32_t* n_interfaces);
+/* node_coordinates [3,n,n,E], aux_node_vars [26,n,n,E],
+ * interface_neighbor_ids [2,I], interface_node_indices [2,2,I] (2D codes
+ * begin/end/i_forward/i_backward) */
+int atmo_mesh2d_arrays(const atmo_mesh2d* mesh, const double** node_coordinates,
+                       const double** aux_node_vars, const int32_t** interface_neighbor_ids,
+                       const int32_t** interface_node_indices);
+void atmo_mesh2d_destroy(atmo_mesh2d* mesh);
+
 /* ---------------------------------------------- semidiscretization */
 typedef struct atmo_handle atmo_handle;
 
@@ -148,6 +171,50 @@ typedef struct {
   const int32_t* boundary_node_indices;  /* [3,B] */
   const int32_t* boundary_condition;     /* [B] ATMO_BC_* */
 } atmo_geometry3d;
+
+/* covariant shallow water: fluxes */
+#define ATMO_SWE_VOL_EC 0                  /* flux_ec */
+#define ATMO_SWE_VOL_CENTRAL 1             /* flux_central */
+#define ATMO_SWE_SURF_EC_PLUS_LLF 0        /* FluxPlusDissipation(flux_ec, DissipationLocalLaxFriedrichs()) */
+#define ATMO_SWE_SURF_LAX_FRIEDRICHS 1     /* flux_lax_friedrichs */
+#define ATMO_SWE_SURF_EC 2                 /* flux_ec */
+#define ATMO_SWE_NC_SURFACE_SIMPLIFIED 0   /* flux_nonconservative_surface_simplified */
+#define ATMO_SWE_NC_EC 1                   /* flux_nonconservative_ec */
+
+typedef struct {
+  int32_t polydeg;          /* 1..7 */
+  int32_t n_elements;
+  int32_t volume_flux;      /* ATMO_SWE_VOL_*; the volume nonconservative term is flux_nonconservative_ec */
+  int32_t surface_flux;     /* ATMO_SWE_SURF_* */
+  int32_t surface_noncons;  /* ATMO_SWE_NC_* */
+  int32_t source_terms;     /* 1: source_terms_geometric_coriolis */
+  int32_t precision;        /* ATMO_PREC_FP64 */
+  double gravity, rotation_rate; /* SplitCovariantShallowWaterEquations2D(gravity, rotation_rate) */
+} atmo_swe_config;
+
+typedef struct {
+  const double* aux_node_vars;           /* [26,n,n,E] */
+  const double* node_coordinates;        /* [3,n,n,E] */
+  int32_t n_interfaces;
+  const int32_t* interface_neighbor_ids; /* [2,I] */
+  const int32_t* interface_node_indices; /* [2,2,I] */
+  /* ensemble: members own contiguous element ranges
+   * [member_first_element[m], member_first_element[m+1]); NULL: one member */
+  int32_t n_members;
+  const int32_t* member_first_element;   /* [n_members + 1] */
+} atmo_geometry2d;
+
+/* SemidiscretizationHyperbolic for the covariant form on P4estMesh{2}
+ * (src/semidiscretization/semidiscretization_hyperbolic_2d_manifold_in_3d.jl:78-111,
+ * cache of src/solvers/dgsem_p4est/dg.jl:5-32); an ensemble of independent
+ * members is one handle whose stage launches cover all members. */
+int atmo_swe_create(const atmo_swe_config* config, const atmo_geometry2d* geometry,
+                    atmo_handle** handle);
+
+/* ensemble variants: per-member max_dt (dt[n_members]) and RK steps with
+ * per-member time steps */
+int atmo_max_dt_members(atmo_handle* handle, double cfl, double* dt);
+int atmo_step_members(atmo_handle* handle, const double* dt, int32_t n_steps);
 
 /* SemidiscretizationHyperbolic(mesh, equations, initial_condition, solver;
  * boundary_conditions) for P4estMesh{3}: copies geometry and connectivity to
@@ -184,7 +251,7 @@ int atmo_max_dt(atmo_handle* handle, double cfl, double* dt);
 /* Integrals on the resident state (analysis_covariant.jl:53-105 analogues):
  * out[0] mass, out[1] entropy, out[2] dS/dt = int w(u).du of the last
  * computed rhs (or of a fresh one when `fresh_rhs`), out[3..] integral of each
- * conserved variable. */
+ * conserved variable (3D Euler: out[0..8]; covariant shallow water: out[0..2]). */
 int atmo_integrals(atmo_handle* handle, int32_t fresh_rhs, double* out);
 
 /* wait for all work of the handle's stream */

@@ -119,6 +119,36 @@ void ora_e3_integrals(const ora_e3_params* p, const double* inverse_jacobian,
 /* CarpenterKennedy2N54 coefficients A[5], B[5], c[5] */
 void ora_ck2n54_coefficients(double* A, double* B, double* c);
 
+/* ---- covariant shallow water (SplitCovariantShallowWaterEquations2D) on
+ *      2D P4est meshes of the sphere; state (h, h v^1, h v^2), 26 auxiliary
+ *      variables per node, node coordinates [3, n, n, E] ---- */
+typedef struct {
+  int n, n_elements;
+  double gravity, rotation_rate;
+  int volume_flux;     /* 0 flux_ec, 1 flux_central */
+  int surface_flux;    /* 0 FluxPlusDissipation(flux_ec, DissipationLocalLaxFriedrichs()),
+                          1 flux_lax_friedrichs, 2 flux_ec */
+  int surface_noncons; /* 0 flux_nonconservative_surface_simplified, 1 flux_nonconservative_ec */
+  int source_terms;    /* 1: source_terms_geometric_coriolis */
+} ora_swe_params;
+
+typedef struct {
+  int n_interfaces;
+  const int* neighbor_ids; /* [2, I] */
+  const int* node_indices; /* [2, 2, I] 2D codes: 0 begin, 1 end, 2 i_forward, 3 i_backward */
+} ora_connectivity2d;
+
+int ora_swe_rhs(const ora_swe_params* p, const ora_connectivity2d* c, const double* aux,
+                const double* xyz, const double* u, double* du);
+/* sum of |contribution| for every entry of du (rounding-error scale) */
+int ora_swe_rhs_magnitude(const ora_swe_params* p, const ora_connectivity2d* c, const double* aux,
+                          const double* xyz, const double* u, double* du_abs);
+double ora_swe_max_dt(const ora_swe_params* p, const double* aux, const double* u);
+int ora_swe_ck2n54(const ora_swe_params* p, const ora_connectivity2d* c, const double* aux,
+                   const double* xyz, double* u, double dt, int n_steps);
+void ora_swe_integrals(const ora_swe_params* p, const double* aux, const double* u,
+                       const double* du, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,17 @@ class Connectivity(ctypes.Structure):
                 ("bnd_kind", ip)]
 
 
+class SweParams(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("n_elements", ctypes.c_int), ("gravity", ctypes.c_double),
+                ("rotation_rate", ctypes.c_double), ("volume_flux", ctypes.c_int),
+                ("surface_flux", ctypes.c_int), ("surface_noncons", ctypes.c_int),
+                ("source_terms", ctypes.c_int)]
+
+
+class Connectivity2D(ctypes.Structure):
+    _fields_ = [("n_interfaces", ctypes.c_int), ("neighbor_ids", ip), ("node_indices", ip)]
+
+
 def _stale():
     t = os.path.getmtime(ORACLE_LIB)
     return any(os.path.getmtime(os.path.join(ORACLE_DIR, f)) > t
@@ -57,6 +68,7 @@ class Oracle:
         L.ora_e3_max_abs_speed.restype = ctypes.c_double
         L.ora_e3_entropy.restype = ctypes.c_double
         L.ora_e3_max_dt.restype = ctypes.c_double
+        L.ora_swe_max_dt.restype = ctypes.c_double
         self.L = L
 
     # ---- basis
@@ -177,6 +189,89 @@ class Oracle:
         dptr = _d(np.ascontiguousarray(du)) if du is not None else None
         self.L.ora_e3_integrals(ctypes.byref(p), _d(ij), _d(uf), dptr, _d(out))
         return out
+
+
+class SweOracle:
+    """The covariant shallow-water oracle bound to one mesh / aux set:
+    nbr [2,I], idx [2,2,I], aux [26,n,n,E], xyz [3,n,n,E] (F-ordered)."""
+
+    def __init__(self, oracle, params, aux, xyz, nbr, idx):
+        self.L, self.p = oracle.L, params
+        f = lambda a, t: np.ascontiguousarray(np.asarray(a).ravel(order="F"), dtype=t)
+        self._keep = [f(aux, float), f(xyz, float), f(nbr, np.int32), f(idx, np.int32)]
+        self.c = Connectivity2D(int(np.asarray(nbr).shape[1]), _i(self._keep[2]),
+                                _i(self._keep[3]))
+
+    @classmethod
+    def for_semi(cls, oracle, semi):
+        return cls.from_parts(oracle, semi.mesh, semi.aux_node_vars, semi.equations, semi.solver,
+                              semi.source_terms)
+
+    @classmethod
+    def from_parts(cls, oracle, mesh, aux, equations, solver, source_terms):
+        codes = swe_codes(solver, source_terms)
+        p = SweParams(mesh.nnodes, mesh.n_elements, equations.gravity, equations.rotation_rate,
+                      *codes)
+        return cls(oracle, p, aux, mesh.node_coordinates, mesh.interface_neighbor_ids,
+                   mesh.interface_node_indices)
+
+    def _u(self, u):
+        return np.ascontiguousarray(np.asarray(u).ravel(order="F"), dtype=float)
+
+    def rhs(self, u):
+        uf = self._u(u)
+        du = np.zeros_like(uf)
+        assert self.L.ora_swe_rhs(ctypes.byref(self.p), ctypes.byref(self.c), _d(self._keep[0]),
+                                  _d(self._keep[1]), _d(uf), _d(du)) == 0
+        return du
+
+    def rhs_magnitude(self, u):
+        uf = self._u(u)
+        du = np.zeros_like(uf)
+        assert self.L.ora_swe_rhs_magnitude(ctypes.byref(self.p), ctypes.byref(self.c),
+                                            _d(self._keep[0]), _d(self._keep[1]), _d(uf),
+                                            _d(du)) == 0
+        return du
+
+    def max_dt(self, u):
+        return self.L.ora_swe_max_dt(ctypes.byref(self.p), _d(self._keep[0]), _d(self._u(u)))
+
+    def ck2n54_steps(self, u, dt, n_steps):
+        uf = self._u(u).copy()
+        assert self.L.ora_swe_ck2n54(ctypes.byref(self.p), ctypes.byref(self.c),
+                                     _d(self._keep[0]), _d(self._keep[1]), _d(uf),
+                                     ctypes.c_double(dt), n_steps) == 0
+        return uf
+
+    def integrals(self, u, du=None):
+        out = np.zeros(3)
+        dptr = _d(np.ascontiguousarray(du)) if du is not None else None
+        self.L.ora_swe_integrals(ctypes.byref(self.p), _d(self._keep[0]), _d(self._u(u)), dptr,
+                                 _d(out))
+        return out
+
+    def solve(self, u0, t_end, cfl):
+        """StepsizeCallback(cfl) + CarpenterKennedy2N54 to t_end (the last
+        step cut to end at t_end)."""
+        u = self._u(u0).copy()
+        t, steps = 0.0, 0
+        while t < t_end:
+            dt = min(cfl * self.max_dt(u), t_end - t)
+            u = self.ck2n54_steps(u, dt, 1)
+            t += dt
+            steps += 1
+        return u, steps
+
+
+def swe_codes(solver, source_terms):
+    """(volume_flux, surface_flux, surface_noncons, source_terms) codes of a
+    covariant DGSEM solver"""
+    from trixiatmo_hip.equations import SWE_SURFACE_FLUXES, SWE_SURFACE_NONCONS, SWE_VOLUME_FLUXES
+    from trixiatmo_hip.semidiscretization import COVARIANT_SOURCE_TERMS
+    vol, _ = solver.volume_integral.volume_flux
+    sf, snc = solver.surface_flux
+    return (SWE_VOLUME_FLUXES[vol], SWE_SURFACE_FLUXES[sf], SWE_SURFACE_NONCONS[snc],
+            COVARIANT_SOURCE_TERMS[source_terms])
 
 
 def params_for(semi_args):

@@ -98,3 +98,81 @@ def test_mesh_geometry_closed_forms():
     assert s.n_elements == 6 * 16 * 3
     assert s.n_boundaries == 2 * 6 * 16
     assert s.n_interfaces == (6 * 16 * 3 * 6 - s.n_boundaries) // 2
+
+
+def _swe_geometry(mesh, aux, ids=None, idx=None, n_if=None, first=None):
+    keep = [np.ascontiguousarray(aux.ravel(order="F")),
+            np.ascontiguousarray(mesh.node_coordinates.ravel(order="F")),
+            np.ascontiguousarray((mesh.interface_neighbor_ids if ids is None else ids)
+                                 .ravel(order="F"), dtype=np.int32),
+            np.ascontiguousarray((mesh.interface_node_indices if idx is None else idx)
+                                 .ravel(order="F"), dtype=np.int32)]
+    geo = _lib.AtmoGeometry2D(aux_node_vars=_lib.dptr(keep[0]), node_coordinates=_lib.dptr(keep[1]),
+                              n_interfaces=mesh.n_interfaces if n_if is None else n_if,
+                              interface_neighbor_ids=_lib.iptr(keep[2]),
+                              interface_node_indices=_lib.iptr(keep[3]), n_members=0,
+                              member_first_element=None)
+    if first is not None:
+        keep.append(np.asarray(first, dtype=np.int32))
+        geo.n_members = len(first) - 1
+        geo.member_first_element = _lib.iptr(keep[-1])
+    return geo, keep
+
+
+def test_covariant_invalid_configurations_and_connectivity_rejected():
+    mesh = T.P4estMeshCubedSphere2D(2, 1.0, polydeg=3)
+    aux = mesh.data(False).aux_node_vars
+    good = dict(polydeg=3, n_elements=mesh.n_elements, volume_flux=0, surface_flux=0,
+                surface_noncons=0, source_terms=1, precision=0, gravity=9.81,
+                rotation_rate=7.292e-5)
+    bad = [dict(polydeg=8), dict(volume_flux=2), dict(surface_flux=3), dict(surface_noncons=2),
+           dict(precision=1), dict(gravity=0.0), dict(n_elements=0), dict(source_terms=2)]
+    geo, keep = _swe_geometry(mesh, aux)
+    for kw in bad:
+        cfg = _lib.AtmoSweConfig(**{**good, **kw})
+        h = ctypes.c_void_p()
+        assert _lib.lib().atmo_swe_create(ctypes.byref(cfg), ctypes.byref(geo), ctypes.byref(h)) == -1, kw
+        assert h.value is None
+    cfg = _lib.AtmoSweConfig(**good)
+    ids = mesh.interface_neighbor_ids.copy()
+    idx = mesh.interface_node_indices.copy()
+    i_bad = ids.copy()
+    i_bad[0, 0] = mesh.n_elements
+    x_bad = idx.copy()
+    x_bad[0, 0, 0] = 7
+    cases = [dict(n_if=mesh.n_interfaces - 1), dict(ids=i_bad), dict(idx=x_bad),
+             # members must cover the elements, and no interface may join two members
+             dict(first=[0, mesh.n_elements - 1]), dict(first=[0, 12, mesh.n_elements])]
+    for kw in cases:
+        geo, keep = _swe_geometry(mesh, aux, **kw)
+        h = ctypes.c_void_p()
+        assert _lib.lib().atmo_swe_create(ctypes.byref(cfg), ctypes.byref(geo), ctypes.byref(h)) == -1, kw
+        msg = _lib.lib().atmo_last_error(None)
+        assert msg, kw
+
+
+def test_covariant_mesh_connectivity_closed_forms():
+    """6 k^2 elements, 12 k^2 interfaces, each element face used once; the
+    node coordinates of the two sides of every interface coincide."""
+    for k in (1, 2, 3):
+        mesh = T.P4estMeshCubedSphere2D(k, 2.0, polydeg=2)
+        assert mesh.n_elements == 6 * k * k and mesh.n_interfaces == 12 * k * k
+        n = 3
+        used = set()
+        x = mesh.node_coordinates
+
+        def node(code, kk):
+            return {0: 0, 1: n - 1, 2: kk, 3: n - 1 - kk}[code]
+
+        for I in range(mesh.n_interfaces):
+            ep, es = mesh.interface_neighbor_ids[:, I]
+            ip, isx = mesh.interface_node_indices[:, 0, I], mesh.interface_node_indices[:, 1, I]
+            for side, e, c in ((0, ep, ip), (1, es, isx)):
+                face = (int(e), int(c[0]) if c[0] <= 1 else 2 + int(c[1]))
+                assert face not in used
+                used.add(face)
+            for kk in range(n):
+                a = x[:, node(ip[0], kk), node(ip[1], kk), ep]
+                b = x[:, node(isx[0], kk), node(isx[1], kk), es]
+                np.testing.assert_allclose(a, b, atol=1e-12)
+        assert len(used) == 4 * mesh.n_elements

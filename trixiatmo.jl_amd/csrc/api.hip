@@ -12,6 +12,7 @@
 #include "../../include/atmo_hip.h"
 #include "atmo_internal.hpp"
 #include "euler3d_kernels.hpp"
+#include "handle.hpp"
 
 namespace atmo {
 void build_cartesian(Mesh3D& m, const Basis& b, const int trees[3], const double xmin[3],
@@ -22,50 +23,15 @@ void build_cubed_sphere_shell(Mesh3D& m, const Basis& b, int trees_per_face, int
 
 using namespace atmo;
 
-namespace {
-
-thread_local std::string g_error;
-
-int fail(std::string* slot, int code, const std::string& msg) {
-  if (slot) *slot = msg;
-  g_error = msg;
-  return code;
+namespace atmo {
+std::string& global_error() {
+  thread_local std::string e;
+  return e;
 }
-
-struct TimerPair {
-  hipEvent_t a, b;
-  int cls;
-};
-
-}  // namespace
+}  // namespace atmo
 
 struct atmo_mesh3d {
   Mesh3D m;
-};
-
-struct atmo_handle {
-  atmo_config cfg;
-  int n = 0, n3 = 0, nv = 6;
-  int64_t len = 0;  // doubles per state
-  int device = 0;
-  hipStream_t stream = nullptr;
-  double *d_u = nullptr, *d_u2 = nullptr, *d_k = nullptr, *d_du = nullptr;
-  double *d_uio = nullptr, *d_duio = nullptr;  // staging for atmo_rhs with host arrays
-  double *d_cv = nullptr, *d_invJ = nullptr;
-  float* d_cv32 = nullptr;  // FP32 geometry of the mixed-precision path
-  bool bg_on = false;       // hydrostatic background split (mixed path)
-  double *d_ubg = nullptr, *d_dubg = nullptr;
-  float* d_B = nullptr;
-  std::vector<double> dsplit, weights;
-  int* d_nbr = nullptr;
-  int* d_face_info = nullptr;
-  double *d_partial = nullptr, *d_result = nullptr;
-  bool du_valid = false;
-  bool timing = false;
-  std::vector<TimerPair> timers;
-  int64_t launches[2] = {0, 0};
-  double total_ms[2] = {0, 0};
-  std::string err;
 };
 
 namespace {
@@ -99,14 +65,11 @@ inline void face_ab(int dir, int i, int j, int k, int* a, int* b) {
   *b = d == 2 ? j : k;
 }
 
-#define HIP_TRY(h, expr)                                                         \
-  do {                                                                           \
-    hipError_t e__ = (expr);                                                     \
-    if (e__ != hipSuccess)                                                       \
-      return fail((h) ? &(h)->err : nullptr, ATMO_ERR_DEVICE,                    \
-                  std::string(#expr) + ": " + hipGetErrorString(e__));           \
-  } while (0)
 
+
+}  // namespace
+
+namespace atmo {
 void timer_begin(atmo_handle* h, int cls, TimerPair* tp) {
   tp->cls = cls;
   hipEventCreate(&tp->a);
@@ -118,6 +81,9 @@ void timer_end(atmo_handle* h, TimerPair* tp) {
   hipEventRecord(tp->b, h->stream);
   h->timers.push_back(*tp);
 }
+}  // namespace atmo
+
+namespace {
 
 void collect_timers(atmo_handle* h) {
   for (auto& tp : h->timers) {
@@ -286,7 +252,7 @@ int atmo_mesh3d_arrays(const atmo_mesh3d* mh, const double** xyz, const double**
 void atmo_mesh3d_destroy(atmo_mesh3d* mh) { delete mh; }
 
 const char* atmo_last_error(const atmo_handle* h) {
-  return h ? h->err.c_str() : g_error.c_str();
+  return h ? h->err.c_str() : global_error().c_str();
 }
 
 int atmo_create(const atmo_config* cfg, const atmo_geometry3d* g, atmo_handle** out) {
@@ -450,6 +416,11 @@ void atmo_destroy(atmo_handle* h) {
   if (h->d_ubg) hipFree(h->d_ubg);
   if (h->d_dubg) hipFree(h->d_dubg);
   if (h->d_B) hipFree(h->d_B);
+  double* swe_bufs[] = {h->d_aux, h->d_xyz, h->d_dt_member};
+  for (double* p : swe_bufs)
+    if (p) hipFree(p);
+  if (h->d_member) hipFree(h->d_member);
+  if (h->d_member_first) hipFree(h->d_member_first);
   if (h->d_face_info) hipFree(h->d_face_info);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -462,9 +433,14 @@ int atmo_rhs(atmo_handle* h, double* du, const double* u, double t) {
   if (!h || !du || !u) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
   const size_t sb = sizeof(double) * h->len;
   HIP_TRY(h, hipMemcpyAsync(h->d_uio, u, sb, hipMemcpyHostToDevice, h->stream));
-  int rc = compute_background(h, h->d_uio);
-  if (rc != ATMO_OK) return rc;
-  rc = run_rhs(h, h->d_duio, h->d_uio);
+  int rc = ATMO_OK;
+  if (h->eq == ATMO_EQ_COVARIANT_SWE_2D) {
+    rc = swe_rhs_device(h, h->d_duio, h->d_uio);
+  } else {
+    rc = compute_background(h, h->d_uio);
+    if (rc != ATMO_OK) return rc;
+    rc = run_rhs(h, h->d_duio, h->d_uio);
+  }
   if (rc != ATMO_OK) return rc;
   HIP_TRY(h, hipMemcpyAsync(du, h->d_duio, sb, hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
@@ -474,6 +450,7 @@ int atmo_rhs(atmo_handle* h, double* du, const double* u, double t) {
 int atmo_rhs_device(atmo_handle* h, double* du, const double* u, double t) {
   (void)t;
   if (!h || !du || !u) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
+  if (h->eq == ATMO_EQ_COVARIANT_SWE_2D) return swe_rhs_device(h, du, u);
   int rc = compute_background(h, u);
   if (rc != ATMO_OK) return rc;
   return run_rhs(h, du, u);
@@ -500,6 +477,7 @@ int atmo_get_state(atmo_handle* h, double* u) {
 int atmo_step(atmo_handle* h, double dt, int32_t n_steps) {
   if (!h || n_steps < 0 || !(dt > 0) || !std::isfinite(dt))
     return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "bad step arguments");
+  if (h->eq == ATMO_EQ_COVARIANT_SWE_2D) return swe_step(h, nullptr, dt, n_steps);
   // Carpenter & Kennedy (1994) five-stage fourth-order 2N coefficients
   static const double A[5] = {0.0, -567301805773.0 / 1357537059087.0,
                               -2404267990393.0 / 2016746695238.0,
@@ -530,6 +508,15 @@ int atmo_step(atmo_handle* h, double dt, int32_t n_steps) {
 
 int atmo_max_dt(atmo_handle* h, double cfl, double* dt) {
   if (!h || !dt) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
+  if (h->eq == ATMO_EQ_COVARIANT_SWE_2D) {  // the smallest member time step
+    std::vector<double> dtm(h->n_members);
+    const int rc = swe_max_dt(h, cfl, dtm.data());
+    if (rc != ATMO_OK) return rc;
+    double m = dtm[0];
+    for (double v : dtm) m = std::fmin(m, v);
+    *dt = m;
+    return ATMO_OK;
+  }
   E3Launch L = base_launch(h);
   L.u = h->d_u;
   TimerPair tp;
@@ -548,6 +535,7 @@ int atmo_max_dt(atmo_handle* h, double cfl, double* dt) {
 
 int atmo_integrals(atmo_handle* h, int32_t fresh_rhs, double* out) {
   if (!h || !out) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
+  if (h->eq == ATMO_EQ_COVARIANT_SWE_2D) return swe_integrals(h, fresh_rhs, out);
   if (fresh_rhs) {
     int rc = run_rhs(h, h->d_du, h->d_u);
     if (rc != ATMO_OK) return rc;

@@ -1,0 +1,319 @@
+// C ABI of libatmo_hip for the covariant shallow-water form: the 2D cubed
+// sphere mesh, handle creation and the covariant paths of the generic entry
+// points (rhs!, RK steps, max_dt, integrals), including ensembles of
+// independent members advanced by the same launches.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "atmo_internal.hpp"
+#include "handle.hpp"
+#include "swe_kernels.hpp"
+
+using namespace atmo;
+
+struct atmo_mesh2d {
+  Mesh2D m;
+};
+
+namespace {
+
+inline int code2d(int code, int i, int n) {
+  switch (code) {
+    case kBegin: return 0;
+    case kEnd: return n - 1;
+    case kIFwd: return i;
+    default: return n - 1 - i;
+  }
+}
+
+// face 0 (-x) .. 3 (+y) from 2D node-index codes; -1 if malformed
+int dir2d(const int32_t* idx) {
+  const bool f0 = idx[0] == kBegin || idx[0] == kEnd;
+  const bool f1 = idx[1] == kBegin || idx[1] == kEnd;
+  const bool s0 = idx[0] == kIFwd || idx[0] == kIBwd;
+  const bool s1 = idx[1] == kIFwd || idx[1] == kIBwd;
+  if (f0 && s1) return idx[0] == kBegin ? 0 : 1;
+  if (f1 && s0) return idx[1] == kBegin ? 2 : 3;
+  return -1;
+}
+
+int launch(atmo_handle* h, const SweLaunch& L) {
+  TimerPair tp;
+  if (h->timing) timer_begin(h, 0, &tp);
+  const bool ok = swe_launch_stage(L, h->stream);
+  if (h->timing) timer_end(h, &tp);
+  if (!ok) return fail(&h->err, ATMO_ERR_INVALID, "no covariant kernel for this polydeg");
+  HIP_TRY(h, hipGetLastError());
+  return ATMO_OK;
+}
+
+}  // namespace
+
+namespace atmo {
+
+int swe_rhs_device(atmo_handle* h, double* du, const double* u) {
+  SweLaunch L = h->swe_base;
+  L.mode = 0;
+  L.u = u;
+  L.out = du;
+  return launch(h, L);
+}
+
+int swe_step(atmo_handle* h, const double* dt_members, double dt, int n_steps) {
+  static const double A[5] = {0.0, -567301805773.0 / 1357537059087.0,
+                              -2404267990393.0 / 2016746695238.0,
+                              -3550918686646.0 / 2091501179385.0,
+                              -1275806237668.0 / 842570457699.0};
+  static const double Bc[5] = {1432997174477.0 / 9575080441755.0,
+                               5161836677717.0 / 13612068292357.0,
+                               1720146321549.0 / 2090206949498.0,
+                               3134564353537.0 / 4481467310338.0,
+                               2277821191437.0 / 14882151754819.0};
+  SweLaunch L = h->swe_base;
+  L.mode = 1;
+  L.k = h->d_k;
+  L.rk_dt = dt;
+  if (dt_members) {
+    HIP_TRY(h, hipMemcpyAsync(h->d_dt_member, dt_members, sizeof(double) * h->n_members,
+                              hipMemcpyHostToDevice, h->stream));
+    L.dt_member = h->d_dt_member;
+  }
+  for (int s = 0; s < n_steps; ++s)
+    for (int st = 0; st < 5; ++st) {
+      L.u = h->d_u;
+      L.out = h->d_u2;
+      L.rk_a = A[st];
+      L.rk_b = Bc[st];
+      const int rc = launch(h, L);
+      if (rc != ATMO_OK) return rc;
+      std::swap(h->d_u, h->d_u2);
+    }
+  h->du_valid = false;
+  return ATMO_OK;
+}
+
+int swe_max_dt(atmo_handle* h, double cfl, double* dt_members) {
+  SweLaunch L = h->swe_base;
+  L.u = h->d_u;
+  TimerPair tp;
+  if (h->timing) timer_begin(h, 1, &tp);
+  const bool ok = swe_launch_member_max_speed(L, h->d_partial, h->d_member_first, h->n_members,
+                                              h->d_result, h->stream);
+  if (h->timing) timer_end(h, &tp);
+  if (!ok) return fail(&h->err, ATMO_ERR_INVALID, "no covariant kernel for this polydeg");
+  HIP_TRY(h, hipGetLastError());
+  std::vector<double> mx(h->n_members);
+  HIP_TRY(h, hipMemcpyAsync(mx.data(), h->d_result, sizeof(double) * h->n_members,
+                            hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  for (int m = 0; m < h->n_members; ++m) {
+    if (!std::isfinite(mx[m])) return fail(&h->err, ATMO_ERR_NONFINITE, "non-finite wave speed");
+    dt_members[m] = cfl * 2.0 / (h->n * mx[m]);
+  }
+  return ATMO_OK;
+}
+
+int swe_integrals(atmo_handle* h, int fresh_rhs, double* out) {
+  if (fresh_rhs) {
+    const int rc = swe_rhs_device(h, h->d_du, h->d_u);
+    if (rc != ATMO_OK) return rc;
+    h->du_valid = true;
+  }
+  SweLaunch L = h->swe_base;
+  L.u = h->d_u;
+  // LGL weights live after the dt_member array
+  const double* w = h->d_dt_member + h->n_members;
+  if (!swe_launch_integrals(L, h->du_valid ? h->d_du : nullptr, w, h->d_partial, h->d_result,
+                            h->stream))
+    return fail(&h->err, ATMO_ERR_INVALID, "no covariant kernel for this polydeg");
+  HIP_TRY(h, hipGetLastError());
+  HIP_TRY(h, hipMemcpyAsync(out, h->d_result, sizeof(double) * 3, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return ATMO_OK;
+}
+
+}  // namespace atmo
+
+extern "C" {
+
+int atmo_mesh2d_cubed_sphere(int polydeg, int trees_per_face_dimension, double radius,
+                             int32_t christoffel_collocation, atmo_mesh2d** out) {
+  if (!out) return fail(nullptr, ATMO_ERR_INVALID, "null argument");
+  if (polydeg < 1 || polydeg > 15 || trees_per_face_dimension < 1 || !(radius > 0))
+    return fail(nullptr, ATMO_ERR_INVALID, "bad cubed-sphere parameters");
+  auto* mh = new atmo_mesh2d;
+  mh->m.polydeg = polydeg;
+  mh->m.n = polydeg + 1;
+  Basis b(polydeg + 1);
+  std::string err;
+  if (!build_cubed_sphere_2d(mh->m, b, trees_per_face_dimension, radius,
+                             christoffel_collocation ? 1 : 0, &err)) {
+    delete mh;
+    return fail(nullptr, ATMO_ERR_MESH, err);
+  }
+  *out = mh;
+  return ATMO_OK;
+}
+
+int atmo_mesh2d_sizes(const atmo_mesh2d* mh, int32_t* ne, int32_t* ni) {
+  if (!mh) return fail(nullptr, ATMO_ERR_INVALID, "null mesh");
+  if (ne) *ne = mh->m.n_elements;
+  if (ni) *ni = mh->m.n_interfaces;
+  return ATMO_OK;
+}
+
+int atmo_mesh2d_arrays(const atmo_mesh2d* mh, const double** xyz, const double** aux,
+                       const int32_t** ids, const int32_t** idx) {
+  if (!mh) return fail(nullptr, ATMO_ERR_INVALID, "null mesh");
+  if (xyz) *xyz = mh->m.node_coordinates.data();
+  if (aux) *aux = mh->m.aux.data();
+  if (ids) *ids = mh->m.interface_neighbor_ids.data();
+  if (idx) *idx = mh->m.interface_node_indices.data();
+  return ATMO_OK;
+}
+
+void atmo_mesh2d_destroy(atmo_mesh2d* mh) { delete mh; }
+
+int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_handle** out) {
+  if (!cfg || !g || !out) return fail(nullptr, ATMO_ERR_INVALID, "null argument");
+  if (cfg->polydeg < 1 || cfg->polydeg > 7)
+    return fail(nullptr, ATMO_ERR_INVALID, "polydeg must be in 1..7");
+  if (cfg->n_elements < 1) return fail(nullptr, ATMO_ERR_INVALID, "n_elements must be positive");
+  if (cfg->volume_flux < 0 || cfg->volume_flux > 1 || cfg->surface_flux < 0 ||
+      cfg->surface_flux > 2 || cfg->surface_noncons < 0 || cfg->surface_noncons > 1 ||
+      cfg->source_terms < 0 || cfg->source_terms > 1)
+    return fail(nullptr, ATMO_ERR_INVALID, "unsupported flux or source choice");
+  if (cfg->precision != ATMO_PREC_FP64)
+    return fail(nullptr, ATMO_ERR_INVALID, "the covariant form runs in FP64");
+  if (!(cfg->gravity > 0)) return fail(nullptr, ATMO_ERR_INVALID, "gravity must be positive");
+  if (!g->aux_node_vars || !g->node_coordinates || g->n_interfaces < 0 ||
+      (g->n_interfaces > 0 && (!g->interface_neighbor_ids || !g->interface_node_indices)))
+    return fail(nullptr, ATMO_ERR_INVALID, "missing geometry arrays");
+  const int n = cfg->polydeg + 1, n2 = n * n, E = cfg->n_elements;
+  const int M = g->member_first_element ? g->n_members : 1;
+  std::vector<int32_t> first(M + 1), member(E);
+  if (g->member_first_element) {
+    if (M < 1) return fail(nullptr, ATMO_ERR_INVALID, "n_members must be positive");
+    for (int m = 0; m <= M; ++m) first[m] = g->member_first_element[m];
+    if (first[0] != 0 || first[M] != E)
+      return fail(nullptr, ATMO_ERR_INVALID, "member ranges must cover all elements");
+    for (int m = 0; m < M; ++m) {
+      if (first[m + 1] <= first[m]) return fail(nullptr, ATMO_ERR_INVALID, "empty member range");
+      for (int e = first[m]; e < first[m + 1]; ++e) member[e] = m;
+    }
+  } else {
+    first[0] = 0;
+    first[1] = E;
+  }
+  std::vector<int32_t> nbr((size_t)E * 4 * n, INT32_MIN);
+  for (int I = 0; I < g->n_interfaces; ++I) {
+    const int ep = g->interface_neighbor_ids[2 * I], es = g->interface_neighbor_ids[2 * I + 1];
+    const int32_t* ip = g->interface_node_indices + 4 * I;
+    const int32_t* is = ip + 2;
+    const int dp = dir2d(ip), ds = dir2d(is);
+    if (ep < 0 || ep >= E || es < 0 || es >= E || dp < 0 || ds < 0 ||
+        member[ep] != member[es])
+      return fail(nullptr, ATMO_ERR_INVALID, "malformed interface " + std::to_string(I));
+    for (int k = 0; k < n; ++k) {
+      const int pi = code2d(ip[0], k, n), pj = code2d(ip[1], k, n);
+      const int si = code2d(is[0], k, n), sj = code2d(is[1], k, n);
+      const int kp = dp < 2 ? pj : pi, ks = ds < 2 ? sj : si;
+      int32_t& a = nbr[((size_t)ep * 4 + dp) * n + kp];
+      int32_t& b = nbr[((size_t)es * 4 + ds) * n + ks];
+      if (a != INT32_MIN || b != INT32_MIN)
+        return fail(nullptr, ATMO_ERR_INVALID, "face used twice (interface " + std::to_string(I) + ")");
+      a = es * n2 + si + n * sj;
+      b = ep * n2 + pi + n * pj;
+    }
+  }
+  for (auto v : nbr)
+    if (v == INT32_MIN) return fail(nullptr, ATMO_ERR_INVALID, "element face without interface");
+
+  auto* h = new atmo_handle;
+  h->eq = ATMO_EQ_COVARIANT_SWE_2D;
+  h->swe = *cfg;
+  h->n = n;
+  h->n3 = n2;
+  h->nv = 3;
+  h->len = (int64_t)3 * n2 * E;
+  h->n_members = M;
+  std::memset(&h->cfg, 0, sizeof(h->cfg));
+  h->cfg.n_elements = E;
+  h->cfg.polydeg = cfg->polydeg;
+  auto cleanup_fail = [&](int code, const char* msg) {
+    fail(nullptr, code, msg);
+    atmo_destroy(h);
+    return code;
+  };
+  if (hipGetDevice(&h->device) != hipSuccess) return cleanup_fail(ATMO_ERR_DEVICE, "no HIP device");
+  Basis b(n);
+  const size_t sb = sizeof(double) * h->len;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&h->d_u, sb) != hipSuccess || hipMalloc(&h->d_u2, sb) != hipSuccess ||
+      hipMalloc(&h->d_k, sb) != hipSuccess || hipMalloc(&h->d_du, sb) != hipSuccess ||
+      hipMalloc(&h->d_uio, sb) != hipSuccess || hipMalloc(&h->d_duio, sb) != hipSuccess ||
+      hipMalloc(&h->d_aux, sizeof(double) * 26 * n2 * (size_t)E) != hipSuccess ||
+      hipMalloc(&h->d_xyz, sizeof(double) * 3 * n2 * (size_t)E) != hipSuccess ||
+      hipMalloc(&h->d_nbr, sizeof(int32_t) * nbr.size()) != hipSuccess ||
+      hipMalloc(&h->d_member, sizeof(int32_t) * E) != hipSuccess ||
+      hipMalloc(&h->d_member_first, sizeof(int32_t) * (M + 1)) != hipSuccess ||
+      hipMalloc(&h->d_dt_member, sizeof(double) * (M + n)) != hipSuccess ||
+      hipMalloc(&h->d_partial, sizeof(double) * 3 * (size_t)E) != hipSuccess ||
+      hipMalloc(&h->d_result, sizeof(double) * (M > 16 ? M : 16)) != hipSuccess)
+    return cleanup_fail(ATMO_ERR_DEVICE, "device allocation failed");
+  if (hipMemcpy(h->d_aux, g->aux_node_vars, sizeof(double) * 26 * n2 * (size_t)E,
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->d_xyz, g->node_coordinates, sizeof(double) * 3 * n2 * (size_t)E,
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->d_nbr, nbr.data(), sizeof(int32_t) * nbr.size(), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(h->d_member, member.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(h->d_member_first, first.data(), sizeof(int32_t) * (M + 1),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->d_dt_member + M, b.w.data(), sizeof(double) * n, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemset(h->d_u, 0, sb) != hipSuccess || hipMemset(h->d_k, 0, sb) != hipSuccess)
+    return cleanup_fail(ATMO_ERR_DEVICE, "device upload failed");
+  SweLaunch& L = h->swe_base;
+  L.n = n;
+  L.n_elements = E;
+  L.volume_flux = cfg->volume_flux;
+  L.surface_flux = cfg->surface_flux;
+  L.surface_noncons = cfg->surface_noncons;
+  L.source_terms = cfg->source_terms;
+  L.gravity = cfg->gravity;
+  L.rotation_rate = cfg->rotation_rate;
+  for (int q = 0; q < n * n; ++q) L.dsplit[q] = b.Dsplit[q];
+  L.inv_w = 1.0 / b.w[0];
+  L.aux = h->d_aux;
+  L.xyz = h->d_xyz;
+  L.nbr = h->d_nbr;
+  L.member = h->d_member;
+  L.dt_member = nullptr;
+  *out = h;
+  return ATMO_OK;
+}
+
+int atmo_max_dt_members(atmo_handle* h, double cfl, double* dt) {
+  if (!h || !dt) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
+  if (h->eq != ATMO_EQ_COVARIANT_SWE_2D)
+    return fail(&h->err, ATMO_ERR_INVALID, "ensembles are covariant shallow-water handles");
+  return swe_max_dt(h, cfl, dt);
+}
+
+int atmo_step_members(atmo_handle* h, const double* dt, int32_t n_steps) {
+  if (!h || !dt || n_steps < 0) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "bad argument");
+  if (h->eq != ATMO_EQ_COVARIANT_SWE_2D)
+    return fail(&h->err, ATMO_ERR_INVALID, "ensembles are covariant shallow-water handles");
+  for (int m = 0; m < h->n_members; ++m)
+    if (!(dt[m] > 0) || !std::isfinite(dt[m]))
+      return fail(&h->err, ATMO_ERR_INVALID, "bad member time step");
+  return swe_step(h, dt, 0.0, n_steps);
+}
+
+}  // extern "C"

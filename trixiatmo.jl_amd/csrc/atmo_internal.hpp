@@ -38,6 +38,20 @@ struct Mesh3D {
   double period[3] = {0, 0, 0};                 // > 0: periodic length
 };
 
+// A 2D cubed-sphere mesh (one element per tree) with the 26 auxiliary
+// variables per node of the covariant form and 2D Trixi-style connectivity.
+struct Mesh2D {
+  int polydeg = 0, n = 0, n_elements = 0, n_interfaces = 0;
+  std::vector<double> node_coordinates;         // [3, n, n, E]
+  std::vector<double> aux;                      // [26, n, n, E]
+  std::vector<int32_t> interface_neighbor_ids;  // [2, I]
+  std::vector<int32_t> interface_node_indices;  // [2, 2, I]
+};
+
+// christoffel_mode 0: exact derivatives of the analytic metric; 1: collocation
+bool build_cubed_sphere_2d(Mesh2D& m, const Basis& b, int trees_per_face, double radius,
+                           int christoffel_mode, std::string* err);
+
 // node coordinates -> Jacobian matrix, invariant-curl metric terms, 1/J
 void compute_metric_terms(Mesh3D& m, const Basis& b);
 // face matching by node coordinates -> interfaces + boundaries

@@ -1,0 +1,80 @@
+// Handle of libatmo_hip (internal): device buffers, stream and timers.
+#ifndef ATMO_HANDLE_HPP
+#define ATMO_HANDLE_HPP
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/atmo_hip.h"
+#include "swe_kernels.hpp"
+
+namespace atmo {
+
+struct TimerPair {
+  hipEvent_t a, b;
+  int cls;
+};
+
+std::string& global_error();
+
+inline int fail(std::string* slot, int code, const std::string& msg) {
+  if (slot) *slot = msg;
+  global_error() = msg;
+  return code;
+}
+
+}  // namespace atmo
+
+struct atmo_handle {
+  int eq = ATMO_EQ_EULER_ENERGY_GRAVITY_3D;
+  atmo_config cfg;
+  int n = 0, n3 = 0, nv = 6;
+  int64_t len = 0;  // doubles per state
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double *d_u = nullptr, *d_u2 = nullptr, *d_k = nullptr, *d_du = nullptr;
+  double *d_uio = nullptr, *d_duio = nullptr;  // staging for atmo_rhs with host arrays
+  double *d_cv = nullptr, *d_invJ = nullptr;
+  float* d_cv32 = nullptr;  // FP32 geometry of the mixed-precision path
+  bool bg_on = false;       // hydrostatic background split (mixed path)
+  double *d_ubg = nullptr, *d_dubg = nullptr;
+  float* d_B = nullptr;
+  std::vector<double> dsplit, weights;
+  int* d_nbr = nullptr;
+  int* d_face_info = nullptr;
+  double *d_partial = nullptr, *d_result = nullptr;
+  // covariant shallow water
+  atmo_swe_config swe{};
+  atmo::SweLaunch swe_base{};
+  double *d_aux = nullptr, *d_xyz = nullptr, *d_dt_member = nullptr;
+  int *d_member = nullptr, *d_member_first = nullptr;
+  int n_members = 1;
+  bool du_valid = false;
+  bool timing = false;
+  std::vector<atmo::TimerPair> timers;
+  int64_t launches[2] = {0, 0};
+  double total_ms[2] = {0, 0};
+  std::string err;
+};
+
+#define HIP_TRY(h, expr)                                                         \
+  do {                                                                           \
+    hipError_t e__ = (expr);                                                     \
+    if (e__ != hipSuccess)                                                       \
+      return atmo::fail((h) ? &(h)->err : nullptr, ATMO_ERR_DEVICE,              \
+                        std::string(#expr) + ": " + hipGetErrorString(e__));     \
+  } while (0)
+
+namespace atmo {
+void timer_begin(atmo_handle* h, int cls, TimerPair* tp);
+void timer_end(atmo_handle* h, TimerPair* tp);
+// covariant shallow water paths of the generic entry points
+int swe_rhs_device(atmo_handle* h, double* du, const double* u);
+int swe_step(atmo_handle* h, const double* dt_members, double dt, int n_steps);
+int swe_max_dt(atmo_handle* h, double cfl, double* dt_members);
+int swe_integrals(atmo_handle* h, int fresh_rhs, double* out);
+}  // namespace atmo
+
+#endif

@@ -1,0 +1,444 @@
+// gfx950 kernels of the covariant shallow-water right-hand side
+// (SplitCovariantShallowWaterEquations2D on the cubed sphere) and its RK stage.
+//
+// Layout: u [3, n, n, E] (h, h v^1, h v^2), aux [26, n, n, E] as Trixi's
+// aux_node_vars (src/equations/equations.jl:178-250), node coordinates
+// [3, n, n, E].  A workgroup of 64 lanes holds EPB = max(1, 64 / n^2) elements
+// (one lane per node); each element's state and auxiliary block is staged in
+// LDS.  Volume terms are gathered per node: sum_m Dsplit[own, m] (f#(own, m) +
+// g(own, m) / 2) along both directions (the symmetric f# is evaluated for each
+// ordering, the nonconservative g is non-symmetric anyway).  Face nodes are
+// then mapped to lanes: each element evaluates the interface flux of its own
+// faces in its own contravariant basis with its own auxiliary variables,
+// exactly as the reference does for each side
+// (src/solvers/dgsem_p4est/dg_2d_manifold_in_3d_covariant.jl:301-432), lifts it
+// with 1/w_end, and every node adds its face terms, applies -1/J and the
+// geometric + Coriolis source, and stores du or applies the 2N RK update
+// (per-element time step: ensemble members step with their own dt).
+#include <hip/hip_runtime.h>
+
+#include "swe_kernels.hpp"
+
+namespace atmo {
+
+namespace {
+
+constexpr int NV = 3, NA = 26;
+
+struct AuxV {
+  const double* a;  // 26 entries with stride `s`
+  int s;
+  __device__ __forceinline__ double at(int k) const { return a[k * s]; }
+  __device__ __forceinline__ double A(int r, int c) const { return at(r + 3 * c); }
+  __device__ __forceinline__ double B(int r, int c) const { return at(6 + r + 2 * c); }
+  __device__ __forceinline__ double J() const { return at(12); }
+  __device__ __forceinline__ double Gcov(int i, int j) const {
+    return (i == 0 && j == 0) ? at(13) : (i == 1 && j == 1) ? at(15) : at(14);
+  }
+  __device__ __forceinline__ double Gcon(int i, int j) const {
+    return (i == 0 && j == 0) ? at(16) : (i == 1 && j == 1) ? at(18) : at(17);
+  }
+  __device__ __forceinline__ double hs() const { return at(19); }
+  __device__ __forceinline__ double Gam(int k, int i, int j) const {
+    const int o = 20 + 3 * k;
+    return (i == 0 && j == 0) ? at(o) : (i == 1 && j == 1) ? at(o + 2) : at(o + 1);
+  }
+};
+
+// covariant_shallow_water_split.jl:66-80
+__device__ __forceinline__ void flux_split(const double* u, const AuxV& x, int o, double* f) {
+  const double mass = x.J() * u[1 + o];
+  f[0] = mass;
+  f[1] = mass * (u[1] / u[0]);
+  f[2] = mass * (u[2] / u[0]);
+}
+
+// :100-122 flux_ec
+__device__ __forceinline__ void flux_ec(const double* ul, const double* ur, const AuxV& xl,
+                                        const AuxV& xr, int o, double* f) {
+  const double mass = 0.5 * (xl.J() * ul[1 + o] + xr.J() * ur[1 + o]);
+  f[0] = mass;
+  f[1] = 0.5 * (ul[1] / ul[0] + ur[1] / ur[0]) * mass;
+  f[2] = 0.5 * (ul[2] / ul[0] + ur[2] / ur[0]) * mass;
+}
+
+__device__ __forceinline__ void flux_central(const double* ul, const double* ur, const AuxV& xl,
+                                             const AuxV& xr, int o, double* f) {
+  double fl[NV], fr[NV];
+  flux_split(ul, xl, o, fl);
+  flux_split(ur, xr, o, fr);
+#pragma unroll
+  for (int v = 0; v < NV; ++v) f[v] = 0.5 * (fl[v] + fr[v]);
+}
+
+// :157-187 flux_nonconservative_ec
+__device__ __forceinline__ void noncons_ec(double g, const double* ul, const double* ur,
+                                           const AuxV& xl, const AuxV& xr, int o, double* f) {
+  const double hl = ul[0], hr = ur[0];
+  const double vr0 = ur[1] / hr, vr1 = ur[2] / hr;
+  const double vc0 = xr.Gcov(0, 0) * vr0 + xr.Gcov(0, 1) * vr1;
+  const double vc1 = xr.Gcov(1, 0) * vr0 + xr.Gcov(1, 1) * vr1;
+  const double hs_jump = xr.hs() - xl.hs();
+  const double geo0 = 0.5 * ul[1 + o] * ((xl.Gcon(0, 0) * vc0 + xl.Gcon(0, 1) * vc1) - vr0);
+  const double geo1 = 0.5 * ul[1 + o] * ((xl.Gcon(1, 0) * vc0 + xl.Gcon(1, 1) * vc1) - vr1);
+  const double pr0 = g * xl.Gcon(0, o) * hl * (hr + hs_jump);
+  const double pr1 = g * xl.Gcon(1, o) * hl * (hr + hs_jump);
+  f[0] = 0.0;
+  f[1] = xl.J() * (geo0 + pr0);
+  f[2] = xl.J() * (geo1 + pr1);
+}
+
+__device__ __forceinline__ double max_abs_speed(double g, const double* ul, const double* ur,
+                                                const AuxV& xl, const AuxV& xr, int o) {
+  return fmax(fabs(ul[1 + o] / ul[0]) + sqrt(xl.Gcon(o, o) * ul[0] * g),
+              fabs(ur[1 + o] / ur[0]) + sqrt(xr.Gcon(o, o) * ur[0] * g));
+}
+
+// surface flux (u_ll, u_rr, aux, aux) + 1/2 the nonconservative surface term
+__device__ __forceinline__ void surface_terms(const SweLaunch& L, const double* ul,
+                                              const double* ur, const AuxV& x, int o,
+                                              double* f) {
+  if (L.surface_flux == 1) flux_central(ul, ur, x, x, o, f);
+  else flux_ec(ul, ur, x, x, o, f);
+  if (L.surface_flux != 2) {  // DissipationLocalLaxFriedrichs, equations.jl:283-290
+    const double lam = max_abs_speed(L.gravity, ul, ur, x, x, o);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) f[v] += -0.5 * x.J() * lam * (ur[v] - ul[v]);
+  }
+  double g[NV];
+  if (L.surface_noncons == 0) {  // flux_nonconservative_surface_simplified, :229-250
+    g[0] = 0.0;
+    g[1] = x.J() * (L.gravity * x.Gcon(0, o) * ul[0] * ur[0]);
+    g[2] = x.J() * (L.gravity * x.Gcon(1, o) * ul[0] * ur[0]);
+  } else {
+    noncons_ec(L.gravity, ul, ur, x, x, o, g);
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) f[v] += 0.5 * g[v];
+}
+
+// covariant_shallow_water_split.jl:273-302
+__device__ __forceinline__ void source_split(double omega, const double* u, const double* x,
+                                             const AuxV& a, double* s) {
+  const double hv0 = u[1], hv1 = u[2];
+  const double v0 = u[1] / u[0], v1 = u[2] / u[0];
+  const double hvv[2][2] = {{hv0 * v0, hv0 * v1}, {hv1 * v0, hv1 * v1}};
+  double hcv[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) hcv[i][j] = a.Gcov(i, 0) * hvv[0][j] + a.Gcov(i, 1) * hvv[1][j];
+  const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  const double f = 2 * omega * x[2] / r;
+  double t1[2], t2[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    t1[k] = a.Gam(k, 0, 0) * hvv[0][0] + a.Gam(k, 1, 0) * hvv[1][0] + a.Gam(k, 0, 1) * hvv[0][1] +
+            a.Gam(k, 1, 1) * hvv[1][1];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+    t2[i] = a.Gam(0, i, 0) * hcv[0][0] + a.Gam(0, i, 1) * hcv[0][1] + a.Gam(1, i, 0) * hcv[1][0] +
+            a.Gam(1, i, 1) * hcv[1][1];
+  const double sg0 = 0.5 * (t1[0] - (a.Gcon(0, 0) * t2[0] + a.Gcon(0, 1) * t2[1]));
+  const double sg1 = 0.5 * (t1[1] - (a.Gcon(1, 0) * t2[0] + a.Gcon(1, 1) * t2[1]));
+  s[0] = 0.0;
+  s[1] = -(sg0 + f * a.J() * (a.Gcon(0, 1) * hv0 - a.Gcon(0, 0) * hv1));
+  s[2] = -(sg1 + f * a.J() * (a.Gcon(1, 1) * hv0 - a.Gcon(1, 0) * hv1));
+}
+
+template <int N>
+struct SweShape {
+  static constexpr int N2 = N * N;
+  static constexpr int EPB = N2 >= 64 ? 1 : 64 / N2;  // elements per workgroup
+  static constexpr int BLOCK = ((EPB * N2 + 63) / 64) * 64;
+  static constexpr int NF = 4 * N;  // face nodes per element
+};
+
+}  // namespace
+
+template <int N>
+__global__ __launch_bounds__(SweShape<N>::BLOCK) void swe_stage(SweLaunch L) {
+  using S = SweShape<N>;
+  constexpr int N2 = S::N2, EPB = S::EPB, B = S::BLOCK;
+  __shared__ double s_u[EPB][NV][N2];
+  __shared__ double s_a[EPB][NA][N2];
+  __shared__ double s_sf[EPB][NV][S::NF];
+  __shared__ double s_ds[N * N];
+
+  const int t = threadIdx.x;
+  const int e0 = blockIdx.x * EPB;
+  if (t < N * N) s_ds[t] = L.dsplit[t];
+  // stage the elements' state and auxiliary variables (contiguous per element)
+  for (int idx = t; idx < EPB * N2 * NV; idx += B) {
+    const int es = idx / (N2 * NV), r = idx - es * N2 * NV;
+    if (e0 + es < L.n_elements) s_u[es][r % NV][r / NV] = L.u[(size_t)(e0 + es) * N2 * NV + r];
+  }
+  for (int idx = t; idx < EPB * N2 * NA; idx += B) {
+    const int es = idx / (N2 * NA), r = idx - es * N2 * NA;
+    if (e0 + es < L.n_elements) s_a[es][r % NA][r / NA] = L.aux[(size_t)(e0 + es) * N2 * NA + r];
+  }
+  __syncthreads();
+
+  const int es = t / N2, q = t - es * N2;
+  const int e = e0 + es;
+  const bool active = es < EPB && e < L.n_elements;
+  const int ti = q % N, tj = q / N;
+  double du[NV] = {0, 0, 0};
+  if (active) {
+    const double uo[NV] = {s_u[es][0][q], s_u[es][1][q], s_u[es][2][q]};
+    const AuxV ao{&s_a[es][0][q], N2};
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+      const int own = o == 0 ? ti : tj;
+#pragma unroll
+      for (int m = 0; m < N; ++m) {
+        if (m == own) continue;
+        const int pq = o == 0 ? m + N * tj : ti + N * m;
+        const double up[NV] = {s_u[es][0][pq], s_u[es][1][pq], s_u[es][2][pq]};
+        const AuxV ap{&s_a[es][0][pq], N2};
+        double f[NV], g[NV];
+        if (L.volume_flux == 0) flux_ec(uo, up, ao, ap, o, f);
+        else flux_central(uo, up, ao, ap, o, f);
+        noncons_ec(L.gravity, uo, up, ao, ap, o, g);
+        const double ds = s_ds[own * N + m];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) du[v] += ds * f[v] + 0.5 * ds * g[v];
+      }
+    }
+  }
+  // face nodes of the workgroup's elements, one lane each
+  const double inv_w = L.inv_w;
+  for (int fid = t; fid < EPB * S::NF; fid += B) {
+    const int fe = fid / S::NF, r = fid - fe * S::NF;
+    const int f = r / N, k = r - f * N;
+    if (e0 + fe >= L.n_elements) continue;
+    const int i = f == 0 ? 0 : (f == 1 ? N - 1 : k);
+    const int j = f == 2 ? 0 : (f == 3 ? N - 1 : k);
+    const int qn = i + N * j;
+    const double ul[NV] = {s_u[fe][0][qn], s_u[fe][1][qn], s_u[fe][2][qn]};
+    const AuxV ax{&s_a[fe][0][qn], N2};
+    const int link = L.nbr[((size_t)(e0 + fe) * 4 + f) * N + k];
+    const double* ur = L.u + (size_t)link * NV;
+    const double* ar = L.aux + (size_t)link * NA;  // neighbour basis_covariant = ar[0..5]
+    // neighbour state in this element's contravariant basis (contravariant2global,
+    // then global2contravariant: covariant_shallow_water.jl:140-153)
+    double gl[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) gl[c] = ar[c] * ur[1] + ar[3 + c] * ur[2];
+    const double tr[NV] = {ur[0], ax.B(0, 0) * gl[0] + ax.B(0, 1) * gl[1] + ax.B(0, 2) * gl[2],
+                           ax.B(1, 0) * gl[0] + ax.B(1, 1) * gl[1] + ax.B(1, 2) * gl[2]};
+    const int o = f / 2;
+    double fs[NV];
+    if (f % 2 == 0) {  // negative-side face: -(f*(u_nbr, u_own) + g/2)
+      surface_terms(L, tr, ul, ax, o, fs);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) fs[v] = -fs[v];
+    } else {
+      surface_terms(L, ul, tr, ax, o, fs);
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s_sf[fe][v][r] = fs[v] * inv_w;
+  }
+  __syncthreads();
+  if (!active) return;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    const bool on = f == 0 ? ti == 0 : f == 1 ? ti == N - 1 : f == 2 ? tj == 0 : tj == N - 1;
+    if (!on) continue;
+    const int k = f < 2 ? tj : ti;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) du[v] += s_sf[es][v][f * N + k];
+  }
+  const AuxV ao{&s_a[es][0][q], N2};
+  const double fac = -1.0 / ao.J();
+  const size_t g = (size_t)e * N2 + q;
+  const double uo[NV] = {s_u[es][0][q], s_u[es][1][q], s_u[es][2][q]};
+#pragma unroll
+  for (int v = 0; v < NV; ++v) du[v] *= fac;
+  if (L.source_terms) {
+    double sv[NV];
+    source_split(L.rotation_rate, uo, L.xyz + g * 3, ao, sv);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) du[v] += sv[v];
+  }
+  if (L.mode == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) L.out[g * NV + v] = du[v];
+  } else {
+    const double dt = L.dt_member ? L.dt_member[L.member[e]] : L.rk_dt;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double kn = L.rk_a * L.k[g * NV + v] + dt * du[v];
+      L.k[g * NV + v] = kn;
+      L.out[g * NV + v] = uo[v] + L.rk_b * kn;
+    }
+  }
+}
+
+// per-element maximum of the scaled wave speeds (stepsize_dg2d.jl:47-71)
+template <int N>
+__global__ __launch_bounds__(64) void swe_max_speed(SweLaunch L, double* partial) {
+  constexpr int N2 = N * N;
+  const int e = blockIdx.x, t = threadIdx.x;
+  __shared__ double s1[64], s2[64];
+  double l1 = 0.0, l2 = 0.0;
+  for (int q = t; q < N2; q += 64) {
+    const size_t g = (size_t)e * N2 + q;
+    const double* u = L.u + g * NV;
+    const AuxV a{L.aux + g * NA, 1};
+    l1 = fmax(l1, fabs(u[1] / u[0]) + sqrt(a.Gcon(0, 0) * u[0] * L.gravity));
+    l2 = fmax(l2, fabs(u[2] / u[0]) + sqrt(a.Gcon(1, 1) * u[0] * L.gravity));
+  }
+  s1[t] = l1;
+  s2[t] = l2;
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) {
+    if (t < w) {
+      s1[t] = fmax(s1[t], s1[t + w]);
+      s2[t] = fmax(s2[t], s2[t + w]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) partial[e] = s1[0] + s2[0];
+}
+
+// per-member maximum over its elements
+__global__ void swe_member_max(const double* partial, const int* member_first, int n_members,
+                               double* out) {
+  const int m = blockIdx.x, t = threadIdx.x;
+  __shared__ double s[256];
+  double acc = 5e-324;
+  for (int e = member_first[m] + t; e < member_first[m + 1]; e += 256) acc = fmax(acc, partial[e]);
+  s[t] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) s[t] = fmax(s[t], s[t + w]);
+    __syncthreads();
+  }
+  if (t == 0) out[m] = s[0];
+}
+
+bool swe_launch_stage(const SweLaunch& L, hipStream_t s) {
+#define ATMO_SWE_CASE(NN)                                                                      \
+  case NN: {                                                                                   \
+    const int grid = (L.n_elements + SweShape<NN>::EPB - 1) / SweShape<NN>::EPB;               \
+    hipLaunchKernelGGL(swe_stage<NN>, dim3(grid), dim3(SweShape<NN>::BLOCK), 0, s, L);         \
+    return true;                                                                               \
+  }
+  switch (L.n) {
+    ATMO_SWE_CASE(2)
+    ATMO_SWE_CASE(3)
+    ATMO_SWE_CASE(4)
+    ATMO_SWE_CASE(5)
+    ATMO_SWE_CASE(6)
+    ATMO_SWE_CASE(7)
+    ATMO_SWE_CASE(8)
+  }
+#undef ATMO_SWE_CASE
+  return false;
+}
+
+bool swe_launch_member_max_speed(const SweLaunch& L, double* partial, const int* member_first,
+                                 int n_members, double* out, hipStream_t s) {
+#define ATMO_SWE_CASE(NN)                                                                       \
+  case NN:                                                                                      \
+    hipLaunchKernelGGL(swe_max_speed<NN>, dim3(L.n_elements), dim3(64), 0, s, L, partial);     \
+    break;
+  switch (L.n) {
+    ATMO_SWE_CASE(2)
+    ATMO_SWE_CASE(3)
+    ATMO_SWE_CASE(4)
+    ATMO_SWE_CASE(5)
+    ATMO_SWE_CASE(6)
+    ATMO_SWE_CASE(7)
+    ATMO_SWE_CASE(8)
+    default:
+      return false;
+  }
+#undef ATMO_SWE_CASE
+  hipLaunchKernelGGL(swe_member_max, dim3(n_members), dim3(256), 0, s, partial, member_first,
+                     n_members, out);
+  return true;
+}
+
+}  // namespace atmo
+
+namespace atmo {
+
+// per element: mass, total energy (entropy) and int w(u).du with the exact
+// area element (analysis_covariant.jl:53-105; covariant_shallow_water.jl:129-164)
+template <int N>
+__global__ __launch_bounds__(64) void swe_integrals_kernel(SweLaunch L, const double* du,
+                                                           const double* w, double* partial) {
+  constexpr int N2 = N * N;
+  const int e = blockIdx.x, t = threadIdx.x;
+  __shared__ double s[3][64];
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (int q = t; q < N2; q += 64) {
+    const size_t g = (size_t)e * N2 + q;
+    const double* u = L.u + g * NV;
+    const AuxV a{L.aux + g * NA, 1};
+    const double h = u[0], v1 = u[1] / h, v2 = u[2] / h;
+    const double vc1 = a.Gcov(0, 0) * v1 + a.Gcov(0, 1) * v2;
+    const double vc2 = a.Gcov(1, 0) * v1 + a.Gcov(1, 1) * v2;
+    const double wJ = w[q % N] * w[q / N] * a.J();
+    a0 += wJ * h;
+    a1 += wJ * (0.5 * (h * (vc1 * v1 + vc2 * v2) + L.gravity * h * h) + L.gravity * h * a.hs());
+    if (du) {
+      const double w1 = L.gravity * (h + a.hs()) - 0.5 * (vc1 * v1 + vc2 * v2);
+      a2 += wJ * (w1 * du[g * NV] + vc1 * du[g * NV + 1] + vc2 * du[g * NV + 2]);
+    }
+  }
+  s[0][t] = a0;
+  s[1][t] = a1;
+  s[2][t] = a2;
+  __syncthreads();
+  for (int k = 32; k > 0; k >>= 1) {
+    if (t < k)
+      for (int c = 0; c < 3; ++c) s[c][t] += s[c][t + k];
+    __syncthreads();
+  }
+  if (t < 3) partial[(size_t)e * 3 + t] = s[t][0];
+}
+
+__global__ void swe_sum_columns(const double* partial, int n_items, double* out) {
+  __shared__ double s[256];
+  const int t = threadIdx.x;
+  for (int c = 0; c < 3; ++c) {
+    double acc = 0.0;
+    for (int i = t; i < n_items; i += 256) acc += partial[(size_t)i * 3 + c];
+    s[t] = acc;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+      if (t < k) s[t] += s[t + k];
+      __syncthreads();
+    }
+    if (t == 0) out[c] = s[0];
+    __syncthreads();
+  }
+}
+
+bool swe_launch_integrals(const SweLaunch& L, const double* du, const double* w, double* partial,
+                          double* out, hipStream_t s) {
+#define ATMO_SWE_CASE(NN)                                                                     \
+  case NN:                                                                                    \
+    hipLaunchKernelGGL(swe_integrals_kernel<NN>, dim3(L.n_elements), dim3(64), 0, s, L, du, w, \
+                       partial);                                                              \
+    break;
+  switch (L.n) {
+    ATMO_SWE_CASE(2)
+    ATMO_SWE_CASE(3)
+    ATMO_SWE_CASE(4)
+    ATMO_SWE_CASE(5)
+    ATMO_SWE_CASE(6)
+    ATMO_SWE_CASE(7)
+    ATMO_SWE_CASE(8)
+    default:
+      return false;
+  }
+#undef ATMO_SWE_CASE
+  hipLaunchKernelGGL(swe_sum_columns, dim3(1), dim3(256), 0, s, partial, L.n_elements, out);
+  return true;
+}
+
+}  // namespace atmo

@@ -1,0 +1,41 @@
+// Launch interface of the covariant shallow-water kernels (internal).
+#ifndef ATMO_SWE_KERNELS_HPP
+#define ATMO_SWE_KERNELS_HPP
+
+#include <hip/hip_runtime.h>
+
+namespace atmo {
+
+struct SweLaunch {
+  int n, n_elements;
+  int volume_flux;      // 0 flux_ec, 1 flux_central
+  int surface_flux;     // 0 FluxPlusDissipation(flux_ec, LLF), 1 flux_lax_friedrichs, 2 flux_ec
+  int surface_noncons;  // 0 surface_simplified, 1 flux_nonconservative_ec
+  int source_terms;     // 1 source_terms_geometric_coriolis
+  double gravity, rotation_rate;
+  double dsplit[64];    // Dsplit[i * n + m]
+  double inv_w;         // 1 / w_end
+  const double* u;
+  double* out;
+  double* k;
+  const double* aux;    // [26, n, n, E]
+  const double* xyz;    // [3, n, n, E]
+  const int* nbr;       // [E, 4, n] neighbour global node of each face node
+  int mode;             // 0 rhs, 1 RK stage
+  double rk_a, rk_dt, rk_b;
+  const double* dt_member;  // per-member time steps (NULL: rk_dt for all)
+  const int* member;        // [E] member of each element
+};
+
+bool swe_launch_stage(const SweLaunch& L, hipStream_t s);
+// per-member max over elements of the summed scaled wave speeds -> out[n_members]
+bool swe_launch_member_max_speed(const SweLaunch& L, double* partial, const int* member_first,
+                                 int n_members, double* out, hipStream_t s);
+
+// integrals (mass, entropy, dS/dt) -> out[3]; w = device copy of the LGL weights
+bool swe_launch_integrals(const SweLaunch& L, const double* du, const double* w, double* partial,
+                          double* out, hipStream_t s);
+
+}  // namespace atmo
+
+#endif

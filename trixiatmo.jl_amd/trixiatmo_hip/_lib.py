@@ -59,6 +59,32 @@ class AtmoGeometry3D(ctypes.Structure):
     ]
 
 
+class AtmoSweConfig(ctypes.Structure):
+    _fields_ = [
+        ("polydeg", ctypes.c_int32),
+        ("n_elements", ctypes.c_int32),
+        ("volume_flux", ctypes.c_int32),
+        ("surface_flux", ctypes.c_int32),
+        ("surface_noncons", ctypes.c_int32),
+        ("source_terms", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("gravity", ctypes.c_double),
+        ("rotation_rate", ctypes.c_double),
+    ]
+
+
+class AtmoGeometry2D(ctypes.Structure):
+    _fields_ = [
+        ("aux_node_vars", c_double_p),
+        ("node_coordinates", c_double_p),
+        ("n_interfaces", ctypes.c_int32),
+        ("interface_neighbor_ids", c_int32_p),
+        ("interface_node_indices", c_int32_p),
+        ("n_members", ctypes.c_int32),
+        ("member_first_element", c_int32_p),
+    ]
+
+
 # (name, restype, argtypes) of every exported symbol of include/atmo_hip.h
 SIGNATURES = [
     ("atmo_lgl_basis", ctypes.c_int, [ctypes.c_int, c_double_p, c_double_p, c_double_p, c_double_p]),
@@ -71,6 +97,16 @@ SIGNATURES = [
     ("atmo_mesh3d_arrays", ctypes.c_int,
      [ctypes.c_void_p] + [ctypes.POINTER(c_double_p)] * 3 + [ctypes.POINTER(c_int32_p)] * 5),
     ("atmo_mesh3d_destroy", None, [ctypes.c_void_p]),
+    ("atmo_mesh2d_cubed_sphere", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    ("atmo_mesh2d_sizes", ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int32_p]),
+    ("atmo_mesh2d_arrays", ctypes.c_int,
+     [ctypes.c_void_p] + [ctypes.POINTER(c_double_p)] * 2 + [ctypes.POINTER(c_int32_p)] * 2),
+    ("atmo_mesh2d_destroy", None, [ctypes.c_void_p]),
+    ("atmo_swe_create", ctypes.c_int,
+     [ctypes.POINTER(AtmoSweConfig), ctypes.POINTER(AtmoGeometry2D), ctypes.POINTER(ctypes.c_void_p)]),
+    ("atmo_max_dt_members", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, c_double_p]),
+    ("atmo_step_members", ctypes.c_int, [ctypes.c_void_p, c_double_p, ctypes.c_int32]),
     ("atmo_create", ctypes.c_int,
      [ctypes.POINTER(AtmoConfig), ctypes.POINTER(AtmoGeometry3D), ctypes.POINTER(ctypes.c_void_p)]),
     ("atmo_destroy", None, [ctypes.c_void_p]),

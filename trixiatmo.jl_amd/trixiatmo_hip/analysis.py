@@ -54,3 +54,16 @@ def calc_error_norms(u, mesh, exact, analysis_polydeg=None):
     l2 = np.sqrt((diff ** 2 * wJ).sum(axis=(1, 2, 3, 4)) / wJ.sum())
     linf = np.abs(diff).max(axis=(1, 2, 3, 4))
     return l2, linf
+
+
+def calc_error_norms_covariant(u, mesh, aux, exact, t):
+    """src/callbacks_step/analysis_covariant.jl:144-189: errors of the
+    conserved variables at the solution nodes, the l2 norm weighted by
+    w_i w_j J with J the area element of the auxiliary variables.
+    u [3,n,n,E], aux [26,n,n,E], exact(x, t, aux) -> [3,n,n,E]."""
+    _, w, _, _ = lgl_basis(mesh.polydeg)
+    diff = exact(mesh.node_coordinates, t, aux) - u
+    wJ = w[:, None, None] * w[None, :, None] * aux[12]
+    l2 = np.sqrt((diff ** 2 * wJ).sum(axis=(1, 2, 3)) / wJ.sum())
+    linf = np.abs(diff).max(axis=(1, 2, 3))
+    return l2, linf

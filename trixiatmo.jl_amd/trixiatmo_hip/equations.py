@@ -18,12 +18,22 @@ SURFACE_FLUXES = {"flux_lax_friedrichs": 0, "flux_ranocha": 1, "flux_kennedy_gru
                   "FluxPlusDissipation(flux_ranocha, DissipationLocalLaxFriedrichs())": 5}
 
 
+# covariant shallow water: volume fluxes -> ATMO_SWE_VOL_*, surface fluxes ->
+# ATMO_SWE_SURF_*, surface nonconservative terms -> ATMO_SWE_NC_*
+SWE_VOLUME_FLUXES = {"flux_ec": 0, "flux_central": 1}
+SWE_VOLUME_NONCONS = {"flux_nonconservative_ec"}
+SWE_SURFACE_FLUXES = {"FluxPlusDissipation(flux_ec, DissipationLocalLaxFriedrichs())": 0,
+                      "flux_lax_friedrichs": 1, "flux_ec": 2}
+SWE_SURFACE_NONCONS = {"flux_nonconservative_surface_simplified": 0,
+                       "flux_nonconservative_ec": 1}
+
+
 def FluxPlusDissipation(numerical_flux, dissipation="DissipationLocalLaxFriedrichs()"):
     """`FluxPlusDissipation(flux, DissipationLocalLaxFriedrichs())`: a two-point
     flux plus local Lax-Friedrichs dissipation -lambda_max/2 [u] (the
     construction of src/equations/equations.jl:256-290)."""
     name = f"FluxPlusDissipation({numerical_flux}, {dissipation})"
-    if name not in SURFACE_FLUXES:
+    if name not in SURFACE_FLUXES and name not in SWE_SURFACE_FLUXES:
         raise ValueError(f"unsupported surface flux {name}")
     return name
 
@@ -58,3 +68,21 @@ class CompressibleEulerEnergyEquationsWithGravity3D:
 class FluxLMARS:
     """FluxLMARS(c): low-Mach approximate Riemann solver with sound speed c."""
     speed_of_sound: float
+
+
+@dataclass(frozen=True)
+class SplitCovariantShallowWaterEquations2D:
+    """src/equations/covariant_shallow_water_split.jl:1-64: shallow water on a
+    2D manifold in 3D, variables (h, h v^1, h v^2) with contravariant
+    momentum, split form with entropy-conservative two-point fluxes.  The
+    global coordinate system is GlobalCartesianCoordinates()."""
+    gravity: float
+    rotation_rate: float
+    global_coordinate_system: str = "GlobalCartesianCoordinates()"
+
+    nvariables = 3
+    varnames = ("h", "h_vcon1", "h_vcon2")
+
+    def __post_init__(self):
+        if self.global_coordinate_system != "GlobalCartesianCoordinates()":
+            raise ValueError("the covariant path uses GlobalCartesianCoordinates()")

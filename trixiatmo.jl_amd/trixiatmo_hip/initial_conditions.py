@@ -149,3 +149,212 @@ def initial_condition_baroclinic_instability(x, equations):
     v3 = np.cos(lat) * v
     phi = -EARTH_RADIUS ** 2 * 9.80616 / r
     return np.stack(equations.prim2cons(rho, v1, v2, v3, p, phi), axis=0)
+
+
+# ---------------------------------------------------------------------------
+# Covariant shallow water on the sphere (src/equations/reference_data.jl).
+# Every function takes node positions x [3, ...] and the time t and returns the
+# tuple (H, v_x, v_y, v_z, h_s) of total height, Cartesian velocity and bottom
+# topography that the reference's initial conditions return for
+# GlobalCartesianCoordinates; `transform_initial_condition` turns it into the
+# conserved covariant variables with the auxiliary variables of each node.
+
+EARTH_GRAVITATIONAL_ACCELERATION = 9.80616
+EARTH_ROTATION_RATE = 7.292e-5
+SECONDS_PER_DAY = 8.64e4
+
+
+def _f32(v):
+    """A Julia Float32 literal such as 7.848f-6, promoted to Float64."""
+    return float(np.float32(v))
+
+
+def _lon_lat(x):
+    a = np.sqrt(x[0] ** 2 + x[1] ** 2 + x[2] ** 2)
+    return a, np.arctan2(x[1], x[0]), np.arcsin(x[2] / a)
+
+
+def spherical2cartesian(vlon, vlat, vrad, x):
+    """src/equations/equations.jl:294-307"""
+    _, lon, lat = _lon_lat(x)
+    sinlon, coslon = np.sin(lon), np.cos(lon)
+    sinlat, coslat = np.sin(lat), np.cos(lat)
+    vx = -sinlon * vlon - sinlat * coslon * vlat + coslat * coslon * vrad
+    vy = coslon * vlon - sinlat * sinlon * vlat + coslat * sinlon * vrad
+    vz = coslat * vlat + sinlat * vrad
+    return vx, vy, vz
+
+
+def _spherical_state(h, vlon, vlat, b, x):
+    vx, vy, vz = spherical2cartesian(vlon, vlat, np.zeros_like(h), x)
+    return h, vx, vy, vz, b
+
+
+def bottom_topography_unsteady_solid_body_rotation(x):
+    """reference_data.jl:386-388"""
+    return 0.5 * (EARTH_ROTATION_RATE * x[2]) ** 2 / EARTH_GRAVITATIONAL_ACCELERATION
+
+
+def initial_condition_unsteady_solid_body_rotation(x, t):
+    """reference_data.jl:348-384 (Lauter et al. 2005)."""
+    a = np.sqrt(x[0] ** 2 + x[1] ** 2 + x[2] ** 2)
+    v_0 = 2.0 * np.pi * a / (12 * SECONDS_PER_DAY)
+    k1 = 133681.0
+    alpha = np.pi / 4
+    wt = EARTH_ROTATION_RATE * t
+    b1 = np.array([np.cos(wt), np.sin(wt), 0.0])
+    b2 = np.array([-np.sin(wt), np.cos(wt), 0.0])
+    b3 = np.array([0.0, 0.0, 1.0])
+    c = np.array([-np.sin(alpha), 0.0, np.cos(alpha)])
+    nrm = x / a
+    omega_dot_x = EARTH_ROTATION_RATE * x[2]
+    phi_t = np.array([c @ b1, c @ b2, c @ b3])
+    v = v_0 * np.stack([phi_t[1] * nrm[2] - phi_t[2] * nrm[1],
+                        phi_t[2] * nrm[0] - phi_t[0] * nrm[2],
+                        phi_t[0] * nrm[1] - phi_t[1] * nrm[0]])
+    pn = phi_t[0] * nrm[0] + phi_t[1] * nrm[1] + phi_t[2] * nrm[2]
+    H = (-0.5 * (v_0 * pn + omega_dot_x) ** 2 + 0.5 * omega_dot_x ** 2 + k1) / \
+        EARTH_GRAVITATIONAL_ACCELERATION
+    return H, v[0], v[1], v[2], bottom_topography_unsteady_solid_body_rotation(x)
+
+
+def bottom_topography_isolated_mountain(x):
+    """reference_data.jl:286-294 (Williamson et al. 1992, case 5)."""
+    a, lon, lat = _lon_lat(x)
+    lon_0, lat_0 = -np.pi / 2, np.pi / 6
+    b_0 = 2000.0
+    R = np.pi / 9
+    return b_0 * (1.0 - np.sqrt(np.minimum(R ** 2, (lon - lon_0) ** 2 + (lat - lat_0) ** 2)) / R)
+
+
+def initial_condition_isolated_mountain(x, t):
+    """reference_data.jl:262-275"""
+    a, _, lat = _lon_lat(x)
+    h_0, v_0 = 5960.0, 20.0
+    vlon = v_0 * np.cos(lat)
+    h = h_0 - 1 / EARTH_GRAVITATIONAL_ACCELERATION * \
+        (a * EARTH_ROTATION_RATE * v_0 + 0.5 * v_0 ** 2) * np.sin(lat) ** 2
+    return _spherical_state(h, vlon, np.zeros_like(h), bottom_topography_isolated_mountain(x), x)
+
+
+def initial_condition_well_balanced(x, t):
+    """examples/shallow_water/covariant/elixir_well_balanced.jl:13: H = 5960 m
+    at rest (with the isolated-mountain topography as auxiliary field)."""
+    H = np.full(x.shape[1:], 5960.0)
+    z = np.zeros_like(H)
+    return H, z, z, z, z
+
+
+def initial_condition_rossby_haurwitz(x, t):
+    """reference_data.jl:202-235 (Williamson et al. 1992, case 6); omega and
+    K are Float32 literals in the reference and are promoted the same way."""
+    a, lon, lat = _lon_lat(x)
+    h_0 = 8.0e3
+    omega = _f32(7.848e-6)
+    K = _f32(7.848e-6)
+    R = 4.0
+    cl = np.cos(lat)
+    A = 0.5 * omega * (2 * EARTH_ROTATION_RATE + omega) * cl ** 2 + \
+        0.25 * K ** 2 * cl ** (2 * R) * ((R + 1) * cl ** 2 + (2 * R ** 2 - R - 2) -
+                                         2 * R ** 2 / cl ** 2)
+    B = 2 * (EARTH_ROTATION_RATE + omega) * K / ((R + 1) * (R + 2)) * cl ** R * \
+        ((R ** 2 + 2 * R + 2) - (R + 1) ** 2 * cl ** 2)
+    C = 0.25 * K ** 2 * cl ** (2 * R) * ((R + 1) * cl ** 2 - (R + 2))
+    h = h_0 + (1 / EARTH_GRAVITATIONAL_ACCELERATION) * \
+        (a ** 2 * A + a ** 2 * B * np.cos(R * lon) + a ** 2 * C * np.cos(2 * R * lon))
+    vlon = a * omega * cl + a * K * cl ** (R - 1) * (R * np.sin(lat) ** 2 - cl ** 2) * \
+        np.cos(R * lon)
+    vlat = -a * K * R * cl ** (R - 1) * np.sin(lat) * np.sin(R * lon)
+    return _spherical_state(h, vlon, vlat, np.zeros_like(h), x)
+
+
+def galewsky_velocity(lat, u_0, lat_0, lat_1):
+    """reference_data.jl:457-465"""
+    lat = np.asarray(lat, dtype=float)
+    inside = (lat_0 < lat) & (lat < lat_1)
+    ls = np.where(inside, lat, 0.5 * (lat_0 + lat_1))
+    u = u_0 / np.exp(-4 / (lat_1 - lat_0) ** 2) * np.exp(1 / (ls - lat_0) * 1 / (ls - lat_1))
+    return np.where(inside, u, 0.0)
+
+
+def _galewsky_integral(lat, u_0, lat_0, lat_1, a):
+    """integral from pi/2 to lat of u (2 Omega sin + u tan / a) (the quadgk
+    call of reference_data.jl:436-437); the integrand vanishes outside
+    (lat_0, lat_1), so adaptive Gauss-Kronrod runs on that interval only."""
+    from scipy.integrate import quad
+
+    def f(s):
+        u = float(galewsky_velocity(s, u_0, lat_0, lat_1))
+        return u * (2 * EARTH_ROTATION_RATE * np.sin(s) + u * np.tan(s) / a)
+
+    flat = np.ravel(lat)
+    out = np.empty_like(flat)
+    for q, l in enumerate(flat):
+        lo = min(max(l, lat_0), lat_1)
+        val, _ = quad(f, lo, lat_1, epsabs=0.0, epsrel=1e-13, limit=200)
+        out[q] = -val
+    return out.reshape(np.shape(lat))
+
+
+def initial_condition_barotropic_instability(x, t):
+    """reference_data.jl:425-455 (Galewsky et al. 2004)."""
+    a, lon, lat = _lon_lat(x)
+    u_0 = 80.0
+    lat_0 = np.pi / 7
+    lat_1 = np.pi / 2 - lat_0
+    vlon = galewsky_velocity(lat, u_0, lat_0, lat_1)
+    radius = float(np.ravel(a)[0])
+    h = 10158.0 - a / EARTH_GRAVITATIONAL_ACCELERATION * \
+        _galewsky_integral(lat, u_0, lat_0, lat_1, radius)
+    alpha, beta = 1 / 3, 1 / 15
+    lat_2 = np.pi / 4
+    bump = 120.0 * np.cos(lat) * np.exp(-((lon / alpha) ** 2)) * \
+        np.exp(-((lat_2 - lat) / beta) ** 2)
+    h = np.where((-np.pi < lon) & (lon < np.pi), h + bump, h)
+    return _spherical_state(h, vlon, np.zeros_like(h), np.zeros_like(h), x)
+
+
+def initial_condition_zonal_jet_bumps(seed, radius=6.371e6, gravity=9.81,
+                                      rotation_rate=EARTH_ROTATION_RATE):
+    """Seeded balanced zonal jet plus three Gaussian height bumps: solid-body
+    zonal flow u_0 cos(lat) with u_0 ~ U[20, 40] m/s in geostrophic balance
+    with h_0 = 5960 m (Williamson et al. 1992, case 2 form), plus three bumps
+    A exp(-(d / w)^2) of amplitude A ~ U[10, 120] m, width w ~ U[300, 1500] km
+    and uniformly distributed centres (d = great-circle distance)."""
+    rng = np.random.default_rng(seed)
+    u_0 = rng.uniform(20.0, 40.0)
+    amp = rng.uniform(10.0, 120.0, size=3)
+    width = rng.uniform(3.0e5, 1.5e6, size=3)
+    zc = rng.uniform(-1.0, 1.0, size=3)
+    lc = rng.uniform(-np.pi, np.pi, size=3)
+    centres = np.stack([np.sqrt(1 - zc ** 2) * np.cos(lc), np.sqrt(1 - zc ** 2) * np.sin(lc), zc])
+
+    def ic(x, t):
+        a, _, lat = _lon_lat(x)
+        h = 5960.0 - (a * rotation_rate * u_0 + 0.5 * u_0 ** 2) * np.sin(lat) ** 2 / gravity
+        nrm = x / a
+        for m in range(3):
+            cosd = np.clip(centres[0, m] * nrm[0] + centres[1, m] * nrm[1] +
+                           centres[2, m] * nrm[2], -1.0, 1.0)
+            d = a * np.arccos(cosd)
+            h = h + amp[m] * np.exp(-(d / width[m]) ** 2)
+        vlon = u_0 * np.cos(lat)
+        return _spherical_state(h, vlon, np.zeros_like(h), np.zeros_like(h), x)
+
+    return ic
+
+
+def transform_initial_condition(initial_condition):
+    """src/equations/equations.jl:94-101: (x, t, aux) -> conserved variables
+    (h, h v^1, h v^2) with v^i = a^i . v and h = H - h_s, h_s from the
+    auxiliary variables (row 19), aux [26, ...], x [3, ...]."""
+
+    def transformed(x, t, aux):
+        H, vx, vy, vz, _ = initial_condition(x, t)
+        # basis_contravariant: 2 x 3, column-major in aux rows 6..11
+        vcon1 = aux[6] * vx + aux[8] * vy + aux[10] * vz
+        vcon2 = aux[7] * vx + aux[9] * vy + aux[11] * vz
+        h = H - aux[19]
+        return np.stack([h, h * vcon1, h * vcon2])
+
+    return transformed

@@ -91,6 +91,88 @@ def P4estMeshCubedSphereTopography(trees_per_face_dimension, layers, inner_radiu
     return P4estMesh3D(h, polydeg)
 
 
+class P4estMeshCubedSphere2DData:
+    """A level-0 P4estMesh{2} of the sphere (one element per tree) as
+    P4estMeshCubedSphere2D builds it, with the 26 auxiliary variables per node
+    of the covariant form.
+
+    Attributes (numpy, Julia layout): node_coordinates [3,n,n,E],
+    aux_node_vars [26,n,n,E] (bottom topography row 19 zero),
+    interface_neighbor_ids [2,I], interface_node_indices [2,2,I] (codes
+    begin/end/i_forward/i_backward).
+    """
+
+    def __init__(self, trees_per_face_dimension, radius, polydeg, christoffel_collocation):
+        L = lib()
+        h = ctypes.c_void_p()
+        check(L.atmo_mesh2d_cubed_sphere(polydeg, trees_per_face_dimension, float(radius),
+                                         1 if christoffel_collocation else 0, ctypes.byref(h)))
+        try:
+            ne, ni = ctypes.c_int32(), ctypes.c_int32()
+            check(L.atmo_mesh2d_sizes(h, ctypes.byref(ne), ctypes.byref(ni)))
+            pd = [c_double_p() for _ in range(2)]
+            pi = [c_int32_p() for _ in range(2)]
+            check(L.atmo_mesh2d_arrays(h, *[ctypes.byref(p) for p in pd],
+                                       *[ctypes.byref(p) for p in pi]))
+            n, E, I = polydeg + 1, ne.value, ni.value
+
+            def copy(p, shape):
+                cnt = int(np.prod(shape))
+                return np.ctypeslib.as_array(p, shape=(cnt,)).copy().reshape(shape, order="F")
+
+            self.node_coordinates = copy(pd[0], (3, n, n, E))
+            self.aux_node_vars = copy(pd[1], (26, n, n, E))
+            self.interface_neighbor_ids = copy(pi[0], (2, I))
+            self.interface_node_indices = copy(pi[1], (2, 2, I))
+        finally:
+            L.atmo_mesh2d_destroy(h)
+        self.polydeg = polydeg
+        self.trees_per_face_dimension = trees_per_face_dimension
+        self.radius = float(radius)
+        self.christoffel_collocation = bool(christoffel_collocation)
+        self.n_elements, self.n_interfaces = E, I
+
+    @property
+    def nnodes(self):
+        return self.polydeg + 1
+
+
+class P4estMeshCubedSphere2D:
+    """`P4estMeshCubedSphere2D(trees_per_face_dimension, radius; polydeg,
+    element_local_mapping = true)` (src/meshes/p4est_cubed_sphere.jl:47-79)
+    with initial_refinement_level = 0.
+
+    The auxiliary variables depend on the metric terms chosen at the
+    semidiscretization (`MetricTermsCovariantSphere(christoffel_symbols =
+    ...)`); `data(christoffel_collocation)` builds them for either choice.
+    """
+
+    def __init__(self, trees_per_face_dimension, radius, *, polydeg,
+                 element_local_mapping=True):
+        if not element_local_mapping:
+            raise ValueError("the covariant form uses element_local_mapping = true")
+        self.trees_per_face_dimension = int(trees_per_face_dimension)
+        self.radius = float(radius)
+        self.polydeg = int(polydeg)
+        self._data = {}
+        d = self.data(False)
+        self.node_coordinates = d.node_coordinates
+        self.interface_neighbor_ids = d.interface_neighbor_ids
+        self.interface_node_indices = d.interface_node_indices
+        self.n_elements, self.n_interfaces = d.n_elements, d.n_interfaces
+
+    def data(self, christoffel_collocation=False):
+        key = bool(christoffel_collocation)
+        if key not in self._data:
+            self._data[key] = P4estMeshCubedSphere2DData(self.trees_per_face_dimension,
+                                                         self.radius, self.polydeg, key)
+        return self._data[key]
+
+    @property
+    def nnodes(self):
+        return self.polydeg + 1
+
+
 def lgl_basis(polydeg):
     """LobattoLegendreBasis(polydeg): (nodes, weights, D, Dsplit)."""
     n = polydeg + 1

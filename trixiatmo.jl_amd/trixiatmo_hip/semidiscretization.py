@@ -8,8 +8,11 @@ import ctypes
 
 import numpy as np
 
-from ._lib import AtmoConfig, AtmoGeometry3D, check, dptr, iptr, lib
-from .equations import EARTH_ROTATION_RATE
+from ._lib import (AtmoConfig, AtmoGeometry2D, AtmoGeometry3D, AtmoSweConfig, check, dptr, iptr,
+                   lib)
+from .equations import (EARTH_ROTATION_RATE, SWE_SURFACE_FLUXES, SWE_SURFACE_NONCONS,
+                        SWE_VOLUME_FLUXES, SWE_VOLUME_NONCONS,
+                        SplitCovariantShallowWaterEquations2D)
 
 ATMO_EQ_EULER_ENERGY_GRAVITY_3D = 1
 ATMO_BC_SLIP_WALL = 0
@@ -18,9 +21,68 @@ BOUNDARY_CONDITIONS = {"boundary_condition_slip_wall": ATMO_BC_SLIP_WALL}
 SOURCE_TERMS = {None: (0, 0.0), "source_terms_coriolis": (1, EARTH_ROTATION_RATE)}
 
 
+# covariant source terms -> ATMO_SRC_* of atmo_swe_config
+COVARIANT_SOURCE_TERMS = {None: 0, "source_terms_geometric_coriolis": 1}
+
+
+class MetricTermsCovariantSphere:
+    """`MetricTermsCovariantSphere(christoffel_symbols = ChristoffelSymbolsAutodiff())`
+    (src/semidiscretization/metric_terms.jl:28-73): the Christoffel symbols
+    come from exact derivatives of the analytic metric ("autodiff") or from
+    the collocation derivative of the nodal metric ("collocation")."""
+
+    def __init__(self, christoffel_symbols="ChristoffelSymbolsAutodiff()"):
+        if christoffel_symbols not in ("ChristoffelSymbolsAutodiff()",
+                                       "ChristoffelSymbolsCollocationDerivative()"):
+            raise ValueError(f"unknown christoffel_symbols {christoffel_symbols}")
+        self.christoffel_symbols = christoffel_symbols
+
+    @property
+    def collocation(self):
+        return self.christoffel_symbols == "ChristoffelSymbolsCollocationDerivative()"
+
+
+def _swe_config(equations, solver, n_elements, source_terms):
+    vol, vnc = solver.volume_integral.volume_flux
+    sf, snc = solver.surface_flux
+    if vnc not in SWE_VOLUME_NONCONS:
+        raise ValueError(f"unsupported volume nonconservative flux {vnc}")
+    if solver.precision != "fp64":
+        raise ValueError("the covariant form runs in FP64")
+    return AtmoSweConfig(polydeg=solver.polydeg, n_elements=n_elements,
+                         volume_flux=SWE_VOLUME_FLUXES[vol], surface_flux=SWE_SURFACE_FLUXES[sf],
+                         surface_noncons=SWE_SURFACE_NONCONS[snc],
+                         source_terms=COVARIANT_SOURCE_TERMS[source_terms], precision=0,
+                         gravity=equations.gravity, rotation_rate=equations.rotation_rate)
+
+
+def covariant_aux(mesh, metric_terms=None, auxiliary_field=None):
+    """aux_node_vars [26,n,n,E] of init_auxiliary_node_variables!
+    (src/solvers/dgsem_p4est/containers_2d_manifold_in_3d_covariant.jl:232-305),
+    the bottom topography row filled from `auxiliary_field(x)` at the nodes."""
+    mt = metric_terms or MetricTermsCovariantSphere()
+    aux = mesh.data(mt.collocation).aux_node_vars.copy(order="F")
+    if auxiliary_field is not None:
+        aux[19] = auxiliary_field(mesh.node_coordinates)
+    return aux
+
+
 class SemidiscretizationHyperbolic:
-    def __init__(self, mesh, equations, solver, boundary_conditions=None, source_terms=None):
+    """`SemidiscretizationHyperbolic(mesh, equations, initial_condition, solver;
+    source_terms, boundary_conditions)`.  For P4estMesh{3} the initial state
+    is built by the caller; for the covariant form on P4estMeshCubedSphere2D
+    (src/semidiscretization/semidiscretization_hyperbolic_2d_manifold_in_3d.jl:78-111)
+    `initial_condition` is a transformed initial condition (x, t, aux) -> u,
+    and `metric_terms` / `auxiliary_field` are the reference's keywords."""
+
+    def __init__(self, mesh, equations, solver, boundary_conditions=None, source_terms=None,
+                 initial_condition=None, metric_terms=None, auxiliary_field=None):
         self.mesh, self.equations, self.solver = mesh, equations, solver
+        self.initial_condition = initial_condition
+        self.source_terms = source_terms
+        if isinstance(equations, SplitCovariantShallowWaterEquations2D):
+            self._init_covariant(source_terms, metric_terms, auxiliary_field)
+            return
         if solver.polydeg != mesh.polydeg:
             raise ValueError("solver and mesh polydeg differ")
         codes = solver.codes()
@@ -54,6 +116,38 @@ class SemidiscretizationHyperbolic:
         n = mesh.nnodes
         self.shape = (equations.nvariables, n, n, n, mesh.n_elements)
 
+    def _init_covariant(self, source_terms, metric_terms, auxiliary_field):
+        mesh, solver = self.mesh, self.solver
+        if solver.polydeg != mesh.polydeg:
+            raise ValueError("solver and mesh polydeg differ")
+        self.aux_node_vars = covariant_aux(mesh, metric_terms, auxiliary_field)
+        cfg = _swe_config(self.equations, solver, mesh.n_elements, source_terms)
+        k = [np.ascontiguousarray(self.aux_node_vars.ravel(order="F")),
+             np.ascontiguousarray(mesh.node_coordinates.ravel(order="F")),
+             np.ascontiguousarray(mesh.interface_neighbor_ids.ravel(order="F"), dtype=np.int32),
+             np.ascontiguousarray(mesh.interface_node_indices.ravel(order="F"), dtype=np.int32)]
+        geo = AtmoGeometry2D(aux_node_vars=dptr(k[0]), node_coordinates=dptr(k[1]),
+                             n_interfaces=mesh.n_interfaces, interface_neighbor_ids=iptr(k[2]),
+                             interface_node_indices=iptr(k[3]), n_members=1,
+                             member_first_element=None)
+        h = ctypes.c_void_p()
+        check(lib().atmo_swe_create(ctypes.byref(cfg), ctypes.byref(geo), ctypes.byref(h)))
+        self._h = h
+        n = mesh.nnodes
+        self.shape = (3, n, n, mesh.n_elements)
+        self.n_members = 1
+
+    def compute_coefficients(self, t=0.0):
+        """Trixi.compute_coefficients(initial_condition, t, semi) for the
+        covariant form: the conserved state at the nodes, [3,n,n,E]."""
+        if self.initial_condition is None:
+            raise ValueError("no initial condition")
+        return self.initial_condition(self.mesh.node_coordinates, t, self.aux_node_vars)
+
+    @property
+    def is_covariant(self):
+        return isinstance(self.equations, SplitCovariantShallowWaterEquations2D)
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
@@ -66,8 +160,8 @@ class SemidiscretizationHyperbolic:
 
     @property
     def ndofs(self):
-        """Trixi's DOF count: nodes (E n^3), not nodes x variables."""
-        return self.mesh.n_elements * self.mesh.nnodes ** 3
+        """Trixi's DOF count: nodes (E n^d), not nodes x variables."""
+        return int(np.prod(self.shape[1:]))
 
     def _flat(self, u):
         a = np.asarray(u, dtype=np.float64)
@@ -109,7 +203,7 @@ class SemidiscretizationHyperbolic:
 
     def integrals(self, fresh_rhs=True):
         """(mass, entropy, dS/dt, integral of each conserved variable...)"""
-        out = np.zeros(9)
+        out = np.zeros(3 if self.is_covariant else 9)
         check(lib().atmo_integrals(self._h, 1 if fresh_rhs else 0, dptr(out)), self._h)
         return out
 
@@ -125,3 +219,88 @@ class SemidiscretizationHyperbolic:
         check(lib().atmo_kernel_times(self._h, kernel_class, ctypes.byref(n), ctypes.byref(ms)),
               self._h)
         return n.value, ms.value
+
+
+class SemidiscretizationEnsemble(SemidiscretizationHyperbolic):
+    """Independent covariant shallow-water members on their own cubed-sphere
+    meshes (uneven element counts) in one handle: every RK stage is one
+    launch over all members' elements, each member advancing with its own
+    time step.  `members` is a list of (mesh, transformed initial condition,
+    auxiliary_field or None)."""
+
+    def __init__(self, members, equations, solver, source_terms=None, metric_terms=None):
+        self.equations, self.solver = equations, solver
+        self.source_terms = source_terms
+        self.mesh = None
+        self.initial_condition = None
+        auxs, xyzs, nbrs, idxs, first = [], [], [], [], [0]
+        self.members = []
+        for mesh, ic, field in members:
+            if mesh.polydeg != solver.polydeg:
+                raise ValueError("solver and mesh polydeg differ")
+            aux = covariant_aux(mesh, metric_terms, field)
+            auxs.append(aux)
+            xyzs.append(mesh.node_coordinates)
+            nbrs.append(mesh.interface_neighbor_ids + first[-1])
+            idxs.append(mesh.interface_node_indices)
+            self.members.append((mesh, ic, aux, first[-1]))
+            first.append(first[-1] + mesh.n_elements)
+        self.aux_node_vars = np.concatenate(auxs, axis=-1)
+        self.node_coordinates = np.concatenate(xyzs, axis=-1)
+        nbr = np.concatenate(nbrs, axis=-1)
+        idx = np.concatenate(idxs, axis=-1)
+        E = first[-1]
+        self.member_first_element = np.asarray(first, dtype=np.int32)
+        self.n_members = len(members)
+        cfg = _swe_config(equations, solver, E, source_terms)
+        k = [np.ascontiguousarray(self.aux_node_vars.ravel(order="F")),
+             np.ascontiguousarray(self.node_coordinates.ravel(order="F")),
+             np.ascontiguousarray(nbr.ravel(order="F"), dtype=np.int32),
+             np.ascontiguousarray(idx.ravel(order="F"), dtype=np.int32),
+             self.member_first_element]
+        geo = AtmoGeometry2D(aux_node_vars=dptr(k[0]), node_coordinates=dptr(k[1]),
+                             n_interfaces=nbr.shape[1], interface_neighbor_ids=iptr(k[2]),
+                             interface_node_indices=iptr(k[3]), n_members=self.n_members,
+                             member_first_element=iptr(k[4]))
+        h = ctypes.c_void_p()
+        check(lib().atmo_swe_create(ctypes.byref(cfg), ctypes.byref(geo), ctypes.byref(h)))
+        self._h = h
+        n = solver.polydeg + 1
+        self.shape = (3, n, n, E)
+
+    @property
+    def is_covariant(self):
+        return True
+
+    def compute_coefficients(self, t=0.0):
+        parts = [ic(mesh.node_coordinates, t, aux) for mesh, ic, aux, _ in self.members]
+        return np.concatenate(parts, axis=-1)
+
+    def max_dt_members(self, cfl):
+        """StepsizeCallback max_dt of every member, [n_members]."""
+        dt = np.zeros(self.n_members)
+        check(lib().atmo_max_dt_members(self._h, float(cfl), dptr(dt)), self._h)
+        return dt
+
+    def step_members(self, dt, n_steps=1):
+        """n_steps CarpenterKennedy2N54 steps, member m with time step dt[m]."""
+        d = np.ascontiguousarray(dt, dtype=np.float64)
+        if d.shape != (self.n_members,):
+            raise ValueError("one time step per member")
+        check(lib().atmo_step_members(self._h, dptr(d), int(n_steps)), self._h)
+
+
+def solve(semi, u0, tspan, cfl):
+    """`solve(ode, CarpenterKennedy2N54(williamson_condition = false);
+    callback = StepsizeCallback(cfl))` on the resident state: before every
+    step dt = cfl * max_dt(u), and the last step is cut to end at tspan[1]
+    as OrdinaryDiffEq's tstop handling does.  Returns (final state, steps)."""
+    t, t_end = float(tspan[0]), float(tspan[1])
+    semi.set_state(u0)
+    steps = 0
+    while t < t_end:
+        dt = min(semi.max_dt(cfl), t_end - t)
+        semi.step(dt, 1)
+        t += dt
+        steps += 1
+    return semi.get_state().reshape(semi.shape, order="F"), steps

@@ -70,3 +70,143 @@ def baroclinic_instability(trees_per_face=8, layers=4, polydeg=5, precision="fp6
     u0 = initial_condition_baroclinic_instability(mesh.node_coordinates, eq)
     bcs = {"inside": "boundary_condition_slip_wall", "outside": "boundary_condition_slip_wall"}
     return mesh, eq, solver, bcs, u0
+
+
+# ---------------------------------------------------------------------------
+# covariant shallow water on the cubed sphere
+
+SWE_EC_LLF = "FluxPlusDissipation(flux_ec, DissipationLocalLaxFriedrichs())"
+
+
+class CovariantCase:
+    """mesh, equations, solver and the semidiscretization keywords of one
+    covariant shallow-water setup; `semi()` creates the HIP semidiscretization,
+    `aux()` the host auxiliary variables (no device needed)."""
+
+    def __init__(self, mesh, equations, solver, initial_condition, source_terms,
+                 auxiliary_field=None, christoffel="ChristoffelSymbolsAutodiff()", cfl=0.4):
+        from .semidiscretization import MetricTermsCovariantSphere
+        self.mesh, self.equations, self.solver = mesh, equations, solver
+        self.initial_condition = initial_condition
+        self.source_terms = source_terms
+        self.auxiliary_field = auxiliary_field
+        self.metric_terms = MetricTermsCovariantSphere(christoffel)
+        self.cfl = cfl
+
+    def aux(self):
+        from .semidiscretization import covariant_aux
+        return covariant_aux(self.mesh, self.metric_terms, self.auxiliary_field)
+
+    def u0(self, t=0.0):
+        return self.initial_condition(self.mesh.node_coordinates, t, self.aux())
+
+    def semi(self):
+        from .semidiscretization import SemidiscretizationHyperbolic
+        return SemidiscretizationHyperbolic(self.mesh, self.equations, self.solver,
+                                            source_terms=self.source_terms,
+                                            initial_condition=self.initial_condition,
+                                            metric_terms=self.metric_terms,
+                                            auxiliary_field=self.auxiliary_field)
+
+
+def covariant_elixir(name, cells=5, polydeg=3, **override):
+    """The split covariant elixirs of examples/shallow_water/covariant with the
+    Cartesian global coordinate system, at `cells` trees per face dimension:
+    "unsteady_solid_body_rotation_EC", "barotropic_instability",
+    "isolated_mountain", "rossby_haurwitz", "well_balanced".  `override`
+    replaces volume_flux / surface_flux / christoffel as the reference's tests
+    do through @test_trixi_include."""
+    from . import initial_conditions as ic
+    from .equations import (EARTH_GRAVITATIONAL_ACCELERATION, EARTH_ROTATION_RATE,
+                            SplitCovariantShallowWaterEquations2D)
+    from .meshes import P4estMeshCubedSphere2D
+    eq = SplitCovariantShallowWaterEquations2D(EARTH_GRAVITATIONAL_ACCELERATION,
+                                               EARTH_ROTATION_RATE)
+    mesh = P4estMeshCubedSphere2D(cells, EARTH_RADIUS, polydeg=polydeg)
+    spec = {
+        # examples/shallow_water/covariant/elixir_unsteady_solid_body_rotation_EC.jl:20-46
+        "unsteady_solid_body_rotation_EC": dict(
+            ic=ic.initial_condition_unsteady_solid_body_rotation,
+            field=ic.bottom_topography_unsteady_solid_body_rotation,
+            volume_flux=("flux_ec", "flux_nonconservative_ec"),
+            surface_flux=("flux_ec", "flux_nonconservative_surface_simplified"), cfl=0.4),
+        # elixir_barotropic_instability.jl:20-67
+        "barotropic_instability": dict(
+            ic=ic.initial_condition_barotropic_instability, field=None,
+            volume_flux=("flux_ec", "flux_nonconservative_ec"),
+            surface_flux=(SWE_EC_LLF, "flux_nonconservative_surface_simplified"), cfl=0.4),
+        # elixir_isolated_mountain.jl:21-71
+        "isolated_mountain": dict(
+            ic=ic.initial_condition_isolated_mountain,
+            field=ic.bottom_topography_isolated_mountain,
+            volume_flux=("flux_ec", "flux_nonconservative_ec"),
+            surface_flux=(SWE_EC_LLF, "flux_nonconservative_surface_simplified"), cfl=0.4),
+        # elixir_rossby_haurwitz.jl:20-70
+        "rossby_haurwitz": dict(
+            ic=ic.initial_condition_rossby_haurwitz, field=None,
+            volume_flux=("flux_ec", "flux_nonconservative_ec"),
+            surface_flux=(SWE_EC_LLF, "flux_nonconservative_surface_simplified"), cfl=0.4),
+        # elixir_well_balanced.jl:13-73
+        "well_balanced": dict(
+            ic=ic.initial_condition_well_balanced,
+            field=ic.bottom_topography_isolated_mountain,
+            volume_flux=("flux_ec", "flux_nonconservative_ec"),
+            surface_flux=(SWE_EC_LLF, "flux_nonconservative_ec"), cfl=0.1),
+    }[name]
+    spec.update(override)
+    solver = DGSEM(polydeg=polydeg, surface_flux=tuple(spec["surface_flux"]),
+                   volume_integral=VolumeIntegralFluxDifferencing(tuple(spec["volume_flux"])))
+    return CovariantCase(mesh, eq, solver, ic.transform_initial_condition(spec["ic"]),
+                         "source_terms_geometric_coriolis", spec["field"],
+                         spec.get("christoffel", "ChristoffelSymbolsAutodiff()"), spec["cfl"])
+
+
+SWE_RADIUS = 6.371e6
+SWE_GRAVITY = 9.81
+
+
+def zonal_jet(trees_per_face=32, polydeg=3, seed=0):
+    """Covariant shallow water on the cubed sphere of radius 6.371e6 m with
+    g = 9.81: a seeded balanced zonal jet (u_0 in [20, 40] m/s, mean depth
+    5960 m) plus three Gaussian height bumps; flux_ec +
+    flux_nonconservative_ec volume, FluxPlusDissipation(flux_ec,
+    DissipationLocalLaxFriedrichs()) + flux_nonconservative_surface_simplified
+    surface, source_terms_geometric_coriolis, FP64."""
+    from . import initial_conditions as ic
+    from .equations import EARTH_ROTATION_RATE, SplitCovariantShallowWaterEquations2D
+    from .meshes import P4estMeshCubedSphere2D
+    eq = SplitCovariantShallowWaterEquations2D(SWE_GRAVITY, EARTH_ROTATION_RATE)
+    mesh = P4estMeshCubedSphere2D(trees_per_face, SWE_RADIUS, polydeg=polydeg)
+    solver = DGSEM(polydeg=polydeg,
+                   surface_flux=(SWE_EC_LLF, "flux_nonconservative_surface_simplified"),
+                   volume_integral=VolumeIntegralFluxDifferencing(
+                       ("flux_ec", "flux_nonconservative_ec")))
+    jet = ic.initial_condition_zonal_jet_bumps(seed, radius=SWE_RADIUS, gravity=SWE_GRAVITY)
+    return CovariantCase(mesh, eq, solver, ic.transform_initial_condition(jet),
+                         "source_terms_geometric_coriolis")
+
+
+def ensemble_sizes(n_members=32, seed=0, choices=(8, 16, 24, 32)):
+    """trees per face dimension of each member, drawn uniformly from `choices`"""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return [int(k) for k in rng.choice(choices, size=n_members)]
+
+
+def zonal_jet_ensemble(n_members=32, polydeg=3, seed=0, choices=(8, 16, 24, 32)):
+    """Ensemble of independent zonal-jet members with uneven element counts
+    6 k^2, k drawn from `choices`; member m draws its jet and bumps from seed
+    1000 * seed + m + 1.  Returns (members [(mesh, ic, None)], equations,
+    solver, source_terms) for SemidiscretizationEnsemble."""
+    from . import initial_conditions as ic
+    sizes = ensemble_sizes(n_members, seed, choices)
+    meshes = {}
+    members = []
+    for m, k in enumerate(sizes):
+        if k not in meshes:
+            meshes[k] = zonal_jet(k, polydeg).mesh
+        jet = ic.initial_condition_zonal_jet_bumps(1000 * seed + m + 1, radius=SWE_RADIUS,
+                                                   gravity=SWE_GRAVITY)
+        members.append((meshes[k], ic.transform_initial_condition(jet), None))
+    base = zonal_jet(1, polydeg)
+    return members, base.equations, base.solver, base.source_terms
