@@ -41,7 +41,9 @@ def check_rhs(oracle, case, u):
     plain = diff / np.maximum(np.abs(du_ref.reshape(3, -1, order="F")).max(axis=1), 1e-300)
     print("  rel err per variable (term scale):", err, " (|du| scale):", plain, flush=True)
     assert np.all(err <= RHS_RTOL), err
-    assert np.all(plain <= 1e-9), plain
+    # the |du| criterion applies where du is a resolved value, not rounding
+    resolved = np.abs(du_ref.reshape(3, -1, order="F")).max(axis=1) > 1e-10 * mag
+    assert np.all(plain[resolved] <= 1e-9), plain
     return semi, so, du_gpu, du_ref
 
 
