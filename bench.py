@@ -3,10 +3,14 @@
 Workload (default): dry Euler (energy form) with gravity on the cubed-sphere
 shell, 6 x 16^2 trees x 8 layers = 12,288 elements, polydeg 4 (125 nodes per
 element, 1,536,000 nodes), flux_kennedy_gruber + Waruszewski volume,
-flux_lax_friedrichs + Waruszewski surface, slip walls, isothermal 250 K
-background with 10 seeded 1 K bumps.  A step is one CarpenterKennedy2N54 step
-(5 rhs! evaluations, each one fused launch) on the device-resident state.
-GDOF counts nodes (Trixi's convention), not nodes x variables.
+local Lax-Friedrichs (FluxPlusDissipation(flux_kennedy_gruber,
+DissipationLocalLaxFriedrichs())) + Waruszewski surface, slip walls,
+isothermal 250 K background with 10 seeded 1 K bumps, 100 steps.  A step is
+one CarpenterKennedy2N54 step (5 rhs! evaluations, each one fused launch) on
+the device-resident state.  GDOF counts nodes (Trixi's convention), not
+nodes x variables.  The FP64 form is `value`; the mixed form (FP32 fluxes with
+Kahan sums, FP64 state, pressure, RK and hydrostatic background) is reported
+beside it with its error against the FP64 form.
 """
 import argparse
 import json
@@ -19,13 +23,26 @@ sys.path.insert(0, os.path.join(ROOT, "trixiatmo.jl_amd"))
 
 HBM_PEAK_GBS = 8000.0
 FP64_VECTOR_PEAK_TFLOPS = 78.6
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+METRIC = "GDOF\u00b7RHS/s, mixed vs FP64 on 1 GPU at stated rel. err; % of VALU roofline"
+
+# Per-launch HBM traffic of the FP64 stage kernel at the default workload,
+# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch); FETCH_SIZE
+# reads half the bytes of a plain device copy of known size on this GPU, hence
+# the factor 2 (profiles/e3_stage_shell16x8_p4_pmc.txt).
+STAGE_FETCH_KIB, STAGE_WRITE_KIB, FETCH_FACTOR = 1.621e5, 1.363e5, 2.0
+# Executed VALU flops per node and stage (SQ_INSTS_VALU_{ADD,MUL,TRANS}_F64 x 64
+# + SQ_INSTS_VALU_FMA_F64 x 128 per dispatch / nodes, same profile file); the
+# mixed form executes mostly FP32.
+EXEC_FP64_FLOPS_PER_NODE = 1477.0
+EXEC_MIXED_FLOPS_PER_NODE = {"fp32": 1753.0, "fp64": 354.0}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(mesh, eq, solver, u0, seconds=15.0):
+def cpu_baseline(mesh, eq, solver, u0, seconds=10.0):
     """the oracle (FP64 C++ OpenMP restatement of the reference rhs!) on the
     same workload, timed on this host's cores"""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -38,7 +55,7 @@ def cpu_baseline(mesh, eq, solver, u0, seconds=15.0):
         o.rhs(p, mesh, u0)
         n_calls += 1
         el = time.perf_counter() - t0
-        if el > seconds or n_calls >= 20:
+        if (el > seconds and n_calls >= 20) or n_calls >= 400:
             break
     nodes = mesh.n_elements * mesh.nnodes ** 3
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -47,15 +64,84 @@ def cpu_baseline(mesh, eq, solver, u0, seconds=15.0):
             "sample": f"{n_calls} full rhs! calls of the same workload ({el:.1f} s)"}
 
 
+def bench_covariant(steps, warmup, trees=32, polydeg=3, n_members=32):
+    """covariant shallow water on the cubed sphere (radius 6.371e6 m, g = 9.81,
+    polydeg 3, flux_ec volume, EC + LLF surface): one member of 6 x 32^2
+    elements, and an ensemble of 32 members with 6 k^2 elements each, k drawn
+    from {8, 16, 24, 32}, advanced by one launch per RK stage"""
+    import numpy as np
+    import trixiatmo_hip as T
+    W = T.workloads
+    out = {}
+    case = W.zonal_jet(trees, polydeg, seed=0)
+    semi = case.semi()
+    u0 = case.u0()
+    semi.set_state(u0)
+    dt = semi.max_dt(0.4)
+    semi.step(dt, warmup)
+    semi.synchronize()
+    t0 = time.perf_counter()
+    semi.step(dt, steps)
+    semi.synchronize()
+    el = time.perf_counter() - t0
+    semi.enable_timing(True)
+    semi.step(dt, max(2, steps // 2))
+    launches, total_ms = semi.kernel_times(0)
+    semi.enable_timing(False)
+    nodes = semi.ndofs
+    avg_s = total_ms / launches / 1e3
+    # algorithmic bytes per node and stage: u, k read, k and u_out written
+    # (3 FP64 each), the 17 auxiliary values the volume and sources use
+    # (J, G, G^-1, h_s, Christoffel) plus the node position (3)
+    bpn = 8 * (4 * 3 + 17 + 3)
+    out["single"] = {"workload": f"6x{trees}^2 elements, polydeg {polydeg}", "nodes": nodes,
+                     "value": nodes * 5 * steps / el / 1e9, "unit": "GDOF*RHS/s",
+                     "ms_per_step": el / steps * 1e3, "avg_launch_us": avg_s * 1e6,
+                     "hbm_achieved_gbs": bpn * nodes / avg_s / 1e9, "bytes_per_node": bpn,
+                     "finite": bool(np.all(np.isfinite(semi.get_state())))}
+    del semi
+    members, eq, solver, src = W.zonal_jet_ensemble(n_members=n_members, polydeg=polydeg, seed=0)
+    ens = T.SemidiscretizationEnsemble(members, eq, solver, source_terms=src)
+    u0 = ens.compute_coefficients()
+    ens.set_state(u0)
+    dts = ens.max_dt_members(0.4)
+    ens.step_members(dts, warmup)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.step_members(dts, steps)
+    ens.synchronize()
+    el = time.perf_counter() - t0
+    ens.enable_timing(True)
+    ens.step_members(dts, max(2, steps // 2))
+    launches, total_ms = ens.kernel_times(0)
+    ens.enable_timing(False)
+    nodes = ens.ndofs
+    avg_s = total_ms / launches / 1e3
+    out["ensemble"] = {"workload": f"{n_members} members, 6k^2 elements, k in {{8,16,24,32}}",
+                       "sizes": W.ensemble_sizes(n_members, 0), "nodes": nodes,
+                       "value": nodes * 5 * steps / el / 1e9, "unit": "GDOF*RHS/s",
+                       "ms_per_step": el / steps * 1e3, "avg_launch_us": avg_s * 1e6,
+                       "hbm_achieved_gbs": bpn * nodes / avg_s / 1e9,
+                       "finite": bool(np.all(np.isfinite(ens.get_state())))}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--timing-steps", type=int, default=4,
+                    help="steps run with per-launch HIP-event timing after the timed region")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--trees", type=int, default=16)
     ap.add_argument("--layers", type=int, default=8)
     ap.add_argument("--polydeg", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-covariant", action="store_true",
+                    help="skip the covariant shallow-water lines")
+    ap.add_argument("--traffic-kib", type=float, nargs=3, default=None,
+                    metavar=("FETCH", "WRITE", "FACTOR"),
+                    help="rocprofv3 FETCH_SIZE / WRITE_SIZE of the stage kernel and the FETCH scale")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (to scale HBM counters)")
     a = ap.parse_args()
@@ -79,15 +165,17 @@ def main():
         log(f"copy calibration: 3 device copies of {nbytes} bytes each")
 
     log("building workload")
-    runs = {}
+    semis, results = {}, {}
+    dt = None
     for prec in ("fp64", "fp32_kahan"):
         mesh, eq, solver, bcs, u0 = T.workloads.shell(a.trees, a.layers, polydeg=a.polydeg, seed=0,
                                                       precision=prec)
         semi = T.SemidiscretizationHyperbolic(mesh, eq, solver, bcs)
         nodes = mesh.n_elements * mesh.nnodes ** 3
-        du_one = semi.rhs(None, u0)
         semi.set_state(u0)
-        dt = semi.max_dt(0.5)
+        if dt is None:
+            dt = semi.max_dt(0.5)
+        i0 = semi.integrals(fresh_rhs=True)
         log(f"[{prec}] warmup {a.warmup} steps, dt = {dt:.4g} s")
         semi.step(dt, a.warmup)
         semi.synchronize()
@@ -98,24 +186,49 @@ def main():
         log(f"[{prec}] timed {a.steps} steps in {el:.4f} s")
         # per-launch duration of the fused stage kernel from HIP events on its stream
         semi.enable_timing(True)
-        semi.step(dt, max(2, a.steps // 2))
+        semi.step(dt, a.timing_steps)
         launches, total_ms = semi.kernel_times(0)
         semi.enable_timing(False)
-        ok = bool(np.all(np.isfinite(semi.get_state())))
-        runs[prec] = dict(el=el, gdof=nodes * 5 * a.steps / el / 1e9, avg_s=total_ms / launches / 1e3,
-                          du=du_one, ok=ok)
-        del semi
-    f64, mix = runs["fp64"], runs["fp32_kahan"]
+        u_end = semi.get_state()
+        i1 = semi.integrals(fresh_rhs=True)
+        results[prec] = dict(el=el, gdof=nodes * 5 * a.steps / el / 1e9,
+                             avg_s=total_ms / launches / 1e3, u_end=u_end,
+                             ok=bool(np.all(np.isfinite(u_end))),
+                             mass_drift=(i1[0] - i0[0]) / i0[0],
+                             entropy_drift=(i1[1] - i0[1]) / abs(i0[1]))
+        semis[prec] = semi
+    f64, mix = results["fp64"], results["fp32_kahan"]
     gdof, el, avg_s, ok = f64["gdof"], f64["el"], f64["avg_s"], f64["ok"] and mix["ok"]
+    n_total = a.warmup + a.steps + a.timing_steps
+    # RHS error of the mixed form against the FP64 form on the same evolved
+    # state (the FP64 state after all steps), and the state error after the run
+    d64 = semis["fp64"].rhs(None, f64["u_end"]).reshape(6, -1, order="F")
+    dmx = semis["fp32_kahan"].rhs(None, f64["u_end"]).reshape(6, -1, order="F")
+    rhs_err = (np.abs(dmx - d64).max(axis=1) / np.abs(d64).max(axis=1))[:5]
+    u64 = f64["u_end"].reshape(6, -1, order="F")
+    umx = mix["u_end"].reshape(6, -1, order="F")
+    u_start = u0.reshape(6, -1, order="F")
+    state_err = (np.abs(umx - u64).max(axis=1) / np.abs(u64).max(axis=1))[:5]
+    change_err = (np.abs(umx - u64).max(axis=1) /
+                  np.maximum(np.abs(u64 - u_start).max(axis=1), 1e-300))[:5]
+    semis.clear()
     # algorithmic HBM bytes of one fused stage per node: read u, write u_out,
     # read + write the RK register k (6 FP64 each), read Ja (9) and 1/J (1);
     # the mixed path reads Ja in FP32
     bytes_per_node = 8 * (6 + 6 + 6 + 6 + 9 + 1)
     bytes_mixed = 8 * (6 + 6 + 6 + 6 + 1) + 4 * 9
     achieved = bytes_per_node * nodes / avg_s / 1e9
-    d64 = f64["du"].reshape(6, -1, order="F")
-    dmx = mix["du"].reshape(6, -1, order="F")
-    rel_err = (np.abs(dmx - d64).max(axis=1) / np.abs(d64).max(axis=1))[:5]
+    fetch_kib, write_kib, fetch_factor = a.traffic_kib or (STAGE_FETCH_KIB, STAGE_WRITE_KIB,
+                                                           FETCH_FACTOR)
+    traffic = (fetch_kib * fetch_factor + write_kib) * 1024.0
+    valu64 = EXEC_FP64_FLOPS_PER_NODE * nodes / avg_s / 1e12
+    valu_mix = (EXEC_MIXED_FLOPS_PER_NODE["fp32"] / FP32_VECTOR_PEAK_TFLOPS +
+                EXEC_MIXED_FLOPS_PER_NODE["fp64"] / FP64_VECTOR_PEAK_TFLOPS) * nodes / mix["avg_s"] / 1e12
+
+    cov = None
+    if not a.no_covariant:
+        log("covariant shallow water")
+        cov = bench_covariant(a.steps, a.warmup)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -123,7 +236,7 @@ def main():
         cpu = cpu_baseline(mesh, eq, solver, u0)
 
     out = {
-        "metric": "GDOF*RHS/s (FP64 rhs! on 1 GPU)",
+        "metric": METRIC,
         "value": gdof,
         "unit": "GDOF*RHS/s",
         "n_gpus": 1,
@@ -140,16 +253,27 @@ def main():
                                "CarpenterKennedy2N54",
                    "elements": mesh.n_elements, "nodes": nodes, "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "e3_stage<5,1,1,double,false>", "avg_launch_us": avg_s * 1e6,
-                     "bytes_per_node": bytes_per_node},
+                     "bytes_per_node": bytes_per_node, "traffic_bytes_per_node": traffic / nodes},
+        "valu": {"fp64_executed_tflops": valu64,
+                 "frac_of_fp64_vector_peak": valu64 / FP64_VECTOR_PEAK_TFLOPS,
+                 "mixed_frac_of_vector_peak": valu_mix,
+                 "note": "executed VALU flops from rocprofv3 SQ_INSTS_VALU_* counters"},
         "cpu_baseline": cpu,
         "mixed": {"precision": "fp32 fluxes + Kahan sums, fp64 state/pressure/RK",
                   "value": mix["gdof"], "speedup_vs_fp64": mix["gdof"] / gdof,
                   "ms_per_step": mix["el"] / a.steps * 1e3,
                   "avg_launch_us": mix["avg_s"] * 1e6,
                   "hbm_achieved_gbs": bytes_mixed * nodes / mix["avg_s"] / 1e9,
-                  "rhs_rel_err_vs_fp64_per_var": [float(x) for x in rel_err]},
+                  "rhs_rel_err_vs_fp64_per_var": [float(x) for x in rhs_err],
+                  "state_rel_err_vs_fp64_after_steps": [float(x) for x in state_err],
+                  "state_err_rel_to_change": [float(x) for x in change_err],
+                  "steps_compared": n_total,
+                  "mass_drift": mix["mass_drift"], "entropy_drift": mix["entropy_drift"]},
+        "fp64_drift": {"mass": f64["mass_drift"], "entropy": f64["entropy_drift"],
+                       "steps": n_total},
+        "covariant_swe": cov,
         "state_finite": ok,
     }
     print(json.dumps(out))
