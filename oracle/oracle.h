@@ -149,6 +149,39 @@ int ora_swe_ck2n54(const ora_swe_params* p, const ora_connectivity2d* c, const d
 void ora_swe_integrals(const ora_swe_params* p, const double* aux, const double* u,
                        const double* du, double* out);
 
+/* ---- compressible rainy Euler (CompressibleRainyEulerEquations2D) on a
+ *      uniform Cartesian StructuredMesh{2}; state [9, n, n, E] = (rho_dry,
+ *      rho_moist, rho_rain, rho_v1, rho_v2, energy, rho_vapour, rho_cloud, T),
+ *      elements numbered x fastest ---- */
+typedef struct {
+  int n, nx, ny;
+  double x0, y0, x1, y1;
+  int periodic_y;      /* 0: boundary_condition_simple_slip_wall at y = y0, y1 */
+  int volume_integral; /* 0 VolumeIntegralWeakForm, 1 flux differencing with flux_ec_rain */
+  int source;          /* 0 none, 1 source_terms_rainy, 2 convergence-test source,
+                          3 source_terms_no_phase_change */
+  int surface_flux;    /* 0 flux_lax_friedrichs, 1 flux_ec_rain */
+} ora_rainy_params;
+
+void ora_rainy_saturation_vapour_pressure(double T, double* svp, double* dsvp);
+void ora_rainy_residual(const double* u, const double* guess, double* r);
+void ora_rainy_residual_jacobian(const double* u, const double* guess, double* J);
+int ora_rainy_newton_node(double* u, const double* guess, double tol, int max_iter);
+void ora_rainy_flux(const double* u, const double* normal, double* f);
+void ora_rainy_flux_ec(const double* ul, const double* ur, const double* normal, double* f);
+void ora_rainy_flux_llf(const double* ul, const double* ur, const double* normal, double* f);
+double ora_rainy_max_abs_speed_naive(const double* ul, const double* ur, const double* normal);
+void ora_rainy_source(const double* u, double* s);
+void ora_rainy_node_coordinates(const ora_rainy_params* p, double* xy);
+int ora_rainy_rhs(const ora_rainy_params* p, const double* u, double t, double* du);
+double ora_rainy_max_dt(const ora_rainy_params* p, const double* u);
+int ora_rainy_newton(const ora_rainy_params* p, double* u, double tol, int max_iter,
+                     int* iterations);
+int ora_rainy_ck2n54(const ora_rainy_params* p, double* u, double t, double dt, int n_steps,
+                     double tol, int max_iter);
+void ora_rainy_cons2entropy(const double* u, double* w);
+void ora_rainy_integrals(const ora_rainy_params* p, const double* u, const double* du, double* out);
+
 #ifdef __cplusplus
 }
 #endif

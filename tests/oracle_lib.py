@@ -281,3 +281,121 @@ def params_for(semi_args):
     return Oracle.params(mesh.nnodes, mesh.n_elements, eq.c_p, eq.c_v, eq.gravity,
                          codes["volume_flux"], codes["volume_noncons"], codes["surface_flux"],
                          codes["surface_noncons"], codes["lmars_c"], omega)
+
+
+class RainyParams(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("x0", ctypes.c_double), ("y0", ctypes.c_double), ("x1", ctypes.c_double),
+                ("y1", ctypes.c_double), ("periodic_y", ctypes.c_int),
+                ("volume_integral", ctypes.c_int), ("source", ctypes.c_int),
+                ("surface_flux", ctypes.c_int)]
+
+
+RAINY_SOURCES = {None: 0, "source_terms_rainy": 1, "source_terms_convergence_test_rainy": 2,
+                 "source_terms_no_phase_change": 3}
+
+
+class RainyOracle:
+    """The rainy-Euler oracle bound to one StructuredMesh2D and solver choice:
+    volume_integral 0 weak form / 1 flux_ec_rain, surface_flux 0 LLF / 1
+    flux_ec_rain."""
+
+    def __init__(self, oracle, mesh, volume_integral=1, source=None, surface_flux=0):
+        self.L = oracle.L
+        for fn in ("ora_rainy_max_dt", "ora_rainy_max_abs_speed_naive"):
+            getattr(self.L, fn).restype = ctypes.c_double
+        self.p = RainyParams(mesh.nnodes, mesh.nx, mesh.ny, mesh.x0, mesh.y0, mesh.x1, mesh.y1,
+                             1 if mesh.periodic_y else 0, volume_integral,
+                             RAINY_SOURCES[source], surface_flux)
+        self.shape = (9, mesh.nnodes, mesh.nnodes, mesh.n_elements)
+
+    def _u(self, u):
+        return np.ascontiguousarray(np.asarray(u).ravel(order="F"), dtype=float)
+
+    def rhs(self, u, t=0.0):
+        uf = self._u(u)
+        du = np.zeros_like(uf)
+        assert self.L.ora_rainy_rhs(ctypes.byref(self.p), _d(uf), ctypes.c_double(t), _d(du)) == 0
+        return du
+
+    def max_dt(self, u):
+        return self.L.ora_rainy_max_dt(ctypes.byref(self.p), _d(self._u(u)))
+
+    def newton(self, u, tol=1e-9, max_iter=20):
+        """NonlinearSolveDG on a copy of u: (u_out, iterations per node, failures)"""
+        uf = self._u(u).copy()
+        its = np.zeros(uf.size // 9, dtype=np.int32)
+        fails = self.L.ora_rainy_newton(ctypes.byref(self.p), _d(uf), ctypes.c_double(tol),
+                                        max_iter, its.ctypes.data_as(ip))
+        return uf, its, fails
+
+    def ck2n54(self, u, t, dt, n_steps=1, tol=1e-9, max_iter=20):
+        uf = self._u(u).copy()
+        assert self.L.ora_rainy_ck2n54(ctypes.byref(self.p), _d(uf), ctypes.c_double(t),
+                                       ctypes.c_double(dt), n_steps, ctypes.c_double(tol),
+                                       max_iter) == 0
+        return uf
+
+    def integrals(self, u, du=None):
+        out = np.zeros(10)
+        dptr = _d(np.ascontiguousarray(du)) if du is not None else None
+        self.L.ora_rainy_integrals(ctypes.byref(self.p), _d(self._u(u)), dptr, _d(out))
+        return out
+
+    def solve(self, u0, t_end, cfl, tol=1e-9, max_iter=20):
+        u = self._u(u0).copy()
+        t, steps = 0.0, 0
+        while t < t_end:
+            dt = min(cfl * self.max_dt(u), t_end - t)
+            u = self.ck2n54(u, t, dt, 1, tol, max_iter)
+            t += dt
+            steps += 1
+        return u, steps
+
+    # pointwise functions
+    def residual(self, u, guess):
+        r = np.zeros(3)
+        self.L.ora_rainy_residual(_d(np.ascontiguousarray(u, float)),
+                                  _d(np.ascontiguousarray(guess, float)), _d(r))
+        return r
+
+    def jacobian(self, u, guess):
+        J = np.zeros(9)
+        self.L.ora_rainy_residual_jacobian(_d(np.ascontiguousarray(u, float)),
+                                           _d(np.ascontiguousarray(guess, float)), _d(J))
+        return J.reshape(3, 3, order="F")
+
+    def svp(self, T):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self.L.ora_rainy_saturation_vapour_pressure(ctypes.c_double(T), ctypes.byref(a),
+                                                    ctypes.byref(b))
+        return a.value, b.value
+
+    def newton_node(self, u, guess=None, tol=1e-9, max_iter=20):
+        uu = np.ascontiguousarray(u, float).copy()
+        g = _d(np.ascontiguousarray(guess, float)) if guess is not None else None
+        it = self.L.ora_rainy_newton_node(_d(uu), g, ctypes.c_double(tol), max_iter)
+        return uu, it
+
+    def _flux2(self, fn, ul, ur, n):
+        f = np.zeros(9)
+        getattr(self.L, fn)(_d(np.ascontiguousarray(ul, float)), _d(np.ascontiguousarray(ur, float)),
+                            _d(np.ascontiguousarray(n, float)), _d(f))
+        return f
+
+    def flux_ec(self, ul, ur, n):
+        return self._flux2("ora_rainy_flux_ec", ul, ur, n)
+
+    def flux_llf(self, ul, ur, n):
+        return self._flux2("ora_rainy_flux_llf", ul, ur, n)
+
+    def flux(self, u, n):
+        f = np.zeros(9)
+        self.L.ora_rainy_flux(_d(np.ascontiguousarray(u, float)), _d(np.ascontiguousarray(n, float)),
+                              _d(f))
+        return f
+
+    def source(self, u):
+        s = np.zeros(9)
+        self.L.ora_rainy_source(_d(np.ascontiguousarray(u, float)), _d(s))
+        return s

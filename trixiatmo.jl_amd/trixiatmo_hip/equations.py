@@ -86,3 +86,43 @@ class SplitCovariantShallowWaterEquations2D:
     def __post_init__(self):
         if self.global_coordinate_system != "GlobalCartesianCoordinates()":
             raise ValueError("the covariant path uses GlobalCartesianCoordinates()")
+
+
+@dataclass(frozen=True)
+class CompressibleRainyEulerEquations2D:
+    """src/equations/compressible_rainy_euler_2d.jl:47-113: variables (rho_dry,
+    rho_moist, rho_rain, rho_v1, rho_v2, energy_density, rho_vapour, rho_cloud,
+    temperature); the last three are the solution of the saturation system."""
+    c_liquid_water: float = 4186.0
+    c_dry_air_const_pressure: float = 1004.0
+    c_dry_air_const_volume: float = 717.0
+    c_vapour_const_pressure: float = 1885.0
+    c_vapour_const_volume: float = 1424.0
+    ref_saturation_pressure: float = 610.7
+    ref_temperature: float = 273.15
+    ref_latent_heat_vap_temp: float = 2500000.0
+    ref_pressure: float = 100000.0
+    gravity: float = 9.80616
+    rain_water_distr: float = 8000000.0
+    v_mean_rain: float = 130.0
+
+    nvariables = 9
+    varnames = ("rho_dry", "rho_moist", "rho_rain", "rho_v1", "rho_v2", "energy_density",
+                "rho_vapour", "rho_cloud", "temperature")
+
+    @property
+    def R_dry_air(self):
+        return self.c_dry_air_const_pressure - self.c_dry_air_const_volume
+
+    @property
+    def R_vapour(self):
+        return self.c_vapour_const_pressure - self.c_vapour_const_volume
+
+    def saturation_vapour_pressure(self, T):
+        """:353-373 (Clausius-Clapeyron)"""
+        import numpy as np
+        k1 = (self.c_vapour_const_pressure - self.c_liquid_water) / self.R_vapour
+        k2 = (self.ref_latent_heat_vap_temp -
+              (self.c_vapour_const_pressure - self.c_liquid_water) * self.ref_temperature) / self.R_vapour
+        return self.ref_saturation_pressure * (T / self.ref_temperature) ** k1 * \
+            np.exp(k2 * (1 / self.ref_temperature - 1 / T))

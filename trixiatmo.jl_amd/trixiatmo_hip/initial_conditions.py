@@ -358,3 +358,95 @@ def transform_initial_condition(initial_condition):
         return np.stack([h, h * vcon1, h * vcon2])
 
     return transformed
+
+
+# ---------------------------------------------------------------------------
+# Rainy Euler (CompressibleRainyEulerEquations2D)
+
+def initial_condition_convergence_test_rainy(x, t, equations):
+    """examples/euler/precipitation/tests/convergence_test.jl:8-40: a
+    saturated travelling wave with rho = 2 + 0.1 sin(pi (x + y - t)),
+    T = rho + 250, rho_vapour = e_s(T) / (R_v T), v = (1, 1); x [2, ...]."""
+    eq = equations
+    c = 2.0
+    A = 0.1
+    L = 2.0
+    f = 1 / L
+    w = 2 * np.pi * f
+    rho = c + A * np.sin(w * (x[0] + x[1] - t))
+    T = rho + 250.0
+    rv = eq.saturation_vapour_pressure(T) / (eq.R_vapour * T)
+    rc = rho / eq.c_liquid_water * 3000
+    rr = rho / eq.c_liquid_water * 1000
+    rm = rv + rc
+    rd = rho - rm - rr
+    E = (eq.c_dry_air_const_volume * rd + eq.c_vapour_const_volume * rv +
+         eq.c_liquid_water * (rc + rr)) * T
+    E = E + rv * eq.ref_latent_heat_vap_temp + rho
+    return np.stack([rd, rm, rr, rho, rho, E, rv, rc, T])
+
+
+def moist_hydrostatic_column(z, equations, theta=300.0, rel_humidity=0.5, p_surface=1.0e5,
+                             dz_max=5.0):
+    """pressure, temperature and vapour density at heights z of a hydrostatic
+    atmosphere with constant dry potential temperature theta and constant
+    relative humidity: dp/dz = -(rho_d + rho_v) g with
+    T = theta (p / p_ref)^(R_d / c_pd), p_v = RH e_s(T), integrated by RK4
+    with steps of at most dz_max from the surface."""
+    eq = equations
+    R_d, c_pd = eq.R_dry_air, eq.c_dry_air_const_pressure
+
+    def state(p):
+        T = theta * (p / eq.ref_pressure) ** (R_d / c_pd)
+        pv = rel_humidity * eq.saturation_vapour_pressure(T)
+        rv = pv / (eq.R_vapour * T)
+        rd = (p - pv) / (R_d * T)
+        return T, rv, rd
+
+    def rhs(p):
+        T, rv, rd = state(p)
+        return -(rd + rv) * eq.gravity
+
+    zs = np.unique(np.asarray(z, dtype=float).ravel())
+    ps = np.empty_like(zs)
+    p, zc = p_surface, 0.0
+    for k, zt in enumerate(zs):
+        nsub = max(1, int(np.ceil((zt - zc) / dz_max)))
+        h = (zt - zc) / nsub
+        for _ in range(nsub):
+            k1 = rhs(p)
+            k2 = rhs(p + 0.5 * h * k1)
+            k3 = rhs(p + 0.5 * h * k2)
+            k4 = rhs(p + h * k3)
+            p = p + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        zc = zt
+        ps[k] = p
+    pz = np.interp(np.asarray(z, dtype=float), zs, ps)
+    return pz
+
+
+def initial_condition_rainy_bubble(mesh, equations, seed=0, theta=300.0, radius=2000.0,
+                                   amplitude=2.0):
+    """Seeded moist warm bubble: a hydrostatic background with dry potential
+    temperature 300 K and a relative humidity drawn from U[0.2, 0.95], plus a
+    temperature bump amplitude cos^2(pi r / (2 R)) of R = 2 km at a random
+    centre, at the background pressure and relative humidity; at rest, no
+    cloud or rain, the saturation variables (rho_vapour, 0, T) consistent."""
+    eq = equations
+    rng = np.random.default_rng(seed)
+    rh = rng.uniform(0.2, 0.95)
+    xc = rng.uniform(mesh.x0 + radius, mesh.x1 - radius)
+    zc = rng.uniform(mesh.y0 + radius, mesh.y0 + 0.5 * (mesh.y1 - mesh.y0))
+    x, z = mesh.node_coordinates
+    p = moist_hydrostatic_column(z - mesh.y0, eq, theta, rh)
+    T = theta * (p / eq.ref_pressure) ** (eq.R_dry_air / eq.c_dry_air_const_pressure)
+    r = np.sqrt((x - xc) ** 2 + (z - zc) ** 2)
+    T = T + np.where(r < radius, amplitude * np.cos(0.5 * np.pi * r / radius) ** 2, 0.0)
+    pv = rh * eq.saturation_vapour_pressure(T)
+    rv = pv / (eq.R_vapour * T)
+    rd = (p - pv) / (eq.R_dry_air * T)
+    zero = np.zeros_like(rd)
+    E = (eq.c_dry_air_const_volume * rd + eq.c_vapour_const_volume * rv) * T + \
+        rv * eq.ref_latent_heat_vap_temp
+    u = np.stack([rd, rv, zero, zero, zero, E, rv, zero, T])
+    return u, {"relative_humidity": rh, "centre": (xc, zc)}

@@ -183,3 +183,39 @@ def lgl_basis(polydeg):
     check(lib().atmo_lgl_basis(n, x.ctypes.data_as(c_double_p), w.ctypes.data_as(c_double_p),
                                D.ctypes.data_as(c_double_p), Ds.ctypes.data_as(c_double_p)))
     return x, w, D.reshape(n, n), Ds.reshape(n, n)
+
+
+class StructuredMesh2D:
+    """`StructuredMesh(cells_per_dimension, coordinates_min, coordinates_max;
+    periodicity)` [ext] for a uniform Cartesian rectangle: elements numbered x
+    fastest, node (i, j) of element (ex, ey) at
+    x = x_min + dx (ex + (xi_i + 1) / 2), y likewise.  `periodicity[1]` false
+    means boundary_condition_simple_slip_wall at y_min and y_max."""
+
+    def __init__(self, cells_per_dimension, coordinates_min, coordinates_max, *, polydeg,
+                 periodicity=(True, True)):
+        self.nx, self.ny = (int(c) for c in cells_per_dimension)
+        self.x0, self.y0 = (float(c) for c in coordinates_min)
+        self.x1, self.y1 = (float(c) for c in coordinates_max)
+        if not periodicity[0]:
+            raise ValueError("the rainy path is periodic in x")
+        self.periodic_y = bool(periodicity[1])
+        self.polydeg = int(polydeg)
+        self.n_elements = self.nx * self.ny
+        n = self.polydeg + 1
+        xi, _, _, _ = lgl_basis(self.polydeg)
+        dx = (self.x1 - self.x0) / self.nx
+        dy = (self.y1 - self.y0) / self.ny
+        ex = np.arange(self.nx)
+        ey = np.arange(self.ny)
+        X = self.x0 + dx * (ex[None, None, :, None] + 0.5 * (xi[:, None, None, None] + 1.0))
+        Y = self.y0 + dy * (ey[None, None, None, :] + 0.5 * (xi[None, :, None, None] + 1.0))
+        X, Y = np.broadcast_arrays(X, Y)
+        # [2, i, j, e] with e = ex + nx * ey
+        self.node_coordinates = np.stack([X.reshape(n, n, -1, order="F"),
+                                          Y.reshape(n, n, -1, order="F")])
+        self.dx, self.dy = dx, dy
+
+    @property
+    def nnodes(self):
+        return self.polydeg + 1
