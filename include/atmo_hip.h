@@ -32,6 +32,7 @@ extern "C" {
 /* equation systems */
 #define ATMO_EQ_EULER_ENERGY_GRAVITY_3D 1 /* CompressibleEulerEnergyEquationsWithGravity3D */
 #define ATMO_EQ_COVARIANT_SWE_2D 2        /* SplitCovariantShallowWaterEquations2D */
+#define ATMO_EQ_RAINY_EULER_2D 3          /* CompressibleRainyEulerEquations2D */
 
 /* volume (symmetric two-point) fluxes */
 #define ATMO_VOL_RANOCHA 0        /* flux_ranocha */
@@ -215,6 +216,95 @@ int atmo_swe_create(const atmo_swe_config* config, const atmo_geometry2d* geomet
  * per-member time steps */
 int atmo_max_dt_members(atmo_handle* handle, double cfl, double* dt);
 int atmo_step_members(atmo_handle* handle, const double* dt, int32_t n_steps);
+
+/* ------------------------------------------------- rainy Euler in 2D */
+#define ATMO_RAINY_VOL_WEAK_FORM 0        /* VolumeIntegralWeakForm() */
+#define ATMO_RAINY_VOL_FLUX_EC_RAIN 1     /* VolumeIntegralFluxDifferencing(flux_ec_rain) */
+#define ATMO_RAINY_SURF_LAX_FRIEDRICHS 0  /* flux_lax_friedrichs (max_abs_speed_naive) */
+#define ATMO_RAINY_SURF_FLUX_EC_RAIN 1    /* flux_ec_rain */
+#define ATMO_RAINY_SRC_NONE 0
+#define ATMO_RAINY_SRC_RAINY 1            /* source_terms_rainy */
+#define ATMO_RAINY_SRC_CONVERGENCE_TEST 2 /* source_terms_convergence_test_rainy */
+#define ATMO_RAINY_SRC_NO_PHASE_CHANGE 3  /* source_terms_no_phase_change */
+
+typedef struct {
+  int32_t polydeg;                /* 1..7 */
+  int32_t cells_x, cells_y;       /* StructuredMesh cells_per_dimension */
+  double x_min, y_min, x_max, y_max;
+  int32_t periodic_y;             /* 0: boundary_condition_simple_slip_wall at y_min, y_max */
+  int32_t volume_integral;        /* ATMO_RAINY_VOL_* */
+  int32_t surface_flux;           /* ATMO_RAINY_SURF_* */
+  int32_t source_terms;           /* ATMO_RAINY_SRC_* */
+  double newton_tolerance;        /* NonlinearSolveDG(; tolerance) */
+  int32_t newton_max_iterations;  /* NonlinearSolveDG(; max_iterations); 0: no stage limiter */
+} atmo_rainy_config;
+
+/* SemidiscretizationHyperbolic(StructuredMesh(cells, min, max; periodicity),
+ * CompressibleRainyEulerEquations2D(), initial_condition, DGSEM(...);
+ * source_terms, boundary_conditions) on a uniform Cartesian mesh (periodic
+ * in x) with NonlinearSolveDG(saturation_residual,
+ * saturation_residual_jacobian, SVector(7, 8, 9)) as the stage limiter of
+ * atmo_step (src/callbacks_stage/nonlinear_solve_dg.jl:34-45).  State
+ * [9, n, n, E], elements numbered x fastest.  atmo_integrals writes the
+ * integral of each of the nine variables (out[0..8]). */
+int atmo_rainy_create(const atmo_rainy_config* config, atmo_handle** handle);
+
+/* time of the resident state (used by time-dependent sources); atmo_step
+ * advances it */
+int atmo_set_time(atmo_handle* handle, double t);
+
+typedef struct {
+  int64_t nodes;
+  int64_t updates;         /* Newton updates summed over nodes */
+  int64_t max_updates;     /* largest number of updates of one node */
+  int64_t failures;        /* nodes that missed the tolerance (last iterate kept) */
+  int64_t fallbacks;       /* seeded solves redone from the default guess */
+} atmo_newton_stats;
+
+/* the saturation solve of nonlinear_solve_dg_2d! (nonlinear_solve_dg2d.jl:6-45)
+ * on the resident state: guess u[7:9] (seeded = 0) or the attached MLP's
+ * (T, q_v) prediction (seeded = 1, falling back to u[7:9] where the seeded
+ * solve fails); iterations[node] (device pointer, may be NULL) receives the
+ * update count (-1: tolerance missed). */
+int atmo_saturation_newton(atmo_handle* handle, int32_t seeded, int32_t* iterations_device,
+                           atmo_newton_stats* stats);
+/* Newton statistics accumulated by atmo_step's stage limiter since the last
+ * call (enable with atmo_enable_newton_stats) */
+int atmo_enable_newton_stats(atmo_handle* handle, int32_t enable);
+int atmo_newton_stats_read(atmo_handle* handle, atmo_newton_stats* stats);
+
+/* features of the resident state for the saturation seed (device pointers):
+ * x[3 * node] = (rho, internal energy density, rho_moist + rho_rain),
+ * y[2 * node] = (T, rho_vapour / rho) */
+int atmo_rainy_features(atmo_handle* handle, float* x_device, float* y_device);
+
+/* -------------------------------- MLP seed of the saturation solve
+ * (no counterpart in the reference): 3 -> 32 -> 32 -> 2, tanh, FP32,
+ * parameters W1[32][3], b1[32], W2[32][32], b2[32], W3[2][32], b3[2] (1250),
+ * inputs and outputs normalised as (v - mean) / scale. */
+typedef struct atmo_mlp atmo_mlp;
+#define ATMO_MLP_PARAMS 1250
+int atmo_mlp_create(uint64_t seed, atmo_mlp** mlp);
+void atmo_mlp_destroy(atmo_mlp* mlp);
+/* norm = in_mean[3], in_scale[3], out_mean[2], out_scale[2] */
+int atmo_mlp_set_normalization(atmo_mlp* mlp, const float* norm);
+int atmo_mlp_set_params(atmo_mlp* mlp, const float* params);
+int atmo_mlp_get_params(atmo_mlp* mlp, float* params);
+/* loss = mean over samples of the squared error of the normalised outputs
+ * (summed over the two outputs) and its gradient (host arrays), for samples
+ * x[3 n], y[2 n] in device memory */
+int atmo_mlp_loss_grad(atmo_mlp* mlp, const float* x_device, const float* y_device, int64_t n,
+                       float* loss, float* grad);
+/* Adam (lr, beta1 0.9, beta2 0.999, eps 1e-8) over mini-batches of
+ * `batch` samples taken in order from shuffled index lists held by the
+ * caller: one call runs `n_batches` steps on x[perm[b * batch + i]] */
+int atmo_mlp_train(atmo_mlp* mlp, const float* x_device, const float* y_device,
+                   const int32_t* perm_device, int64_t n, int32_t batch, int32_t n_batches,
+                   float lr, float* last_loss);
+/* y[2 n] = network(x[3 n]) in physical units (device pointers) */
+int atmo_mlp_forward(atmo_mlp* mlp, const float* x_device, float* y_device, int64_t n);
+/* the network that seeds atmo_saturation_newton (NULL detaches) */
+int atmo_rainy_attach_mlp(atmo_handle* handle, atmo_mlp* mlp);
 
 /* SemidiscretizationHyperbolic(mesh, equations, initial_condition, solver;
  * boundary_conditions) for P4estMesh{3}: copies geometry and connectivity to

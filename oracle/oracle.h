@@ -182,6 +182,11 @@ int ora_rainy_ck2n54(const ora_rainy_params* p, double* u, double t, double dt, 
 void ora_rainy_cons2entropy(const double* u, double* w);
 void ora_rainy_integrals(const ora_rainy_params* p, const double* u, const double* du, double* out);
 
+/* ---- the MLP seed of the saturation solve (3 -> 32 -> 32 -> 2, tanh):
+ *      FP64 loss and gradient for FP32 parameters and samples ---- */
+void ora_mlp_loss_grad(const float* params, const float* norm, const float* x, const float* y,
+                       long n, double* loss, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

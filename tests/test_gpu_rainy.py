@@ -1,0 +1,248 @@
+"""GPU parity of the rainy-Euler path: the HIP rhs!/stage kernel, the fused
+saturation Newton solve and the step-size reduction against the FP64 oracle,
+the reference's convergence-test regression values integrated on the GPU,
+and the MLP seed (gradient against the oracle, training, seeded solve).
+
+Tolerances: rhs <= 1e-11 relative per variable (against max |du| of that
+variable on a state with resolved tendencies); Newton solutions <= 1e-12
+relative with identical update counts; RK states after 3 stage-limited steps
+<= 1e-11; MLP loss and gradient (FP32 kernels) <= 1e-4 relative to the FP64
+oracle."""
+import numpy as np
+import pytest
+
+import trixiatmo_hip as T
+from trixiatmo_hip import initial_conditions as ic
+from trixiatmo_hip.meshes import StructuredMesh2D
+from trixiatmo_hip.solvers import DGSEM, VolumeIntegralFluxDifferencing, VolumeIntegralWeakForm
+from oracle_lib import RainyOracle
+from test_oracle_rainy import CONV_L2, CONV_LINF, EQ, calc_error_norms_2d, conv_mesh
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def solver(vol, surf="flux_lax_friedrichs", polydeg=3):
+    vi = VolumeIntegralWeakForm() if vol == 0 else VolumeIntegralFluxDifferencing("flux_ec_rain")
+    return DGSEM(polydeg=polydeg, surface_flux=surf, volume_integral=vi)
+
+
+def bubble_state(nx=6, ny=5, polydeg=3, periodic_y=False, seed=0, rain=True):
+    mesh = StructuredMesh2D((nx, ny), (0.0, 0.0), (3000.0, 2500.0), polydeg=polydeg,
+                            periodicity=(True, periodic_y))
+    u, _ = ic.initial_condition_rainy_bubble(mesh, EQ, seed=seed, radius=700.0)
+    rng = np.random.default_rng(seed + 10)
+    u = u.copy()
+    u[3] += 3.0 * rng.standard_normal(u[3].shape)
+    u[4] += 3.0 * rng.standard_normal(u[4].shape)
+    if rain:
+        # cloud and rain, with saturation variables slightly off the solution
+        u[1] += 2e-4 * rng.uniform(0, 1, u[1].shape)
+        u[2] += 1e-4 * rng.uniform(0, 1, u[2].shape)
+        u[7] += 1e-4 * rng.uniform(0, 1, u[7].shape)
+    return mesh, u
+
+
+def check_rhs(oracle, mesh, u, vol, surf, source, t=0.0):
+    semi = T.SemidiscretizationRainy(mesh, EQ, solver(vol, surf, mesh.polydeg), source_terms=source)
+    print(f"case: E={mesh.n_elements} n={mesh.nnodes} vol={vol} surf={surf} source={source}",
+          flush=True)
+    du_gpu = semi.rhs(None, u, t)
+    ro = RainyOracle(oracle, mesh, volume_integral=vol, source=source,
+                     surface_flux=T.rainy.RAINY_SURFACE_FLUXES[surf])
+    du_ref = ro.rhs(u, t)
+    a = du_gpu.reshape(9, -1, order="F")
+    b = du_ref.reshape(9, -1, order="F")
+    scale = np.maximum(np.abs(b).max(axis=1), 1e-300)
+    if source is not None:
+        # a source can cancel most of the flux terms (the manufactured solution
+        # of the convergence test does): measure on the larger of the two scales
+        flux_only = RainyOracle(oracle, mesh, volume_integral=vol, source=None,
+                                surface_flux=T.rainy.RAINY_SURFACE_FLUXES[surf]).rhs(u, t)
+        scale = np.maximum(scale, np.abs(flux_only.reshape(9, -1, order="F")).max(axis=1))
+    err = np.abs(a - b).max(axis=1) / scale
+    print("  rel err per variable:", err, flush=True)
+    assert np.all(err[:6] <= 1e-11), err
+    # rows 7..9 carry only the dissipation of the saturation variables
+    assert np.all(np.abs(a[6:] - b[6:]).max(axis=1) <= 1e-11 * np.maximum(scale[6:], 1e-30)), err
+    return semi, ro
+
+
+@pytest.mark.parametrize("vol,surf", [(1, "flux_lax_friedrichs"), (1, "flux_ec_rain"),
+                                      (0, "flux_lax_friedrichs")])
+@pytest.mark.parametrize("periodic_y", [False, True])
+def test_rhs_bubble(oracle, vol, surf, periodic_y):
+    mesh, u = bubble_state(periodic_y=periodic_y)
+    ro = RainyOracle(oracle, mesh)
+    u, _, _ = ro.newton(u)
+    check_rhs(oracle, mesh, u.reshape(9, 4, 4, -1, order="F"), vol, surf, "source_terms_rainy")
+
+
+@pytest.mark.parametrize("polydeg", [1, 2, 4, 5])
+def test_rhs_polydeg(oracle, polydeg):
+    mesh, u = bubble_state(nx=4, ny=3, polydeg=polydeg)
+    n = polydeg + 1
+    ro = RainyOracle(oracle, mesh)
+    u, _, _ = ro.newton(u)
+    check_rhs(oracle, mesh, u.reshape(9, n, n, -1, order="F"), 1, "flux_lax_friedrichs",
+              "source_terms_rainy")
+
+
+def test_rhs_convergence_test_source(oracle):
+    mesh = conv_mesh()
+    u = ic.initial_condition_convergence_test_rainy(mesh.node_coordinates, 0.37, EQ)
+    check_rhs(oracle, mesh, u, 0, "flux_lax_friedrichs", "source_terms_convergence_test_rainy",
+              t=0.37)
+
+
+def test_newton_kernel_matches_oracle(oracle):
+    mesh, u = bubble_state()
+    semi = T.SemidiscretizationRainy(mesh, EQ, solver(1), source_terms="source_terms_rainy",
+                                     stage_limiter=T.NonlinearSolveDG(1e-9, 20))
+    rng = np.random.default_rng(3)
+    u = u.copy()
+    u[8] += rng.uniform(-2, 2, u[8].shape)   # guesses away from the solution
+    u[2].flat[::17] = -1e-9                  # a few negative rain densities (clamped)
+    semi.set_state(u)
+    its = torch.zeros(semi.ndofs, dtype=torch.int32, device="cuda")
+    stats = semi.newton(seeded=False, iterations=its)
+    print("newton stats", stats, flush=True)
+    ro = RainyOracle(oracle, mesh)
+    u_ref, its_ref, fails = ro.newton(u, tol=1e-9, max_iter=20)
+    u_gpu = semi.get_state()
+    np.testing.assert_array_equal(its.cpu().numpy(), its_ref)
+    a, b = u_gpu.reshape(-1, 9), u_ref.reshape(-1, 9)
+    np.testing.assert_allclose(a[:, 6:], b[:, 6:], rtol=1e-12, atol=1e-18)
+    np.testing.assert_array_equal(a[:, :6], b[:, :6])
+    assert stats["failures"] == fails == 0
+    assert stats["updates"] == int(its_ref.sum())
+    assert stats["max_updates"] == int(its_ref.max())
+
+
+def test_steps_and_max_dt(oracle):
+    mesh, u = bubble_state(periodic_y=False)
+    ro = RainyOracle(oracle, mesh, volume_integral=1, source="source_terms_rainy")
+    u, _, _ = ro.newton(u)
+    semi = T.SemidiscretizationRainy(mesh, EQ, solver(1), source_terms="source_terms_rainy",
+                                     stage_limiter=T.NonlinearSolveDG(1e-9, 20))
+    semi.set_state(u)
+    dt = semi.max_dt(0.5)
+    dt_ref = 0.5 * ro.max_dt(u)
+    assert abs(dt - dt_ref) <= 1e-14 * dt_ref
+    semi.step(dt_ref, 3)
+    u_gpu = semi.get_state()
+    u_ref = ro.ck2n54(u, 0.0, dt_ref, 3, 1e-9, 20)
+    a, b = u_gpu.reshape(9, -1, order="F"), u_ref.reshape(9, -1, order="F")
+    rel = np.abs(a - b).max(axis=1) / np.abs(b).max(axis=1)
+    print("3-step state rel err", rel, flush=True)
+    assert np.all(rel <= 1e-11), rel
+    ig = semi.integrals()
+    ir = ro.integrals(u_ref)[:9]
+    np.testing.assert_allclose(ig, ir, rtol=1e-12, atol=1e-14 * np.abs(ir).max())
+
+
+def test_convergence_test_regression_gpu():
+    """the reference's convergence_test.jl regression values, integrated by
+    the stage kernels with the fused Newton stage limiter (tol 1e-8)"""
+    mesh = conv_mesh()
+    semi = T.SemidiscretizationRainy(mesh, EQ, solver(0),
+                                     source_terms="source_terms_convergence_test_rainy",
+                                     stage_limiter=T.NonlinearSolveDG(1e-8, 20))
+    exact = lambda x, t: ic.initial_condition_convergence_test_rainy(x, t, EQ)
+    u0 = exact(mesh.node_coordinates, 0.0)
+    semi.set_state(u0, 0.0)
+    t, steps = 0.0, 0
+    while t < 2.0:
+        dt = min(semi.max_dt(1.0), 2.0 - t)
+        semi.step(dt, 1)
+        t += dt
+        steps += 1
+    u = semi.get_state().reshape(u0.shape, order="F")
+    l2, linf = calc_error_norms_2d(u, mesh, exact, 2.0)
+    print("steps", steps, "l2", l2, "linf", linf, flush=True)
+    np.testing.assert_allclose(l2, CONV_L2, rtol=1e-7)
+    np.testing.assert_allclose(linf, CONV_LINF, rtol=1e-6)
+
+
+def _samples(n, seed=0):
+    rng = np.random.default_rng(seed)
+    x = np.stack([rng.uniform(0.8, 1.2, n), rng.uniform(2.0e5, 2.6e5, n),
+                  rng.uniform(1e-3, 2e-2, n)], axis=1).astype(np.float32)
+    y = np.stack([250 + 40 * np.tanh(x[:, 1] / 2.3e5 - 1) + 5 * x[:, 0],
+                  0.5 * x[:, 2] / x[:, 0]], axis=1).astype(np.float32)
+    return x, y
+
+
+def test_mlp_gradient_matches_oracle(oracle):
+    import ctypes
+    from oracle_lib import _d
+    mlp = T.SaturationMLP(seed=7)
+    norm_in = (np.array([1.0, 2.3e5, 1e-2]), np.array([0.12, 2e4, 6e-3]))
+    norm_out = (np.array([260.0, 5e-3]), np.array([20.0, 5e-3]))
+    mlp.set_normalization(*norm_in, *norm_out)
+    x, y = _samples(5000)
+    xd, yd = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    loss, g = mlp.loss_grad(xd, yd)
+    p = mlp.params()
+    lr = ctypes.c_double()
+    gr = np.zeros(p.size)
+    oracle.L.ora_mlp_loss_grad(p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                               mlp.norm.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                               x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                               y.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                               ctypes.c_long(x.shape[0]), ctypes.byref(lr), _d(gr))
+    print("loss", loss, lr.value, "max grad err", np.abs(g - gr).max() / np.abs(gr).max(),
+          flush=True)
+    np.testing.assert_allclose(loss, lr.value, rtol=1e-4)
+    assert np.abs(g - gr).max() <= 1e-4 * np.abs(gr).max()
+    # training lowers the loss; the forward pass agrees with the loss's view
+    perm = torch.randperm(x.shape[0], device="cuda", dtype=torch.int64).to(torch.int32)
+    for _ in range(30):
+        mlp.train(xd, yd, perm, 500, 10, 3e-3)
+    loss2, _ = mlp.loss_grad(xd, yd)
+    print("loss after training", loss2, flush=True)
+    assert loss2 < 0.2 * loss
+    out = torch.empty_like(yd)
+    mlp.forward(xd, out)
+    o = out.cpu().numpy()
+    mse = (((o - y) / np.array(norm_out[1], dtype=np.float32)) ** 2).sum(axis=1).mean()
+    np.testing.assert_allclose(mse, loss2, rtol=1e-3)
+
+
+def test_seeded_newton_converges_to_the_same_states():
+    """the MLP-seeded solve reaches the states of the unseeded solve to the
+    solver tolerance (falling back to u[7:9] where the seed fails)"""
+    mesh, u = bubble_state(nx=8, ny=8, rain=False)
+    semi = T.SemidiscretizationRainy(mesh, EQ, solver(1), source_terms="source_terms_rainy",
+                                     stage_limiter=T.NonlinearSolveDG(1e-9, 20))
+    semi.set_state(u)
+    n = semi.ndofs
+    x = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+    y = torch.empty((n, 2), dtype=torch.float32, device="cuda")
+    semi.newton(seeded=False)
+    semi.features(x, y)
+    mlp = T.SaturationMLP(seed=1)
+    xm, xs = x.mean(0).cpu().numpy(), x.std(0).cpu().numpy() + 1e-12
+    ym, ys = y.mean(0).cpu().numpy(), y.std(0).cpu().numpy() + 1e-12
+    mlp.set_normalization(xm, xs, ym, ys)
+    perm = torch.randperm(n, device="cuda").to(torch.int32)
+    for _ in range(20):
+        mlp.train(x, y, perm, 256, n // 256, 3e-3)
+    semi.attach_mlp(mlp)
+    ref = semi.get_state()
+    cold = ref.reshape(-1, 9).copy()
+    cold[:, 6:9] = np.array([0.0, 0.0, 200.0])   # a poor default guess
+    semi.set_state(cold.reshape(-1))
+    st_default = semi.newton(seeded=False)
+    u_default = semi.get_state()
+    semi.set_state(cold.reshape(-1))
+    st_seeded = semi.newton(seeded=True)
+    u_seeded = semi.get_state()
+    print("default", st_default, "seeded", st_seeded, flush=True)
+    assert st_default["failures"] == 0 and st_seeded["failures"] == 0
+    a, b = u_seeded.reshape(-1, 9), u_default.reshape(-1, 9)
+    np.testing.assert_allclose(a[:, 8], b[:, 8], rtol=1e-11)
+    np.testing.assert_allclose(a[:, 6], b[:, 6], rtol=1e-9)
+    np.testing.assert_allclose(a[:, 8], ref.reshape(-1, 9)[:, 8], rtol=1e-11)
+    assert st_seeded["updates"] <= st_default["updates"]

@@ -253,3 +253,34 @@ def test_lake_at_rest_without_gravity(oracle):
         # rain falls: only the rain terms of the vertical flux survive, and
         # their divergence vanishes in the interior of a uniform column
         assert np.abs(du[:, :, :, 4:8][[0, 1, 3]]).max() < 1e-10
+
+
+def test_mlp_oracle_gradient_finite_differences(oracle):
+    """the FP64 loss gradient of the MLP oracle against central differences of
+    its own loss (parameters perturbed by float32-representable steps)"""
+    import ctypes
+    from oracle_lib import _d
+    rng = np.random.default_rng(0)
+    p = (0.3 * rng.standard_normal(1250)).astype(np.float32)
+    norm = np.array([1.0, 2.3e5, 1e-2, 0.1, 2e4, 5e-3, 260.0, 5e-3, 20.0, 5e-3], dtype=np.float32)
+    x = np.stack([rng.uniform(0.8, 1.2, 64), rng.uniform(2e5, 2.6e5, 64),
+                  rng.uniform(1e-3, 2e-2, 64)], axis=1).astype(np.float32)
+    y = np.stack([rng.uniform(240, 290, 64), rng.uniform(1e-3, 1e-2, 64)], axis=1).astype(np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+
+    def loss_grad(pp):
+        L = ctypes.c_double()
+        g = np.zeros(1250)
+        oracle.L.ora_mlp_loss_grad(pp.ctypes.data_as(fp), norm.ctypes.data_as(fp),
+                                   x.ctypes.data_as(fp), y.ctypes.data_as(fp), ctypes.c_long(64),
+                                   ctypes.byref(L), _d(g))
+        return L.value, g
+
+    _, g = loss_grad(p)
+    for idx in list(range(0, 96, 13)) + [96, 130, 500, 1100, 1160, 1190, 1224, 1248, 1249]:
+        h = np.float32(1e-2) * max(abs(p[idx]), np.float32(0.05))
+        pp, pm = p.copy(), p.copy()
+        pp[idx] += h
+        pm[idx] -= h
+        fd = (loss_grad(pp)[0] - loss_grad(pm)[0]) / (float(pp[idx]) - float(pm[idx]))
+        assert abs(fd - g[idx]) <= 2e-3 * abs(g).max(), (idx, fd, g[idx])

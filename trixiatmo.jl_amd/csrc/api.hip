@@ -422,6 +422,7 @@ void atmo_destroy(atmo_handle* h) {
   if (h->d_member) hipFree(h->d_member);
   if (h->d_member_first) hipFree(h->d_member_first);
   if (h->d_face_info) hipFree(h->d_face_info);
+  rainy_free(h);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
 }
@@ -429,13 +430,14 @@ void atmo_destroy(atmo_handle* h) {
 int64_t atmo_state_length(const atmo_handle* h) { return h ? h->len : -1; }
 
 int atmo_rhs(atmo_handle* h, double* du, const double* u, double t) {
-  (void)t;
   if (!h || !du || !u) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
   const size_t sb = sizeof(double) * h->len;
   HIP_TRY(h, hipMemcpyAsync(h->d_uio, u, sb, hipMemcpyHostToDevice, h->stream));
   int rc = ATMO_OK;
   if (h->eq == ATMO_EQ_COVARIANT_SWE_2D) {
     rc = swe_rhs_device(h, h->d_duio, h->d_uio);
+  } else if (h->eq == ATMO_EQ_RAINY_EULER_2D) {
+    rc = rainy_rhs_device(h, h->d_duio, h->d_uio, t);
   } else {
     rc = compute_background(h, h->d_uio);
     if (rc != ATMO_OK) return rc;
@@ -448,9 +450,9 @@ int atmo_rhs(atmo_handle* h, double* du, const double* u, double t) {
 }
 
 int atmo_rhs_device(atmo_handle* h, double* du, const double* u, double t) {
-  (void)t;
   if (!h || !du || !u) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
   if (h->eq == ATMO_EQ_COVARIANT_SWE_2D) return swe_rhs_device(h, du, u);
+  if (h->eq == ATMO_EQ_RAINY_EULER_2D) return rainy_rhs_device(h, du, u, t);
   int rc = compute_background(h, u);
   if (rc != ATMO_OK) return rc;
   return run_rhs(h, du, u);
@@ -478,6 +480,7 @@ int atmo_step(atmo_handle* h, double dt, int32_t n_steps) {
   if (!h || n_steps < 0 || !(dt > 0) || !std::isfinite(dt))
     return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "bad step arguments");
   if (h->eq == ATMO_EQ_COVARIANT_SWE_2D) return swe_step(h, nullptr, dt, n_steps);
+  if (h->eq == ATMO_EQ_RAINY_EULER_2D) return rainy_step(h, dt, n_steps);
   // Carpenter & Kennedy (1994) five-stage fourth-order 2N coefficients
   static const double A[5] = {0.0, -567301805773.0 / 1357537059087.0,
                               -2404267990393.0 / 2016746695238.0,
@@ -508,6 +511,7 @@ int atmo_step(atmo_handle* h, double dt, int32_t n_steps) {
 
 int atmo_max_dt(atmo_handle* h, double cfl, double* dt) {
   if (!h || !dt) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
+  if (h->eq == ATMO_EQ_RAINY_EULER_2D) return rainy_max_dt(h, cfl, dt);
   if (h->eq == ATMO_EQ_COVARIANT_SWE_2D) {  // the smallest member time step
     std::vector<double> dtm(h->n_members);
     const int rc = swe_max_dt(h, cfl, dtm.data());
@@ -536,6 +540,7 @@ int atmo_max_dt(atmo_handle* h, double cfl, double* dt) {
 int atmo_integrals(atmo_handle* h, int32_t fresh_rhs, double* out) {
   if (!h || !out) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
   if (h->eq == ATMO_EQ_COVARIANT_SWE_2D) return swe_integrals(h, fresh_rhs, out);
+  if (h->eq == ATMO_EQ_RAINY_EULER_2D) return rainy_integrals(h, out);
   if (fresh_rhs) {
     int rc = run_rhs(h, h->d_du, h->d_u);
     if (rc != ATMO_OK) return rc;

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/atmo_hip.h"
+#include "rainy_kernels.hpp"
 #include "swe_kernels.hpp"
 
 namespace atmo {
@@ -51,6 +52,12 @@ struct atmo_handle {
   double *d_aux = nullptr, *d_xyz = nullptr, *d_dt_member = nullptr;
   int *d_member = nullptr, *d_member_first = nullptr;
   int n_members = 1;
+  // rainy Euler
+  atmo_rainy_config rainy{};
+  atmo::RainyLaunch rainy_base{};
+  double time = 0.0;                       // time of the resident state
+  unsigned long long* d_stats = nullptr;   // Newton statistics
+  atmo_mlp* mlp = nullptr;                 // seed of the saturation solve
   bool du_valid = false;
   bool timing = false;
   std::vector<atmo::TimerPair> timers;
@@ -75,6 +82,12 @@ int swe_rhs_device(atmo_handle* h, double* du, const double* u);
 int swe_step(atmo_handle* h, const double* dt_members, double dt, int n_steps);
 int swe_max_dt(atmo_handle* h, double cfl, double* dt_members);
 int swe_integrals(atmo_handle* h, int fresh_rhs, double* out);
+// rainy Euler paths of the generic entry points
+int rainy_rhs_device(atmo_handle* h, double* du, const double* u, double t);
+int rainy_step(atmo_handle* h, double dt, int n_steps);
+int rainy_max_dt(atmo_handle* h, double cfl, double* dt);
+int rainy_integrals(atmo_handle* h, double* out);
+void rainy_free(atmo_handle* h);
 }  // namespace atmo
 
 #endif

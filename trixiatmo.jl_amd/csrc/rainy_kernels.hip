@@ -1,0 +1,834 @@
+// gfx950 kernels of the compressible rainy Euler equations in 2D
+// (CompressibleRainyEulerEquations2D, src/equations/compressible_rainy_euler_2d.jl)
+// on a uniform Cartesian StructuredMesh, periodic in x, periodic or
+// simple-slip-wall in y: the fused rhs!/RK stage with the saturation Newton
+// solve of NonlinearSolveDG (src/callbacks_stage/nonlinear_solve_dg2d.jl)
+// applied to every updated stage state, the standalone Newton solve with an
+// optional MLP seed, the step-size reduction and the integrals.
+//
+// Layout: u [9, n, n, E] (rho_dry, rho_moist, rho_rain, rho_v1, rho_v2,
+// energy, rho_vapour, rho_cloud, T), elements numbered x fastest.  A 64-lane
+// workgroup holds EPB = 64 / n^2 elements (one lane per node); states and the
+// per-node quantities a two-point flux needs are staged in LDS.
+#include <hip/hip_runtime.h>
+
+#include "euler3d_physics.hpp"
+#include "rainy_kernels.hpp"
+
+namespace atmo {
+
+namespace {
+
+constexpr int NV = 9;
+
+// CompressibleRainyEulerEquations2D() constants (:77-113)
+constexpr double c_l = 4186, c_pd = 1004, c_vd = 717, c_pv = 1885, c_vv = 1424;
+constexpr double R_d = c_pd - c_vd, R_v = c_pv - c_vv;
+constexpr double ref_sp = 610.7, ref_T = 273.15, L_ref = 2500000;
+constexpr double grav = 9.80616, N_0 = 8000000, v_0 = 130;
+constexpr double kPi = 3.14159265358979323846;
+
+// cons2nonlinearsystemsol (:219-260)
+__device__ __forceinline__ void nonlinear_sol(const double* u, double& rv, double& rc, double& T) {
+  const double rho = u[0] + u[1] + u[2];
+  const double inv = 1.0 / rho;
+  if (u[1] == 0.0 && u[2] == 0.0) {
+    const double v1 = u[3] * inv, v2 = u[4] * inv;
+    const double ek = 0.5 * (v1 * v1 + v2 * v2) * rho;
+    rv = 0.0;
+    rc = 0.0;
+    T = (u[5] - ek) / (c_vd * rho);
+    return;
+  }
+  rv = u[6];
+  rc = u[7];
+  if (rv < 0.0 && rv >= -1e-15) rv = 0.0;
+  if (rc < 0.0 && rc >= -1e-15) rc = 0.0;
+  T = u[8];
+}
+
+// terminal_velocity_rain (:336-351)
+__device__ __forceinline__ double terminal_velocity(double rho_moist, double rho_rain) {
+  if (rho_rain > 0.0)
+    return v_0 * 1.9386213994279082 * pow(rho_rain / (kPi * (rho_moist + rho_rain) * N_0), 0.125);
+  return 0.0;
+}
+
+// saturation_vapour_pressure and its derivative (:353-398)
+__device__ __forceinline__ double svp(double T) {
+  constexpr double k1 = (c_pv - c_l) / R_v;
+  constexpr double k2 = (L_ref - (c_pv - c_l) * ref_T) / R_v;
+  return ref_sp * pow(T / ref_T, k1) * exp(k2 * (1 / ref_T - 1 / T));
+}
+
+__device__ __forceinline__ double svp_deriv(double T) {
+  constexpr double k1 = (c_pv - c_l) / R_v;
+  constexpr double k2 = (L_ref - (c_pv - c_l) * ref_T) / R_v;
+  double d = ref_sp / pow(ref_T, k1);
+  d *= k1 * pow(T, k1 - 1) + k2 * pow(T, k1 - 2);
+  d *= exp(k2 * (1 / ref_T - 1 / T));
+  return d;
+}
+
+// quantities of one node used by the two-point and surface fluxes
+struct RNode {
+  double rd, rm, rr, rho, inv, v1, v2, rv, rc, T, p, c, vr_m, vr_sol, E;
+};
+
+__device__ __forceinline__ RNode rnode(const double* u) {
+  RNode a;
+  a.rd = u[0];
+  a.rm = u[1];
+  a.rr = u[2];
+  a.rho = u[0] + u[1] + u[2];
+  a.inv = 1.0 / a.rho;
+  a.v1 = u[3] * a.inv;
+  a.v2 = u[4] * a.inv;
+  a.E = u[5];
+  nonlinear_sol(u, a.rv, a.rc, a.T);
+  a.p = (R_d * a.rd + R_v * a.rv) * a.T;
+  // speed_of_sound (:299-334)
+  const double q_v = a.rv / a.rd;
+  const double q_l = (a.rc + a.rr) / a.rd;
+  const double gm = 1 + (R_d + R_v * q_v) / (c_vd + c_vv * q_v + c_l * q_l);
+  a.c = sqrt(gm * a.p * a.inv);
+  a.vr_m = terminal_velocity(a.rm, a.rr);
+  a.vr_sol = terminal_velocity(a.rv + a.rc, a.rr);
+  return a;
+}
+
+// flux in normal direction (:502-545)
+__device__ __forceinline__ void flux_n(const RNode& a, double n1, double n2, double* f) {
+  const double vn = a.v1 * n1 + a.v2 * n2;
+  const double vrn = a.vr_m * n2;
+  f[0] = a.rd * vn;
+  f[1] = a.rm * vn;
+  f[2] = a.rr * (vn - vrn);
+  f[3] = a.rho * vn * a.v1 + a.p * n1 - a.rr * vrn * a.v1;
+  f[4] = a.rho * vn * a.v2 + a.p * n2 - a.rr * vrn * a.v2;
+  f[5] = (a.E + a.p) * vn - (c_l * a.T + 0.5 * (a.v1 * a.v1 + a.v2 * a.v2)) * a.rr * vrn;
+  f[6] = f[7] = f[8] = 0.0;
+}
+
+// flux in orientation form (:448-500): orientation 1 has no rain terms
+__device__ __forceinline__ void flux_o(const RNode& a, int orientation, double* f) {
+  if (orientation == 1) {
+    f[0] = a.rd * a.v1;
+    f[1] = a.rm * a.v1;
+    f[2] = a.rr * a.v1;
+    f[3] = a.rho * a.v1 * a.v1 + a.p;
+    f[4] = a.rho * a.v1 * a.v2;
+    f[5] = (a.E + a.p) * a.v1;
+  } else {
+    const double vr = a.vr_m;
+    f[0] = a.rd * a.v2;
+    f[1] = a.rm * a.v2;
+    f[2] = a.rr * (a.v2 - vr);
+    f[3] = a.rho * a.v1 * a.v2 - a.rr * vr * a.v1;
+    f[4] = a.rho * a.v2 * a.v2 + a.p - a.rr * vr * a.v2;
+    f[5] = (a.E + a.p) * a.v2 - (c_l * a.T + 0.5 * (a.v1 * a.v1 + a.v2 * a.v2)) * a.rr * vr;
+  }
+  f[6] = f[7] = f[8] = 0.0;
+}
+
+// flux_ec_rain (:872-960)
+__device__ __forceinline__ void flux_ec_rain(const RNode& l, const RNode& r, double n1, double n2,
+                                             double* f) {
+  const double itl = 1.0 / l.T, itr = 1.0 / r.T;
+  const double v1a = 0.5 * (l.v1 + r.v1), v2a = 0.5 * (l.v2 + r.v2);
+  const double v1sq = 0.5 * (l.v1 * l.v1 + r.v1 * r.v1), v2sq = 0.5 * (l.v2 * l.v2 + r.v2 * r.v2);
+  const double K = 0.5 * (v1sq + v2sq);
+  const double vdn = n1 * v1a + n2 * v2a;
+  const double vrn = 0.5 * n2 * (l.vr_sol + r.vr_sol);
+  const double dry_a = 0.5 * (l.rd + r.rd);
+  const double vap_a = 0.5 * (l.rv + r.rv);
+  const double cld_a = 0.5 * (l.rc + r.rc);
+  const double rain_a = 0.5 * (l.rr + r.rr);
+  double dry_log = 0.0, vap_log = 0.0;
+  if (l.rd != 0.0 && r.rd != 0.0) dry_log = ln_mean(l.rd, r.rd);
+  if (l.rv != 0.0 && r.rv != 0.0) vap_log = ln_mean(l.rv, r.rv);
+  const double it_avg = 0.5 * (itl + itr);
+  const double it_log = inv_ln_mean(itl, itr);
+  const double p_int = (1.0 / it_avg) * (R_d * dry_a + R_v * vap_a);
+  const double f_vap = vap_log * vdn;
+  const double f_cld = cld_a * vdn;
+  const double f_dry = dry_log * vdn;
+  const double f_rain = rain_a * (vdn - vrn);
+  const double f_moist = f_vap + f_cld;
+  const double f_rho = f_dry + f_moist + f_rain;
+  const double f_m1 = f_rho * v1a + p_int * n1;
+  const double f_m2 = f_rho * v2a + p_int * n2;
+  f[0] = f_dry;
+  f[1] = f_moist;
+  f[2] = f_rain;
+  f[3] = f_m1;
+  f[4] = f_m2;
+  f[5] = (c_vd * it_log - K) * f_dry + (c_vv * it_log - K + L_ref) * f_vap +
+         (c_l * it_log - K) * (f_cld + f_rain) + (v1a * f_m1 + v2a * f_m2);
+  f[6] = f[7] = f[8] = 0.0;
+}
+
+// flux_lax_friedrichs with max_abs_speed_naive (:607-624) on all nine entries
+__device__ __forceinline__ void flux_llf(const RNode& l, const RNode& r, const double* ul,
+                                         const double* ur, double n1, double n2, double* f) {
+  double fl[NV], fr[NV];
+  flux_n(l, n1, n2, fl);
+  flux_n(r, n1, n2, fr);
+  double vl = fabs(l.v1 * n1 + l.v2 * n2);
+  vl += fabs(l.vr_m * n2);
+  double vr = fabs(r.v1 * n1 + r.v2 * n2);
+  vr += fabs(r.vr_m * n2);
+  const double lam = fmax(vl, vr) + fmax(l.c, r.c) * sqrt(n1 * n1 + n2 * n2);
+#pragma unroll
+  for (int v = 0; v < NV; ++v) f[v] = 0.5 * (fl[v] + fr[v]) - 0.5 * lam * (ur[v] - ul[v]);
+}
+
+__device__ __forceinline__ void surface_flux(const RainyLaunch& L, const double* ul,
+                                             const double* ur, double n1, double n2, double* f) {
+  const RNode l = rnode(ul), r = rnode(ur);
+  if (L.surface_flux == 1) flux_ec_rain(l, r, n1, n2, f);
+  else flux_llf(l, r, ul, ur, n1, n2, f);
+}
+
+// source_terms_rainy (:547-577)
+__device__ __forceinline__ void source_rainy(const double* u, double* s) {
+  const double rho = u[0] + u[1] + u[2];
+  double rv, rc, T;
+  nonlinear_sol(u, rv, rc, T);
+  const double rho_vs = svp(T) / (R_v * T);
+  const double rr = u[2];
+  double S_ev = (3.86e-3 - 9.41e-5 * (T - ref_T)) * (1 + 9.1 * pow(rr, 0.1875));
+  S_ev *= (rho_vs - rv) * pow(rr, 0.5);
+  const double S = 0.001 * rc + 1.72 * rc * pow(rr, 0.875) - S_ev;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) s[v] = 0.0;
+  s[1] = -S;
+  s[2] = S;
+  s[4] = -rho * grav;
+  s[5] = -u[4] * grav;
+}
+
+// examples/euler/precipitation/tests/convergence_test.jl:45-139
+__device__ void source_convergence(double x, double y, double t, double* s) {
+  const double c = 2, A = 0.1, Lw = 2, fr = 1 / Lw, w = 2 * kPi * fr;
+  const double c_help = v_0 * 1.9386213994279082 * pow(kPi * N_0, -0.125);
+  const double si = sin(w * (x + y - t)), co = cos(w * (x + y - t));
+  const double rho = c + A * si;
+  const double rho_x = w * A * co;
+  const double rho_t = -rho_x;
+  const double T = rho + 250.0;
+  const double sv = svp(T), dsv = svp_deriv(T);
+  const double rv = sv / (R_v * T);
+  const double rc = rho / c_l * 3000;
+  const double rr = rho / c_l * 1000;
+  const double rm = rv + rc;
+  const double rd = rho - rm - rr;
+  const double vr = terminal_velocity(rm, rr);
+  const double svp_t = rho_t * dsv, svp_x = rho_x * dsv;
+  const double rv_t = (svp_t * T - rho_t * sv) / (R_v * T * T);
+  const double rv_x = (svp_x * T - rho_x * sv) / (R_v * T * T);
+  const double rc_t = rho_t / c_l * 3000, rc_x = rho_x / c_l * 3000;
+  const double rr_t = rho_t / c_l * 1000, rr_x = rho_x / c_l * 1000;
+  const double rm_t = rv_t + rc_t, rm_x = rv_x + rc_x;
+  const double rd_t = rho_t - rm_t - rr_t, rd_x = rho_x - rm_x - rr_x;
+  double E_t = (c_vd * rd_t + c_vv * rv_t + c_l * (rc_t + rr_t)) * T;
+  E_t += (c_vd * rd + c_vv * rv + c_l * (rc + rr)) * rho_t;
+  E_t += rv_t * L_ref + rho_t;
+  double E_x = (c_vd * rd_x + c_vv * rv_x + c_l * (rc_x + rr_x)) * T;
+  E_x += (c_vd * rd + c_vv * rv + c_l * (rc + rr)) * rho_x;
+  E_x += rv_x * L_ref + rho_x;
+  double p_x = (rd_x * R_d + rv_x * R_v) * T;
+  p_x += (rd * R_d + rv * R_v) * rho_x;
+  double vr_x = c_help * 0.125 * ((rr_x * rm - rr * rm_x) / ((rm + rr) * (rm + rr)));
+  vr_x *= pow(rr / (rm + rr), -0.875);
+  const double rrvr_x = rr_x * vr + rr * vr_x;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) s[v] = 0.0;
+  s[0] = rd_t + 2 * rd_x;
+  s[1] = rm_t + 2 * rm_x;
+  s[2] = rr_t + 2 * rr_x - rrvr_x;
+  s[3] = rho_x + p_x - rrvr_x;
+  s[4] = rho_x + p_x - rrvr_x;
+  double sE = E_t + 2 * (E_x + p_x) - (c_l * rho_x * rr * vr);
+  sE -= (c_l * T + 1) * rrvr_x;
+  s[5] = sE;
+}
+
+// saturation_residual and its Jacobian (:738-809); J[row + 3 col]
+__device__ __forceinline__ void residual(const double* u, const double* x, double* r) {
+  const double rho = u[0] + u[1] + u[2];
+  const double inv = 1.0 / rho;
+  const double v1 = u[3] * inv, v2 = u[4] * inv;
+  double r1 = (c_vd * u[0] + c_vv * x[0] + c_l * (x[1] + u[2])) * x[2];
+  r1 += x[0] * L_ref;
+  r1 -= (u[5] - rho * 0.5 * (v1 * v1 + v2 * v2));
+  double r2 = fmin(svp(x[2]) / (R_v * x[2]), u[1]);
+  r2 -= x[0];
+  r2 *= 1.0e7;
+  double r3 = u[1];
+  r3 -= x[0] + x[1];
+  r3 *= 1.0e7;
+  r[0] = r1;
+  r[1] = r2;
+  r[2] = r3;
+}
+
+__device__ __forceinline__ void jacobian(const double* u, const double* x, double* J) {
+  const double sv = svp(x[2]), svt = svp_deriv(x[2]);
+  J[0] = c_vv * x[2] + L_ref;
+  J[3] = c_l * x[2];
+  J[6] = c_vd * u[0] + c_vv * x[0] + c_l * (x[1] + u[2]);
+  J[1] = -10000000.0;
+  J[4] = 0.0;
+  J[7] = (sv / (R_v * x[2]) < u[1]) ? (svt * x[2] - sv) / (R_v * x[2] * x[2]) * 1.0e7 : 0.0;
+  J[2] = -10000000.0;
+  J[5] = -10000000.0;
+  J[8] = 0.0;
+}
+
+// 3x3 Gaussian elimination with partial pivoting
+__device__ __forceinline__ void solve3(const double* J, const double* r, double* dx) {
+  double a[3][4];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) a[i][j] = J[i + 3 * j];
+    a[i][3] = r[i];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    int p = c;
+#pragma unroll
+    for (int i = c + 1; i < 3; ++i)
+      if (fabs(a[i][c]) > fabs(a[p][c])) p = i;
+    if (p != c) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double tmp = a[c][j];
+        a[c][j] = a[p][j];
+        a[p][j] = tmp;
+      }
+    }
+#pragma unroll
+    for (int i = c + 1; i < 3; ++i) {
+      const double f = a[i][c] / a[c][c];
+#pragma unroll
+      for (int j = c; j < 4; ++j) a[i][j] -= f * a[c][j];
+    }
+  }
+#pragma unroll
+  for (int i = 2; i >= 0; --i) {
+    double s = a[i][3];
+#pragma unroll
+    for (int j = i + 1; j < 3; ++j) s -= a[i][j] * dx[j];
+    dx[i] = s / a[i][i];
+  }
+}
+
+// one node of nonlinear_solve_dg_2d!: solve from guess x (updated in place),
+// the residual reading `un` as it was before the rain clamp; returns the
+// number of updates or -1 when max_iter updates did not reach the tolerance
+__device__ int newton_node(const double* un, double* x, double tol, int max_iter) {
+  for (int it = 1; it <= max_iter; ++it) {
+    double r[3];
+    residual(un, x, r);
+    if (fmax(fabs(r[0]), fmax(fabs(r[1]), fabs(r[2]))) < tol) return it - 1;
+    double J[9], dx[3];
+    jacobian(un, x, J);
+    solve3(J, r, dx);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) x[k] += -dx[k];
+  }
+  return -1;
+}
+
+template <int N>
+struct RShape {
+  static constexpr int N2 = N * N;
+  static constexpr int EPB = N2 >= 64 ? 1 : 64 / N2;
+  static constexpr int BLOCK = ((EPB * N2 + 63) / 64) * 64;
+  static constexpr int NF = 4 * N;
+};
+
+__device__ __forceinline__ void accumulate_stats(unsigned long long* stats, int it) {
+  // wave-level sums keep the number of atomics small
+  unsigned long long upd = it > 0 ? (unsigned long long)it : 0ull;
+  unsigned long long fail = it < 0 ? 1ull : 0ull;
+  unsigned long long mx = upd;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    upd += __shfl_xor(upd, o, 64);
+    fail += __shfl_xor(fail, o, 64);
+    const unsigned long long other = __shfl_xor(mx, o, 64);
+    mx = other > mx ? other : mx;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&stats[0], upd);
+    atomicAdd(&stats[1], fail);
+    atomicMax(&stats[2], mx);
+  }
+}
+
+}  // namespace
+
+template <int N>
+__global__ __launch_bounds__(RShape<N>::BLOCK) void rainy_stage(RainyLaunch L) {
+  using S = RShape<N>;
+  constexpr int N2 = S::N2, EPB = S::EPB, B = S::BLOCK;
+  __shared__ double s_u[EPB][NV][N2];
+  __shared__ double s_f[EPB][2][6][N2];  // weak form: contravariant fluxes of the 6 flux rows
+  __shared__ double s_sf[EPB][NV][S::NF];
+
+  const int t = threadIdx.x;
+  const int e0 = blockIdx.x * EPB;
+  for (int idx = t; idx < EPB * N2 * NV; idx += B) {
+    const int es = idx / (N2 * NV), r = idx - es * N2 * NV;
+    if (e0 + es < L.n_elements) s_u[es][r % NV][r / NV] = L.u[(size_t)(e0 + es) * N2 * NV + r];
+  }
+  __syncthreads();
+
+  const int es = t / N2, q = t - es * N2;
+  const int e = e0 + es;
+  const bool active = es < EPB && e < L.n_elements;
+  const int ti = q % N, tj = q / N;
+  double du[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) du[v] = 0.0;
+
+  if (L.volume_integral == 0) {
+    // weak form: node fluxes into LDS, then du_i = sum_m dhat[i][m] F_m per line
+    if (active) {
+      double uo[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) uo[v] = s_u[es][v][q];
+      const RNode a = rnode(uo);
+      double f1[NV], f2[NV];
+      flux_o(a, 1, f1);
+      flux_o(a, 2, f2);
+#pragma unroll
+      for (int v = 0; v < 6; ++v) {
+        s_f[es][0][v][q] = L.Ja11 * f1[v] + 0.0 * f2[v];
+        s_f[es][1][v][q] = 0.0 * f1[v] + L.Ja22 * f2[v];
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < N; ++m) {
+        const double d1 = L.dhat[ti * N + m], d2 = L.dhat[tj * N + m];
+#pragma unroll
+        for (int v = 0; v < 6; ++v)
+          du[v] += d1 * s_f[es][0][v][m + N * tj] + d2 * s_f[es][1][v][ti + N * m];
+      }
+    }
+  } else if (active) {
+    // flux differencing with flux_ec_rain, gathered per node; each pair is
+    // evaluated with the lower node index on the left as the scatter form does
+    double uo[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) uo[v] = s_u[es][v][q];
+    const RNode a = rnode(uo);
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+      const int own = o == 0 ? ti : tj;
+      const double n1 = o == 0 ? L.Ja11 : 0.0, n2 = o == 0 ? 0.0 : L.Ja22;
+#pragma unroll
+      for (int m = 0; m < N; ++m) {
+        if (m == own) continue;
+        const int pq = o == 0 ? m + N * tj : ti + N * m;
+        double up[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) up[v] = s_u[es][v][pq];
+        const RNode b = rnode(up);
+        double f[NV];
+        if (own < m) flux_ec_rain(a, b, n1, n2, f);
+        else flux_ec_rain(b, a, n1, n2, f);
+        const double ds = L.dsplit[own * N + m];
+#pragma unroll
+        for (int v = 0; v < 6; ++v) du[v] += ds * f[v];
+      }
+    }
+  }
+
+  // surface fluxes, one lane per face node of the workgroup's elements
+  for (int fid = t; fid < EPB * S::NF; fid += B) {
+    const int fe = fid / S::NF, r = fid - fe * S::NF;
+    const int f = r / N, l = r - f * N;
+    const int ee = e0 + fe;
+    if (ee >= L.n_elements) continue;
+    const int ex = ee % L.nx, ey = ee / L.nx;
+    double ul[NV], ur[NV], fl[NV];
+    double n1 = 0.0, n2 = 0.0;
+    if (f < 2) {
+      n1 = L.Ja11;
+      const int en = f == 0 ? ((ex + L.nx - 1) % L.nx) + L.nx * ey : ((ex + 1) % L.nx) + L.nx * ey;
+      const int qo = f == 0 ? N * l : (N - 1) + N * l;          // own node
+      const int qn = f == 0 ? (N - 1) + N * l : N * l;          // neighbour node
+      const double* un = L.u + ((size_t)en * N2 + qn) * NV;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const double own = s_u[fe][v][qo];
+        ul[v] = f == 0 ? un[v] : own;
+        ur[v] = f == 0 ? own : un[v];
+      }
+    } else {
+      n2 = L.Ja22;
+      const bool bottom = f == 2;
+      const int qo = bottom ? l : l + N * (N - 1);
+      const bool wall = !L.periodic_y && (bottom ? ey == 0 : ey == L.ny - 1);
+      double nb[NV];
+      if (wall) {
+        // boundary_condition_simple_slip_wall (:963-983): mirrored state
+#pragma unroll
+        for (int v = 0; v < NV; ++v) nb[v] = s_u[fe][v][qo];
+        nb[4] = -nb[4];
+      } else {
+        const int en = bottom ? ex + L.nx * ((ey + L.ny - 1) % L.ny) : ex + L.nx * ((ey + 1) % L.ny);
+        const int qn = bottom ? l + N * (N - 1) : l;
+        const double* un = L.u + ((size_t)en * N2 + qn) * NV;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) nb[v] = un[v];
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const double own = s_u[fe][v][qo];
+        ul[v] = bottom ? nb[v] : own;
+        ur[v] = bottom ? own : nb[v];
+      }
+    }
+    surface_flux(L, ul, ur, n1, n2, fl);
+    const double sgn = (f % 2 == 0) ? -L.inv_w0 : L.inv_wn;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s_sf[fe][v][r] = sgn * fl[v];
+  }
+  __syncthreads();
+  int it_stat = 0;
+  if (active) {
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const bool on = f == 0 ? ti == 0 : f == 1 ? ti == N - 1 : f == 2 ? tj == 0 : tj == N - 1;
+      if (!on) continue;
+      const int l = f < 2 ? tj : ti;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) du[v] += s_sf[es][v][f * N + l];
+    }
+    double uo[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      uo[v] = s_u[es][v][q];
+      du[v] *= -L.inv_J;
+    }
+    if (L.source) {
+      double sv[NV];
+      if (L.source == 1) {
+        source_rainy(uo, sv);
+      } else if (L.source == 2) {
+        const int ex = e % L.nx, ey = e / L.nx;
+        const double x = L.x0 + L.dx * (ex + 0.5 * (L.xi[ti] + 1.0));
+        const double y = L.y0 + L.dy * (ey + 0.5 * (L.xi[tj] + 1.0));
+        source_convergence(x, y, L.t, sv);
+      } else {  // source_terms_no_phase_change (:579-593)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sv[v] = 0.0;
+        sv[4] = -grav * (uo[0] + uo[1] + uo[2]);
+        sv[5] = -grav * uo[4];
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) du[v] += sv[v];
+    }
+    const size_t g = ((size_t)e * N2 + q) * NV;
+    if (L.mode == 0) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) L.out[g + v] = du[v];
+    } else {
+      double un[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const double kn = L.rk_a * L.k[g + v] + L.rk_dt * du[v];
+        L.k[g + v] = kn;
+        un[v] = uo[v] + L.rk_b * kn;
+      }
+      // the stage limiter: NonlinearSolveDG on the updated state
+      if (L.newton_max_iter > 0) {
+        double x[3] = {un[6], un[7], un[8]};
+        it_stat = newton_node(un, x, L.newton_tol, L.newton_max_iter);
+        if (un[2] < 0.0) un[2] = 0.0;
+        un[6] = x[0];
+        un[7] = x[1];
+        un[8] = x[2];
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) L.out[g + v] = un[v];
+    }
+  }
+  // every lane of the (single-wave) workgroup takes part in the reduction
+  if (L.stats && L.mode == 1 && L.newton_max_iter > 0) accumulate_stats(L.stats, it_stat);
+}
+
+// per-element maximum of the scaled wave speeds (stepsize_dg2d.jl:5-43 pattern)
+template <int N>
+__global__ __launch_bounds__(64) void rainy_max_speed(RainyLaunch L, double* partial) {
+  constexpr int N2 = N * N;
+  const int e = blockIdx.x, t = threadIdx.x;
+  __shared__ double s1[64], s2[64];
+  double l1 = 0.0, l2 = 0.0;
+  for (int q = t; q < N2; q += 64) {
+    const RNode a = rnode(L.u + ((size_t)e * N2 + q) * NV);
+    const double lam1 = fabs(a.v1) + a.c;
+    const double lam2 = fabs(a.v2) + a.c + fabs(a.vr_m);
+    l1 = fmax(l1, fabs(L.Ja11 * lam1 + 0.0 * lam2) * L.inv_J);
+    l2 = fmax(l2, fabs(0.0 * lam1 + L.Ja22 * lam2) * L.inv_J);
+  }
+  s1[t] = l1;
+  s2[t] = l2;
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) {
+    if (t < w) {
+      s1[t] = fmax(s1[t], s1[t + w]);
+      s2[t] = fmax(s2[t], s2[t + w]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) partial[e] = s1[0] + s2[0];
+}
+
+__global__ __launch_bounds__(256) void rainy_max_reduce(const double* partial, int n, double* out) {
+  __shared__ double s[256];
+  const int t = threadIdx.x;
+  double acc = 5e-324;
+  for (int i = t; i < n; i += 256) acc = fmax(acc, partial[i]);
+  s[t] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) s[t] = fmax(s[t], s[t + w]);
+    __syncthreads();
+  }
+  if (t == 0) out[0] = s[0];
+}
+
+template <int N>
+__global__ __launch_bounds__(64) void rainy_integrals(RainyLaunch L, const double* w,
+                                                      double* partial) {
+  constexpr int N2 = N * N;
+  const int e = blockIdx.x, t = threadIdx.x;
+  __shared__ double s[NV][64];
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  const double J = 1.0 / L.inv_J;
+  for (int q = t; q < N2; q += 64) {
+    const double wJ = w[q % N] * w[q / N] * J;
+    const double* u = L.u + ((size_t)e * N2 + q) * NV;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] += wJ * u[v];
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) s[v][t] = acc[v];
+  __syncthreads();
+  for (int k = 32; k > 0; k >>= 1) {
+    if (t < k)
+      for (int v = 0; v < NV; ++v) s[v][t] += s[v][t + k];
+    __syncthreads();
+  }
+  if (t < NV) partial[(size_t)e * NV + t] = s[t][0];
+}
+
+__global__ __launch_bounds__(256) void rainy_sum_columns(const double* partial, int n_items,
+                                                         double* out) {
+  __shared__ double s[256];
+  const int t = threadIdx.x;
+  for (int c = 0; c < NV; ++c) {
+    double acc = 0.0;
+    for (int i = t; i < n_items; i += 256) acc += partial[(size_t)i * NV + c];
+    s[t] = acc;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+      if (t < k) s[t] += s[t + k];
+      __syncthreads();
+    }
+    if (t == 0) out[c] = s[0];
+    __syncthreads();
+  }
+}
+
+// MLP forward for one input (FP32, tanh hidden layers), normalised in/out
+__device__ __forceinline__ void mlp_eval(const float* P, const float* norm, const float* xin,
+                                         float* y) {
+  const float* W1 = P;
+  const float* b1 = W1 + kMlpHidden * kMlpIn;
+  const float* W2 = b1 + kMlpHidden;
+  const float* b2 = W2 + kMlpHidden * kMlpHidden;
+  const float* W3 = b2 + kMlpHidden;
+  const float* b3 = W3 + kMlpOut * kMlpHidden;
+  float x[kMlpIn];
+#pragma unroll
+  for (int i = 0; i < kMlpIn; ++i) x[i] = (xin[i] - norm[i]) / norm[3 + i];
+  float h1[kMlpHidden], h2[kMlpHidden];
+#pragma unroll
+  for (int j = 0; j < kMlpHidden; ++j) {
+    float a = b1[j];
+#pragma unroll
+    for (int i = 0; i < kMlpIn; ++i) a += W1[j * kMlpIn + i] * x[i];
+    h1[j] = tanhf(a);
+  }
+#pragma unroll 4
+  for (int j = 0; j < kMlpHidden; ++j) {
+    float a = b2[j];
+#pragma unroll
+    for (int i = 0; i < kMlpHidden; ++i) a += W2[j * kMlpHidden + i] * h1[i];
+    h2[j] = tanhf(a);
+  }
+#pragma unroll
+  for (int o = 0; o < kMlpOut; ++o) {
+    float a = b3[o];
+#pragma unroll
+    for (int i = 0; i < kMlpHidden; ++i) a += W3[o * kMlpHidden + i] * h2[i];
+    y[o] = a * norm[8 + o] + norm[6 + o];
+  }
+}
+
+// network inputs of one state: (rho, internal energy density, total water)
+__device__ __forceinline__ void mlp_inputs(const double* u, float* x) {
+  const double rho = u[0] + u[1] + u[2];
+  const double inv = 1.0 / rho;
+  const double v1 = u[3] * inv, v2 = u[4] * inv;
+  x[0] = (float)rho;
+  x[1] = (float)(u[5] - 0.5 * rho * (v1 * v1 + v2 * v2));
+  x[2] = (float)(u[1] + u[2]);
+}
+
+__global__ __launch_bounds__(256) void rainy_newton_kernel(RainyLaunch L, double* u,
+                                                           MlpView mlp, int use_mlp,
+                                                           int* iterations,
+                                                           unsigned long long* stats) {
+  __shared__ float sP[kMlpParams + 10];
+  if (use_mlp) {
+    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) sP[i] = mlp.params[i];
+    if (threadIdx.x < 10) sP[kMlpParams + threadIdx.x] = mlp.norm[threadIdx.x];
+    __syncthreads();
+  }
+  const size_t nodes = (size_t)L.n_elements * L.n * L.n;
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int it = 0;
+  if (q < nodes) {
+    double* un = u + q * NV;
+    double uc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) uc[v] = un[v];
+    double x[3] = {uc[6], uc[7], uc[8]};
+    int fallback = 0;
+    if (use_mlp) {
+      float xi[kMlpIn], y[kMlpOut];
+      mlp_inputs(uc, xi);
+      mlp_eval(sP, sP + kMlpParams, xi, y);
+      const double rho = uc[0] + uc[1] + uc[2];
+      const double T = (double)y[0];
+      double rv = (double)y[1] * rho;
+      rv = fmin(fmax(rv, 0.0), uc[1]);
+      x[0] = rv;
+      x[1] = uc[1] - rv;
+      x[2] = T;
+      it = newton_node(uc, x, L.newton_tol, L.newton_max_iter);
+      if (it < 0 || !(x[2] > 0.0)) {
+        // fall back to the default guess, as the unseeded solve
+        fallback = 1;
+        const int used = it < 0 ? L.newton_max_iter : it;
+        x[0] = uc[6];
+        x[1] = uc[7];
+        x[2] = uc[8];
+        const int it2 = newton_node(uc, x, L.newton_tol, L.newton_max_iter);
+        it = it2 < 0 ? -1 : used + it2;
+      }
+    } else {
+      it = newton_node(uc, x, L.newton_tol, L.newton_max_iter);
+    }
+    if (uc[2] < 0.0) un[2] = 0.0;
+    un[6] = x[0];
+    un[7] = x[1];
+    un[8] = x[2];
+    if (iterations) iterations[q] = it;
+    if (stats && fallback) atomicAdd(&stats[3], 1ull);
+  } else {
+    it = 0;
+  }
+  if (stats) accumulate_stats(stats, q < nodes ? it : 0);
+}
+
+// features for training: x = (rho, internal energy, total water), y = (T, q_v)
+__global__ void rainy_features_kernel(const double* u, size_t nodes, float* x, float* y) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nodes) return;
+  const double* un = u + q * NV;
+  float xi[kMlpIn];
+  mlp_inputs(un, xi);
+  x[3 * q] = xi[0];
+  x[3 * q + 1] = xi[1];
+  x[3 * q + 2] = xi[2];
+  const double rho = un[0] + un[1] + un[2];
+  y[2 * q] = (float)un[8];
+  y[2 * q + 1] = (float)(un[6] / rho);
+}
+
+#define ATMO_RAINY_SWITCH(MACRO) \
+  switch (L.n) {                 \
+    MACRO(2)                     \
+    MACRO(3)                     \
+    MACRO(4)                     \
+    MACRO(5)                     \
+    MACRO(6)                     \
+    MACRO(7)                     \
+    MACRO(8)                     \
+    default: return false;       \
+  }
+
+bool rainy_launch_stage(const RainyLaunch& L, hipStream_t s) {
+#define ATMO_CASE(NN)                                                                         \
+  case NN: {                                                                                  \
+    const int grid = (L.n_elements + RShape<NN>::EPB - 1) / RShape<NN>::EPB;                  \
+    hipLaunchKernelGGL(rainy_stage<NN>, dim3(grid), dim3(RShape<NN>::BLOCK), 0, s, L);        \
+    return true;                                                                              \
+  }
+  ATMO_RAINY_SWITCH(ATMO_CASE)
+#undef ATMO_CASE
+}
+
+bool rainy_launch_max_speed(const RainyLaunch& L, double* partial, double* out, hipStream_t s) {
+#define ATMO_CASE(NN)                                                                         \
+  case NN:                                                                                    \
+    hipLaunchKernelGGL(rainy_max_speed<NN>, dim3(L.n_elements), dim3(64), 0, s, L, partial); \
+    break;
+  ATMO_RAINY_SWITCH(ATMO_CASE)
+#undef ATMO_CASE
+  hipLaunchKernelGGL(rainy_max_reduce, dim3(1), dim3(256), 0, s, partial, L.n_elements, out);
+  return true;
+}
+
+bool rainy_launch_integrals(const RainyLaunch& L, const double* w, double* partial, double* out,
+                            hipStream_t s) {
+#define ATMO_CASE(NN)                                                                            \
+  case NN:                                                                                       \
+    hipLaunchKernelGGL(rainy_integrals<NN>, dim3(L.n_elements), dim3(64), 0, s, L, w, partial); \
+    break;
+  ATMO_RAINY_SWITCH(ATMO_CASE)
+#undef ATMO_CASE
+  hipLaunchKernelGGL(rainy_sum_columns, dim3(1), dim3(256), 0, s, partial, L.n_elements, out);
+  return true;
+}
+
+void rainy_launch_newton(const RainyLaunch& L, double* u, const MlpView* mlp, int* iterations,
+                         hipStream_t s) {
+  const size_t nodes = (size_t)L.n_elements * L.n * L.n;
+  const int grid = (int)((nodes + 255) / 256);
+  MlpView m = mlp ? *mlp : MlpView{nullptr, nullptr};
+  hipLaunchKernelGGL(rainy_newton_kernel, dim3(grid), dim3(256), 0, s, L, u, m, mlp ? 1 : 0,
+                     iterations, L.stats);
+}
+
+void rainy_launch_features(const RainyLaunch& L, const double* u, float* x, float* y,
+                           hipStream_t s) {
+  const size_t nodes = (size_t)L.n_elements * L.n * L.n;
+  hipLaunchKernelGGL(rainy_features_kernel, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0,
+                     s, u, nodes, x, y);
+}
+
+}  // namespace atmo

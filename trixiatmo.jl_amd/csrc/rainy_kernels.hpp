@@ -1,0 +1,70 @@
+// Launch interface of the rainy-Euler kernels (internal).
+#ifndef ATMO_RAINY_KERNELS_HPP
+#define ATMO_RAINY_KERNELS_HPP
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace atmo {
+
+// small MLP 3 -> 32 -> 32 -> 2 (tanh) that seeds the saturation solve; the
+// parameters are packed W1[32][3], b1[32], W2[32][32], b2[32], W3[2][32], b3[2]
+constexpr int kMlpIn = 3, kMlpHidden = 32, kMlpOut = 2;
+constexpr int kMlpParams = kMlpHidden * kMlpIn + kMlpHidden + kMlpHidden * kMlpHidden +
+                           kMlpHidden + kMlpOut * kMlpHidden + kMlpOut;  // 1250
+
+struct MlpView {
+  const float* params;  // kMlpParams
+  const float* norm;    // in_mean[3], in_scale[3], out_mean[2], out_scale[2]
+};
+
+struct RainyLaunch {
+  int n, nx, ny, n_elements;
+  int periodic_y, volume_integral, surface_flux, source;
+  double x0, y0, dx, dy;
+  double Ja11, Ja22, inv_J;   // uniform Cartesian cells
+  double xi[8];               // LGL nodes (n <= 8)
+  double inv_w0, inv_wn;      // 1 / w_1, 1 / w_n
+  double dsplit[64];          // Dsplit[i * n + m]
+  double dhat[64];            // Trixi derivative_dhat[i * n + m]
+  const double* u;
+  double* out;
+  double* k;
+  int mode;                   // 0 rhs, 1 RK stage (+ Newton on the new state)
+  double t;                   // time of the rhs evaluation (time-dependent sources)
+  double rk_a, rk_dt, rk_b;
+  double newton_tol;
+  int newton_max_iter;
+  unsigned long long* stats;  // optional: [sum of updates, failures, max updates]
+};
+
+bool rainy_launch_stage(const RainyLaunch& L, hipStream_t s);
+// per-element scaled wave speeds, then their maximum -> out[0]
+bool rainy_launch_max_speed(const RainyLaunch& L, double* partial, double* out, hipStream_t s);
+// integral of each of the 9 variables -> out[9]
+bool rainy_launch_integrals(const RainyLaunch& L, const double* w, double* partial, double* out,
+                            hipStream_t s);
+// NonlinearSolveDG on the state L.out in place; with `mlp` the guess comes from
+// the network and a node whose seeded solve fails is solved again from its
+// own u[7:9]; iterations[node] (optional) receives the update count (-1: failed)
+void rainy_launch_newton(const RainyLaunch& L, double* u, const MlpView* mlp, int* iterations,
+                         hipStream_t s);
+
+// MLP training: per-wave loss and gradient partials of samples
+// [offset, offset + count) (through `perm` when given), returning the number
+// of partials; their sum in a fixed order; Adam; forward in physical units
+int mlp_launch_loss_grad(const float* params, const float* norm, const float* x, const float* y,
+                         const int* perm, int64_t offset, int64_t count, float* partial,
+                         hipStream_t s);
+void mlp_launch_reduce(const float* partial, int blocks, float* grad_loss, hipStream_t s);
+void mlp_launch_adam(float* params, float* m, float* v, const float* grad, float lr, float beta1,
+                     float beta2, float eps, int step, hipStream_t s);
+void mlp_launch_forward(const float* params, const float* norm, const float* x, float* y,
+                        int64_t n, hipStream_t s);
+// features of a rainy state: x = (rho, internal energy, total water), y = (T, q_v)
+void rainy_launch_features(const RainyLaunch& L, const double* u, float* x, float* y,
+                           hipStream_t s);
+
+}  // namespace atmo
+
+#endif

@@ -14,6 +14,7 @@ from .meshes import P4estMesh, P4estMeshCubedSphere2D, P4estMeshCubedSphereTopog
 from .semidiscretization import (MetricTermsCovariantSphere, SemidiscretizationEnsemble,
                                  SemidiscretizationHyperbolic, solve)
 from .solvers import DGSEM, VolumeIntegralFluxDifferencing
+from .rainy import NonlinearSolveDG, SaturationMLP, SemidiscretizationRainy
 from . import analysis, workloads
 
 __all__ = [
@@ -23,5 +24,6 @@ __all__ = [
     "P4estMesh", "P4estMeshCubedSphereTopography", "lgl_basis", "SemidiscretizationHyperbolic",
     "DGSEM", "VolumeIntegralFluxDifferencing", "analysis", "workloads",
     "SplitCovariantShallowWaterEquations2D", "P4estMeshCubedSphere2D", "MetricTermsCovariantSphere",
-    "SemidiscretizationEnsemble", "solve",
+    "SemidiscretizationEnsemble", "solve", "NonlinearSolveDG", "SaturationMLP",
+    "SemidiscretizationRainy",
 ]

@@ -85,6 +85,32 @@ class AtmoGeometry2D(ctypes.Structure):
     ]
 
 
+class AtmoRainyConfig(ctypes.Structure):
+    _fields_ = [
+        ("polydeg", ctypes.c_int32),
+        ("cells_x", ctypes.c_int32),
+        ("cells_y", ctypes.c_int32),
+        ("x_min", ctypes.c_double),
+        ("y_min", ctypes.c_double),
+        ("x_max", ctypes.c_double),
+        ("y_max", ctypes.c_double),
+        ("periodic_y", ctypes.c_int32),
+        ("volume_integral", ctypes.c_int32),
+        ("surface_flux", ctypes.c_int32),
+        ("source_terms", ctypes.c_int32),
+        ("newton_tolerance", ctypes.c_double),
+        ("newton_max_iterations", ctypes.c_int32),
+    ]
+
+
+class AtmoNewtonStats(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_int64), ("updates", ctypes.c_int64),
+                ("max_updates", ctypes.c_int64), ("failures", ctypes.c_int64),
+                ("fallbacks", ctypes.c_int64)]
+
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+
 # (name, restype, argtypes) of every exported symbol of include/atmo_hip.h
 SIGNATURES = [
     ("atmo_lgl_basis", ctypes.c_int, [ctypes.c_int, c_double_p, c_double_p, c_double_p, c_double_p]),
@@ -107,6 +133,27 @@ SIGNATURES = [
      [ctypes.POINTER(AtmoSweConfig), ctypes.POINTER(AtmoGeometry2D), ctypes.POINTER(ctypes.c_void_p)]),
     ("atmo_max_dt_members", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, c_double_p]),
     ("atmo_step_members", ctypes.c_int, [ctypes.c_void_p, c_double_p, ctypes.c_int32]),
+    ("atmo_rainy_create", ctypes.c_int,
+     [ctypes.POINTER(AtmoRainyConfig), ctypes.POINTER(ctypes.c_void_p)]),
+    ("atmo_set_time", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    ("atmo_saturation_newton", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.POINTER(AtmoNewtonStats)]),
+    ("atmo_enable_newton_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("atmo_newton_stats_read", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(AtmoNewtonStats)]),
+    ("atmo_rainy_features", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("atmo_rainy_attach_mlp", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("atmo_mlp_create", ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    ("atmo_mlp_destroy", None, [ctypes.c_void_p]),
+    ("atmo_mlp_set_normalization", ctypes.c_int, [ctypes.c_void_p, c_float_p]),
+    ("atmo_mlp_set_params", ctypes.c_int, [ctypes.c_void_p, c_float_p]),
+    ("atmo_mlp_get_params", ctypes.c_int, [ctypes.c_void_p, c_float_p]),
+    ("atmo_mlp_loss_grad", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, c_float_p, c_float_p]),
+    ("atmo_mlp_train", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+      ctypes.c_int32, ctypes.c_int32, ctypes.c_float, c_float_p]),
+    ("atmo_mlp_forward", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
     ("atmo_create", ctypes.c_int,
      [ctypes.POINTER(AtmoConfig), ctypes.POINTER(AtmoGeometry3D), ctypes.POINTER(ctypes.c_void_p)]),
     ("atmo_destroy", None, [ctypes.c_void_p]),

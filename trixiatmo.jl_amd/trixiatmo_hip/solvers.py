@@ -14,6 +14,11 @@ class VolumeIntegralFluxDifferencing:
 
 
 @dataclass(frozen=True)
+class VolumeIntegralWeakForm:
+    """`VolumeIntegralWeakForm()`"""
+
+
+@dataclass(frozen=True)
 class DGSEM:
     """`DGSEM(polydeg = p, surface_flux = (flux, noncons), volume_integral = ...)`
 
@@ -21,8 +26,8 @@ class DGSEM:
     ("fp64", "fp32_kahan" or "fp32"); RK registers stay FP64 in every form.
     """
     polydeg: int
-    surface_flux: Tuple[Union[str, FluxLMARS], str]
-    volume_integral: VolumeIntegralFluxDifferencing
+    surface_flux: Union[Tuple[Union[str, FluxLMARS], str], str]
+    volume_integral: Union[VolumeIntegralFluxDifferencing, VolumeIntegralWeakForm]
     precision: str = "fp64"
     # mixed paths: (p0, T0) of an isothermal hydrostatic background whose
     # pressure and density are split off in FP64 (None: no split)
