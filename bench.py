@@ -126,6 +126,86 @@ def bench_covariant(steps, warmup, trees=32, polydeg=3, n_members=32):
     return out
 
 
+def bench_rainy(steps, warmup, train_seeds=4, mlp_epochs=20):
+    """rainy Euler 2D, 256 x 128 cells, polydeg 3 (524,288 nodes): stage-limited
+    CarpenterKennedy2N54 with the FP64 saturation Newton solve (tol 1e-9, the
+    reference's NonlinearSolveDG default) fused into every stage, unseeded
+    and seeded by the MLP trained on 4 x 524,288 = 2,097,152 states of this
+    configuration (4 further seeds, each after 2 steps)"""
+    import numpy as np
+    import torch
+    import trixiatmo_hip as T
+    from trixiatmo_hip.rainy import harvest_features, train_saturation_mlp
+    W = T.workloads
+    mesh, eq, solver, src, lim, u0, info = W.rainy_bubble(seed=0)
+    semi = T.SemidiscretizationRainy(mesh, eq, solver, source_terms=src, stage_limiter=lim)
+    nodes = semi.ndofs
+    out = {"workload": f"moist bubble {mesh.nx}x{mesh.ny} cells, polydeg {mesh.polydeg}, "
+                       "flux_ec_rain + LLF, source_terms_rainy, Newton tol 1e-9",
+           "nodes": nodes, "relative_humidity": info["relative_humidity"]}
+
+    def run(tag):
+        semi.set_state(u0)
+        dt = semi.max_dt(0.5)
+        semi.step(dt, warmup)
+        semi.synchronize()
+        t0 = time.perf_counter()
+        semi.step(dt, steps)
+        semi.synchronize()
+        el = time.perf_counter() - t0
+        semi.enable_timing(True)
+        semi.step(dt, 2)
+        launches, total_ms = semi.kernel_times(0)
+        semi.enable_timing(False)
+        semi.enable_newton_stats(True)
+        semi.step(dt, 2)
+        st = semi.newton_stats()
+        semi.enable_newton_stats(False)
+        finite = bool(np.all(np.isfinite(semi.get_state())))
+        log(f"[rainy {tag}] {steps} steps in {el:.4f} s, Newton {st['updates_per_node'] / 10:.3f} "
+            "updates per node and stage")
+        return {"value": nodes * 5 * steps / el / 1e9, "unit": "GDOF*RHS/s",
+                "ms_per_step": el / steps * 1e3, "avg_stage_launch_us": total_ms / launches * 1e3,
+                "newton_updates_per_node_per_stage": st["updates"] / (10.0 * nodes),
+                "newton_max_updates": st["max_updates"], "newton_failures": st["failures"],
+                "fallbacks": st["fallbacks"], "fallback_rate": st["fallbacks"] / (10.0 * nodes),
+                "finite": finite}, dt
+
+    out["unseeded"], dt = run("unseeded")
+    u_unseeded = semi.get_state()
+    states = [W.rainy_bubble(seed=s)[5] for s in range(1, train_seeds + 1)]
+    t0 = time.perf_counter()
+    x, y = harvest_features(semi, states, steps=2, dt=dt)
+    mlp, hist = train_saturation_mlp(x, y, epochs=mlp_epochs)
+    torch.cuda.synchronize()
+    out["mlp"] = {"samples": int(x.shape[0]), "epochs": mlp_epochs,
+                  "final_batch_loss": hist[-1], "first_epoch_loss": hist[0],
+                  "train_seconds": time.perf_counter() - t0}
+    semi.attach_mlp(mlp)
+    semi.seed_stage_limiter(True)
+    out["seeded"], _ = run("seeded")
+    u_seeded = semi.get_state()
+    semi.seed_stage_limiter(False)
+    a, b = u_seeded.reshape(-1, 9), u_unseeded.reshape(-1, 9)
+    out["seeded_vs_unseeded_state_rel_diff"] = float(np.abs(a - b).max() / np.abs(b).max())
+    out["seeded_vs_unseeded_T_rel_diff"] = float(np.abs(a[:, 8] - b[:, 8]).max() / np.abs(b[:, 8]).max())
+    # cold solve: the saturation variables reset to a fixed guess (T = 250 K,
+    # no vapour, no cloud), then the default and the seeded solve
+    cold = u_unseeded.reshape(-1, 9).copy()
+    cold[:, 6:9] = np.array([0.0, 0.0, 250.0])
+    semi.set_state(cold.reshape(-1))
+    st0 = semi.newton(seeded=False)
+    semi.set_state(cold.reshape(-1))
+    st1 = semi.newton(seeded=True)
+    out["cold_solve"] = {"guess": "T = 250 K, rho_vapour = rho_cloud = 0",
+                         "default_updates_per_node": st0["updates_per_node"],
+                         "seeded_updates_per_node": st1["updates_per_node"],
+                         "seeded_fallbacks": st1["fallbacks"],
+                         "failures": [st0["failures"], st1["failures"]]}
+    del semi
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +219,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-covariant", action="store_true",
                     help="skip the covariant shallow-water lines")
+    ap.add_argument("--no-rainy", action="store_true", help="skip the rainy-Euler line")
+    ap.add_argument("--rainy-steps", type=int, default=20)
     ap.add_argument("--traffic-kib", type=float, nargs=3, default=None,
                     metavar=("FETCH", "WRITE", "FACTOR"),
                     help="rocprofv3 FETCH_SIZE / WRITE_SIZE of the stage kernel and the FETCH scale")
@@ -230,6 +312,11 @@ def main():
         log("covariant shallow water")
         cov = bench_covariant(a.steps, a.warmup)
 
+    rainy = None
+    if not a.no_rainy:
+        log("rainy Euler")
+        rainy = bench_rainy(a.rainy_steps, min(a.warmup, 3))
+
     cpu = None
     if not a.no_cpu_baseline:
         log("cpu baseline")
@@ -274,6 +361,7 @@ def main():
         "fp64_drift": {"mass": f64["mass_drift"], "entropy": f64["entropy_drift"],
                        "steps": n_total},
         "covariant_swe": cov,
+        "rainy_euler": rainy,
         "state_finite": ok,
     }
     print(json.dumps(out))

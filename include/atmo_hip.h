@@ -305,6 +305,9 @@ int atmo_mlp_train(atmo_mlp* mlp, const float* x_device, const float* y_device,
 int atmo_mlp_forward(atmo_mlp* mlp, const float* x_device, float* y_device, int64_t n);
 /* the network that seeds atmo_saturation_newton (NULL detaches) */
 int atmo_rainy_attach_mlp(atmo_handle* handle, atmo_mlp* mlp);
+/* seed the stage limiter of atmo_step with the attached network (with the
+ * same fallback to u[7:9] per node) */
+int atmo_rainy_seed_stage_limiter(atmo_handle* handle, int32_t enable);
 
 /* SemidiscretizationHyperbolic(mesh, equations, initial_condition, solver;
  * boundary_conditions) for P4estMesh{3}: copies geometry and connectivity to

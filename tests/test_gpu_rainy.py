@@ -246,3 +246,35 @@ def test_seeded_newton_converges_to_the_same_states():
     np.testing.assert_allclose(a[:, 6], b[:, 6], rtol=1e-9)
     np.testing.assert_allclose(a[:, 8], ref.reshape(-1, 9)[:, 8], rtol=1e-11)
     assert st_seeded["updates"] <= st_default["updates"]
+
+
+def test_seeded_stage_limiter_matches_unseeded():
+    """atmo_step with the network seeding every stage's solve gives the states
+    of the unseeded stage limiter to the solver tolerance"""
+    from trixiatmo_hip.rainy import harvest_features, train_saturation_mlp
+    mesh, u = bubble_state(nx=8, ny=8, rain=True)
+    semi = T.SemidiscretizationRainy(mesh, EQ, solver(1), source_terms="source_terms_rainy",
+                                     stage_limiter=T.NonlinearSolveDG(1e-9, 20))
+    semi.set_state(u)
+    semi.newton(seeded=False)
+    u = semi.get_state()
+    dt = semi.max_dt(0.4)
+    x, y = harvest_features(semi, [u], steps=1, dt=dt)
+    mlp, hist = train_saturation_mlp(x, y, epochs=30, batch=256, lr=3e-3)
+    assert hist[-1] < hist[0]
+    semi.set_state(u)
+    semi.enable_newton_stats(True)
+    semi.step(dt, 3)
+    st0 = semi.newton_stats()
+    ua = semi.get_state()
+    semi.attach_mlp(mlp)
+    semi.seed_stage_limiter(True)
+    semi.set_state(u)
+    semi.step(dt, 3)
+    st1 = semi.newton_stats()
+    ub = semi.get_state()
+    print("unseeded", st0, "seeded", st1, flush=True)
+    assert st0["failures"] == 0 and st1["failures"] == 0
+    a, b = ua.reshape(9, -1, order="F"), ub.reshape(9, -1, order="F")
+    rel = np.abs(a - b).max(axis=1) / np.abs(a).max(axis=1)
+    assert np.all(rel[:6] <= 1e-10) and rel[8] <= 1e-10, rel

@@ -132,7 +132,7 @@ int rainy_max_dt(atmo_handle* h, double cfl, double* dt) {
 int rainy_integrals(atmo_handle* h, double* out) {
   RainyLaunch L = h->rainy_base;
   L.u = h->d_u;
-  // LGL weights are kept after the first 16 doubles of d_dt_member
+  // d_dt_member holds the LGL weights on rainy handles
   if (!rainy_launch_integrals(L, h->d_dt_member, h->d_partial, h->d_result, h->stream))
     return fail(&h->err, ATMO_ERR_INVALID, "no rainy kernel for this polydeg");
   HIP_TRY(h, hipGetLastError());
@@ -309,6 +309,16 @@ int atmo_rainy_attach_mlp(atmo_handle* h, atmo_mlp* m) {
   if (!h || h->eq != ATMO_EQ_RAINY_EULER_2D)
     return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "not a rainy handle");
   h->mlp = m;
+  if (!m) h->rainy_base.mlp = MlpView{nullptr, nullptr};
+  else if (h->rainy_base.mlp.params) h->rainy_base.mlp = MlpView{m->d_params, m->d_norm};
+  return ATMO_OK;
+}
+
+int atmo_rainy_seed_stage_limiter(atmo_handle* h, int32_t enable) {
+  if (!h || h->eq != ATMO_EQ_RAINY_EULER_2D)
+    return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "not a rainy handle");
+  if (enable && !h->mlp) return fail(&h->err, ATMO_ERR_INVALID, "no MLP attached");
+  h->rainy_base.mlp = enable ? MlpView{h->mlp->d_params, h->mlp->d_norm} : MlpView{nullptr, nullptr};
   return ATMO_OK;
 }
 

@@ -371,6 +371,67 @@ __device__ __forceinline__ void accumulate_stats(unsigned long long* stats, int 
 
 }  // namespace
 
+// MLP forward for one input (FP32, tanh hidden layers), normalised in/out
+__device__ __forceinline__ void mlp_eval(const float* P, const float* norm, const float* xin,
+                                         float* y) {
+  const float* W1 = P;
+  const float* b1 = W1 + kMlpHidden * kMlpIn;
+  const float* W2 = b1 + kMlpHidden;
+  const float* b2 = W2 + kMlpHidden * kMlpHidden;
+  const float* W3 = b2 + kMlpHidden;
+  const float* b3 = W3 + kMlpOut * kMlpHidden;
+  float x[kMlpIn];
+#pragma unroll
+  for (int i = 0; i < kMlpIn; ++i) x[i] = (xin[i] - norm[i]) / norm[3 + i];
+  float h1[kMlpHidden], h2[kMlpHidden];
+#pragma unroll
+  for (int j = 0; j < kMlpHidden; ++j) {
+    float a = b1[j];
+#pragma unroll
+    for (int i = 0; i < kMlpIn; ++i) a += W1[j * kMlpIn + i] * x[i];
+    h1[j] = tanhf(a);
+  }
+#pragma unroll 4
+  for (int j = 0; j < kMlpHidden; ++j) {
+    float a = b2[j];
+#pragma unroll
+    for (int i = 0; i < kMlpHidden; ++i) a += W2[j * kMlpHidden + i] * h1[i];
+    h2[j] = tanhf(a);
+  }
+#pragma unroll
+  for (int o = 0; o < kMlpOut; ++o) {
+    float a = b3[o];
+#pragma unroll
+    for (int i = 0; i < kMlpHidden; ++i) a += W3[o * kMlpHidden + i] * h2[i];
+    y[o] = a * norm[8 + o] + norm[6 + o];
+  }
+}
+
+// network inputs of one state: (rho, internal energy density, total water)
+__device__ __forceinline__ void mlp_inputs(const double* u, float* x) {
+  const double rho = u[0] + u[1] + u[2];
+  const double inv = 1.0 / rho;
+  const double v1 = u[3] * inv, v2 = u[4] * inv;
+  x[0] = (float)rho;
+  x[1] = (float)(u[5] - 0.5 * rho * (v1 * v1 + v2 * v2));
+  x[2] = (float)(u[1] + u[2]);
+}
+
+// seeded guess of one node from the network's (T, q_v); returns false when
+// the prediction is unusable
+__device__ __forceinline__ bool mlp_guess(const float* P, const double* u, double* x) {
+  float xi[kMlpIn], y[kMlpOut];
+  mlp_inputs(u, xi);
+  mlp_eval(P, P + kMlpParams, xi, y);
+  const double rho = u[0] + u[1] + u[2];
+  double rv = (double)y[1] * rho;
+  rv = fmin(fmax(rv, 0.0), u[1]);
+  x[0] = rv;
+  x[1] = u[1] - rv;
+  x[2] = (double)y[0];
+  return x[2] > 0.0;
+}
+
 template <int N>
 __global__ __launch_bounds__(RShape<N>::BLOCK) void rainy_stage(RainyLaunch L) {
   using S = RShape<N>;
@@ -378,12 +439,18 @@ __global__ __launch_bounds__(RShape<N>::BLOCK) void rainy_stage(RainyLaunch L) {
   __shared__ double s_u[EPB][NV][N2];
   __shared__ double s_f[EPB][2][6][N2];  // weak form: contravariant fluxes of the 6 flux rows
   __shared__ double s_sf[EPB][NV][S::NF];
+  extern __shared__ float s_mlp[];  // network parameters + normalisation when seeding
 
   const int t = threadIdx.x;
   const int e0 = blockIdx.x * EPB;
   for (int idx = t; idx < EPB * N2 * NV; idx += B) {
     const int es = idx / (N2 * NV), r = idx - es * N2 * NV;
     if (e0 + es < L.n_elements) s_u[es][r % NV][r / NV] = L.u[(size_t)(e0 + es) * N2 * NV + r];
+  }
+  const bool seeded = L.mlp.params != nullptr && L.mode == 1 && L.newton_max_iter > 0;
+  if (seeded) {
+    for (int i = t; i < kMlpParams; i += B) s_mlp[i] = L.mlp.params[i];
+    if (t < 10) s_mlp[kMlpParams + t] = L.mlp.norm[t];
   }
   __syncthreads();
 
@@ -551,7 +618,22 @@ __global__ __launch_bounds__(RShape<N>::BLOCK) void rainy_stage(RainyLaunch L) {
       // the stage limiter: NonlinearSolveDG on the updated state
       if (L.newton_max_iter > 0) {
         double x[3] = {un[6], un[7], un[8]};
-        it_stat = newton_node(un, x, L.newton_tol, L.newton_max_iter);
+        if (seeded) {
+          // network seed; a failed seeded solve is redone from u[7:9]
+          const bool ok = mlp_guess(s_mlp, un, x);
+          it_stat = ok ? newton_node(un, x, L.newton_tol, L.newton_max_iter) : -1;
+          if (it_stat < 0 || !(x[2] > 0.0)) {
+            const int used = ok ? L.newton_max_iter : 0;
+            x[0] = un[6];
+            x[1] = un[7];
+            x[2] = un[8];
+            const int it2 = newton_node(un, x, L.newton_tol, L.newton_max_iter);
+            it_stat = it2 < 0 ? -1 : used + it2;
+            if (L.stats) atomicAdd(&L.stats[3], 1ull);
+          }
+        } else {
+          it_stat = newton_node(un, x, L.newton_tol, L.newton_max_iter);
+        }
         if (un[2] < 0.0) un[2] = 0.0;
         un[6] = x[0];
         un[7] = x[1];
@@ -651,52 +733,6 @@ __global__ __launch_bounds__(256) void rainy_sum_columns(const double* partial, 
   }
 }
 
-// MLP forward for one input (FP32, tanh hidden layers), normalised in/out
-__device__ __forceinline__ void mlp_eval(const float* P, const float* norm, const float* xin,
-                                         float* y) {
-  const float* W1 = P;
-  const float* b1 = W1 + kMlpHidden * kMlpIn;
-  const float* W2 = b1 + kMlpHidden;
-  const float* b2 = W2 + kMlpHidden * kMlpHidden;
-  const float* W3 = b2 + kMlpHidden;
-  const float* b3 = W3 + kMlpOut * kMlpHidden;
-  float x[kMlpIn];
-#pragma unroll
-  for (int i = 0; i < kMlpIn; ++i) x[i] = (xin[i] - norm[i]) / norm[3 + i];
-  float h1[kMlpHidden], h2[kMlpHidden];
-#pragma unroll
-  for (int j = 0; j < kMlpHidden; ++j) {
-    float a = b1[j];
-#pragma unroll
-    for (int i = 0; i < kMlpIn; ++i) a += W1[j * kMlpIn + i] * x[i];
-    h1[j] = tanhf(a);
-  }
-#pragma unroll 4
-  for (int j = 0; j < kMlpHidden; ++j) {
-    float a = b2[j];
-#pragma unroll
-    for (int i = 0; i < kMlpHidden; ++i) a += W2[j * kMlpHidden + i] * h1[i];
-    h2[j] = tanhf(a);
-  }
-#pragma unroll
-  for (int o = 0; o < kMlpOut; ++o) {
-    float a = b3[o];
-#pragma unroll
-    for (int i = 0; i < kMlpHidden; ++i) a += W3[o * kMlpHidden + i] * h2[i];
-    y[o] = a * norm[8 + o] + norm[6 + o];
-  }
-}
-
-// network inputs of one state: (rho, internal energy density, total water)
-__device__ __forceinline__ void mlp_inputs(const double* u, float* x) {
-  const double rho = u[0] + u[1] + u[2];
-  const double inv = 1.0 / rho;
-  const double v1 = u[3] * inv, v2 = u[4] * inv;
-  x[0] = (float)rho;
-  x[1] = (float)(u[5] - 0.5 * rho * (v1 * v1 + v2 * v2));
-  x[2] = (float)(u[1] + u[2]);
-}
-
 __global__ __launch_bounds__(256) void rainy_newton_kernel(RainyLaunch L, double* u,
                                                            MlpView mlp, int use_mlp,
                                                            int* iterations,
@@ -718,21 +754,12 @@ __global__ __launch_bounds__(256) void rainy_newton_kernel(RainyLaunch L, double
     double x[3] = {uc[6], uc[7], uc[8]};
     int fallback = 0;
     if (use_mlp) {
-      float xi[kMlpIn], y[kMlpOut];
-      mlp_inputs(uc, xi);
-      mlp_eval(sP, sP + kMlpParams, xi, y);
-      const double rho = uc[0] + uc[1] + uc[2];
-      const double T = (double)y[0];
-      double rv = (double)y[1] * rho;
-      rv = fmin(fmax(rv, 0.0), uc[1]);
-      x[0] = rv;
-      x[1] = uc[1] - rv;
-      x[2] = T;
-      it = newton_node(uc, x, L.newton_tol, L.newton_max_iter);
+      const bool ok = mlp_guess(sP, uc, x);
+      it = ok ? newton_node(uc, x, L.newton_tol, L.newton_max_iter) : -1;
       if (it < 0 || !(x[2] > 0.0)) {
         // fall back to the default guess, as the unseeded solve
         fallback = 1;
-        const int used = it < 0 ? L.newton_max_iter : it;
+        const int used = ok ? (it < 0 ? L.newton_max_iter : it) : 0;
         x[0] = uc[6];
         x[1] = uc[7];
         x[2] = uc[8];
@@ -785,7 +812,8 @@ bool rainy_launch_stage(const RainyLaunch& L, hipStream_t s) {
 #define ATMO_CASE(NN)                                                                         \
   case NN: {                                                                                  \
     const int grid = (L.n_elements + RShape<NN>::EPB - 1) / RShape<NN>::EPB;                  \
-    hipLaunchKernelGGL(rainy_stage<NN>, dim3(grid), dim3(RShape<NN>::BLOCK), 0, s, L);        \
+    const size_t lds = L.mlp.params ? sizeof(float) * (kMlpParams + 10) : 0;                 \
+    hipLaunchKernelGGL(rainy_stage<NN>, dim3(grid), dim3(RShape<NN>::BLOCK), lds, s, L);      \
     return true;                                                                              \
   }
   ATMO_RAINY_SWITCH(ATMO_CASE)

@@ -35,7 +35,8 @@ struct RainyLaunch {
   double rk_a, rk_dt, rk_b;
   double newton_tol;
   int newton_max_iter;
-  unsigned long long* stats;  // optional: [sum of updates, failures, max updates]
+  unsigned long long* stats;  // optional: [sum of updates, failures, max updates, fallbacks]
+  MlpView mlp;                // seeds the stage limiter's solve when mlp.params is set
 };
 
 bool rainy_launch_stage(const RainyLaunch& L, hipStream_t s);

@@ -142,6 +142,7 @@ SIGNATURES = [
     ("atmo_newton_stats_read", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(AtmoNewtonStats)]),
     ("atmo_rainy_features", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("atmo_rainy_attach_mlp", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("atmo_rainy_seed_stage_limiter", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("atmo_mlp_create", ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
     ("atmo_mlp_destroy", None, [ctypes.c_void_p]),
     ("atmo_mlp_set_normalization", ctypes.c_int, [ctypes.c_void_p, c_float_p]),
@@ -190,6 +191,13 @@ def lib():
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with trixiatmo.jl_amd/build_lib.py "
                 "(there is no CPU fallback)")
+        # torch, when installed, brings its own copy of the HIP runtime; loading
+        # it first lets this library bind to the same runtime, so device
+        # tensors handed across (training samples, iteration counts) are valid
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, res, args in SIGNATURES:
             fn = getattr(L, name)

@@ -154,6 +154,10 @@ class SemidiscretizationRainy:
         self._mlp = mlp
         check(lib().atmo_rainy_attach_mlp(self._h, mlp.handle if mlp else None), self._h)
 
+    def seed_stage_limiter(self, on=True):
+        """seed the Newton solve of every stage with the attached network"""
+        check(lib().atmo_rainy_seed_stage_limiter(self._h, 1 if on else 0), self._h)
+
 
 def _stats(st):
     return {"nodes": st.nodes, "updates": st.updates, "max_updates": st.max_updates,
@@ -213,3 +217,48 @@ class SaturationMLP:
 
     def forward(self, x, y):
         check(lib().atmo_mlp_forward(self._h, _ptr(x), _ptr(y), int(x.shape[0])))
+
+
+def harvest_features(semi, states, steps=0, dt=None):
+    """training samples from rainy states: for each state, `steps` stage-limited
+    RK steps on the device, then (rho, internal energy, total water) ->
+    (T, q_v) of every node; returns float32 device tensors x [N, 3], y [N, 2]"""
+    import torch
+    xs, ys = [], []
+    n = semi.ndofs
+    for u in states:
+        semi.set_state(u)
+        if steps:
+            semi.step(dt if dt is not None else semi.max_dt(0.5), steps)
+        x = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+        y = torch.empty((n, 2), dtype=torch.float32, device="cuda")
+        semi.features(x, y)
+        xs.append(x)
+        ys.append(y)
+    return torch.cat(xs), torch.cat(ys)
+
+
+def train_saturation_mlp(x, y, epochs=20, batch=4096, lr=3e-3, seed=0, log=None):
+    """train a SaturationMLP on device samples with the library's kernels:
+    inputs and outputs standardised by their means and standard deviations,
+    Adam on mini-batches of a fresh permutation every epoch, the learning
+    rate halved over the last third.  Returns (mlp, loss history)."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    mlp = SaturationMLP(seed)
+    xm, xs = x.double().mean(0), x.double().std(0)
+    ym, ys = y.double().mean(0), y.double().std(0)
+    mlp.set_normalization(xm.cpu().numpy(), xs.cpu().numpy() + 1e-30, ym.cpu().numpy(),
+                          ys.cpu().numpy() + 1e-30)
+    n = x.shape[0]
+    n_batches = n // batch
+    history = []
+    for ep in range(epochs):
+        perm = torch.randperm(n, device="cuda", generator=g).to(torch.int32)
+        rate = lr if ep < 2 * epochs // 3 else 0.5 * lr
+        loss = mlp.train(x, y, perm, batch, n_batches, rate)
+        history.append(loss)
+        if log:
+            log(f"mlp epoch {ep}: last batch loss {loss:.3e}")
+    return mlp, history
