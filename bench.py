@@ -126,6 +126,51 @@ def bench_covariant(steps, warmup, trees=32, polydeg=3, n_members=32):
     return out
 
 
+def bench_precision_stress(steps=1000):
+    """the entropy-conservative shell (6 x 8^2 x 4, polydeg 7) in three forms:
+    FP64, FP32 fluxes with Kahan sums, FP32 fluxes with plain sums; each runs
+    `steps` CarpenterKennedy2N54 steps from the same state and reports the
+    relative RHS error against FP64 on the FP64 end state, and the mass and
+    entropy drift over the run"""
+    import numpy as np
+    import trixiatmo_hip as T
+    forms = {}
+    u_end64 = None
+    semis = {}
+    dt = None
+    for prec in ("fp64", "fp32_kahan", "fp32"):
+        mesh, eq, solver, bcs, u0 = T.workloads.precision_stress(prec)
+        semi = T.SemidiscretizationHyperbolic(mesh, eq, solver, bcs)
+        semi.set_state(u0)
+        if dt is None:
+            dt = semi.max_dt(0.5)
+        i0 = semi.integrals(fresh_rhs=True)
+        t0 = time.perf_counter()
+        semi.step(dt, steps)
+        semi.synchronize()
+        el = time.perf_counter() - t0
+        u_end = semi.get_state()
+        i1 = semi.integrals(fresh_rhs=True)
+        forms[prec] = {"mass_drift": (i1[0] - i0[0]) / i0[0],
+                       "entropy_drift": (i1[1] - i0[1]) / abs(i0[1]),
+                       "seconds": el, "ms_per_step": el / steps * 1e3,
+                       "finite": bool(np.all(np.isfinite(u_end)))}
+        log(f"[stress {prec}] {steps} steps in {el:.2f} s, entropy drift "
+            f"{forms[prec]['entropy_drift']:.3e}")
+        if prec == "fp64":
+            u_end64 = u_end
+        semis[prec] = semi
+    d64 = semis["fp64"].rhs(None, u_end64).reshape(6, -1, order="F")
+    for prec in ("fp32_kahan", "fp32"):
+        d = semis[prec].rhs(None, u_end64).reshape(6, -1, order="F")
+        forms[prec]["rhs_rel_err_vs_fp64_per_var"] = [
+            float(x) for x in (np.abs(d - d64).max(axis=1) / np.abs(d64).max(axis=1))[:5]]
+    semis.clear()
+    nodes = mesh.n_elements * mesh.nnodes ** 3
+    return {"workload": "shell 6x8^2x4, polydeg 7, flux_ranocha + Waruszewski volume and surface",
+            "nodes": nodes, "steps": steps, "dt": dt, "forms": forms}
+
+
 def bench_rainy(steps, warmup, train_seeds=4, mlp_epochs=20):
     """rainy Euler 2D, 256 x 128 cells, polydeg 3 (524,288 nodes): stage-limited
     CarpenterKennedy2N54 with the FP64 saturation Newton solve (tol 1e-9, the
@@ -220,6 +265,8 @@ def main():
     ap.add_argument("--no-covariant", action="store_true",
                     help="skip the covariant shallow-water lines")
     ap.add_argument("--no-rainy", action="store_true", help="skip the rainy-Euler line")
+    ap.add_argument("--stress-steps", type=int, default=1000,
+                    help="steps of the precision-stress forms (0 skips them)")
     ap.add_argument("--rainy-steps", type=int, default=20)
     ap.add_argument("--traffic-kib", type=float, nargs=3, default=None,
                     metavar=("FETCH", "WRITE", "FACTOR"),
@@ -317,6 +364,11 @@ def main():
         log("rainy Euler")
         rainy = bench_rainy(a.rainy_steps, min(a.warmup, 3))
 
+    stress = None
+    if a.stress_steps > 0:
+        log("precision stress")
+        stress = bench_precision_stress(a.stress_steps)
+
     cpu = None
     if not a.no_cpu_baseline:
         log("cpu baseline")
@@ -362,6 +414,7 @@ def main():
                        "steps": n_total},
         "covariant_swe": cov,
         "rainy_euler": rainy,
+        "precision_stress": stress,
         "state_finite": ok,
     }
     print(json.dumps(out))

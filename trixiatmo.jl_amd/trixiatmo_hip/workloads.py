@@ -237,3 +237,13 @@ def rainy_bubble(cells=(256, 128), polydeg=3, seed=0, domain=(20000.0, 10000.0),
     u0, info = initial_condition_rainy_bubble(mesh, eq, seed=seed)
     lim = NonlinearSolveDG(newton_tolerance, newton_max_iterations)
     return mesh, eq, solver, "source_terms_rainy", lim, u0, info
+
+
+def precision_stress(precision="fp64", seed=0):
+    """The shell at 6 x 8^2 trees x 4 layers, polydeg 7 (786,432 nodes), with
+    the isothermal 250 K background and 10 seeded 1 K bumps, entropy
+    conservative: (flux_ranocha, flux_nonconservative_waruszewski_etal) in
+    the volume and on the surface, no dissipation, slip walls."""
+    return shell(trees_per_face=8, layers=4, polydeg=7, seed=seed, precision=precision,
+                 volume_flux="flux_ranocha", surface_flux="flux_ranocha",
+                 surface_noncons="flux_nonconservative_waruszewski_etal")
