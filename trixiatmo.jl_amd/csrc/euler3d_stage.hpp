@@ -41,17 +41,21 @@ struct Shape {
   static constexpr int P = N * (N - 1) / 2;  // unordered pairs per line
 };
 
-// pair-buffer budget per workgroup (bytes); lines are chunked to fit
-template <int N, typename T>
+// pair-buffer budget per workgroup (bytes); lines are chunked to fit.  A pair
+// record holds the 5 volume-flux entries and, with a gravity term, its
+// momentum factor sm and energy entry (the momentum entries n * sm are formed
+// again from the averaged normal when the record is gathered).
+template <int N, typename T, int NC = 1>
 struct Chunking {
   static constexpr int BUDGET = 40 * 1024;
-  static constexpr int PAIR_BYTES = 9 * (int)sizeof(T);
+  static constexpr int REC = NC != 0 ? 7 : 5;
+  static constexpr int PAIR_BYTES = REC * (int)sizeof(T);
   static constexpr int LINES_MAX = BUDGET / (PAIR_BYTES * Shape<N>::P);
   static constexpr int LINES = LINES_MAX >= Shape<N>::N2 ? Shape<N>::N2 : LINES_MAX;
   static constexpr int CHUNKS = (Shape<N>::N2 + LINES - 1) / LINES;
   static constexpr int NPC = LINES * Shape<N>::P;  // pairs per chunk
   static constexpr int FACE = 36 * Shape<N>::N2;   // face buffer [6 vars][6 n^2]
-  static constexpr int FLEN = 9 * NPC > FACE ? 9 * NPC : FACE;
+  static constexpr int FLEN = REC * NPC > FACE ? REC * NPC : FACE;
 };
 
 // running sum, optionally Kahan-compensated
@@ -145,31 +149,39 @@ __device__ __forceinline__ T delta_ln_mean(T A, T s, T Ab, T sb, T Ap, T sp, T d
 
 // ln_mean / inv_ln_mean from node logarithms: (y - x) / (log y - log x) away
 // from x = y, the series of Ranocha et al. (2023) where (x-y)^2/(x+y)^2 < 1e-4.
+// Both branches end in one division of a selected numerator and denominator,
+// so lanes of a wave that take different branches do not serialise; f^2 uses
+// a reciprocal refined by one Newton step (it only enters the small series
+// terms).
+__device__ __forceinline__ double rcp_refined(double x) {
+  const double r = __builtin_amdgcn_rcp(x);
+  return r * fma(-x, r, 2.0);
+}
+__device__ __forceinline__ float rcp_refined(float x) { return __builtin_amdgcn_rcpf(x); }
+
 template <typename T>
 __device__ __forceinline__ T ln_mean_logs(T x, T y, T lx, T ly) {
   const T d = y - x, s = x + y;
-  if (d * d < T(1.0e-4) * (s * s)) {
-    const T f2 = div_(d * d, s * s);
-    return div_(s, T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0))));
-  }
-  return div_(d, ly - lx);
+  const bool ser = d * d < T(1.0e-4) * (s * s);
+  const T f2 = ser ? (d * d) * rcp_refined(s * s) : T(0);
+  const T q = T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0)));
+  return div_(ser ? s : d, ser ? q : ly - lx);
 }
 
 template <typename T>
 __device__ __forceinline__ T inv_ln_mean_logs(T x, T y, T lx_minus_ly) {
   const T d = y - x, s = x + y;
-  if (d * d < T(1.0e-4) * (s * s)) {
-    const T f2 = div_(d * d, s * s);
-    return div_(T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0))), s);
-  }
-  return div_(-lx_minus_ly, d);
+  const bool ser = d * d < T(1.0e-4) * (s * s);
+  const T f2 = ser ? (d * d) * rcp_refined(s * s) : T(0);
+  const T q = T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0)));
+  return div_(ser ? q : -lx_minus_ly, ser ? s : d);
 }
 
 // the volume flux f (5 entries) and the gravity term g (4 entries) of one pair
 template <int VOL, int NC, typename T>
 __device__ __forceinline__ void pair_flux(const E3Params& q, const FastPrim<T>& l,
                                           const FastPrim<T>& r, T n0, T n1, T n2, T dphi,
-                                          const Background& bg, T f[5], T g[4]) {
+                                          const Background& bg, T f[5], T g[5]) {
   const T v1a = T(0.5) * (l.v1 + r.v1), v2a = T(0.5) * (l.v2 + r.v2), v3a = T(0.5) * (l.v3 + r.v3);
   const T pa = T(0.5) * (l.p + r.p);
   // momentum pressure term: p' when the hydrostatic background is split off
@@ -226,6 +238,7 @@ __device__ __forceinline__ void pair_flux(const E3Params& q, const FastPrim<T>& 
     g[1] = n1 * sm;
     g[2] = n2 * sm;
     g[3] = n0 * sj * v1a + n1 * sj * v2a + n2 * sj * v3a;
+    g[4] = sm;
   }
 }
 
@@ -401,18 +414,22 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
     const int* __restrict__ face_info, int surf, int snc, int mode, double rk_a, double rk_dt,
     double rk_b, double coriolis, Background bg) {
   using S = Shape<N>;
-  using C = Chunking<N, T>;
+  using C = Chunking<N, T, NC>;
   constexpr int N2 = S::N2, N3 = S::N3, P = S::P, B = S::BLOCK;
   constexpr int NV = 6;
   constexpr bool MIXED = sizeof(T) == 4;
-  constexpr int NW = MIXED ? 13 : 9;  // per-node fields (the last 4: background split)
+  // per-node fields: rho v1 v2 v3 p e phi log(rho) [log(rho/p)] [4 of the background split]
+  constexpr int NW = MIXED ? 13 : (VOL == 0 ? 9 : 8);
   __shared__ T s_w[NW][N3];
   __shared__ double s_phi[MIXED ? N3 : 1];  // FP64 phi for the jumps of the mixed path
   __shared__ double s_pb[MIXED ? N3 : 1];   // FP64 background pressure
   __shared__ T s_ja[3][N3];
   __shared__ T s_fl[C::FLEN];  // pair terms [9][NPC], later face fluxes [6][6 n^2]
   __shared__ T s_ds[N * N];
+  __shared__ unsigned char s_pa[P], s_pm[P];  // node pair (a, m), a < m, of each pair slot
 
+  // the FP64 form never splits off a background: make that a compile-time fact
+  if constexpr (!MIXED) bg.on = 0;
   const int e = xcd_element(blockIdx.x, n_elements);
   if (e >= n_elements) return;  // whole workgroup: padding of the XCD mapping
   const int t = threadIdx.x;
@@ -420,9 +437,15 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
   const int ti = t % N, tj = (t / N) % N, tk = t / N2;
   const size_t gnode = (size_t)e * N3 + t;
   if (t < N * N) s_ds[t] = T(dst.ds[t]);
+  if (t < P) {
+    int qq = t, a = 0;
+    while (qq >= N - 1 - a) { qq -= N - 1 - a; ++a; }
+    s_pa[t] = (unsigned char)a;
+    s_pm[t] = (unsigned char)(a + 1 + qq);
+  }
 
-  double uo[NV];
   if (active) {
+    double uo[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) uo[v] = u[gnode * NV + v];
     // primitives (pressure from total minus kinetic energy) in FP64
@@ -466,16 +489,17 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
       for (int pid = t; pid < n_lines * P; pid += B) {
         const int lrel = pid / P;
         const int line = line0 + lrel;
-        int qq = pid - lrel * P, a = 0;
-        while (qq >= N - 1 - a) { qq -= N - 1 - a; ++a; }
-        const int m = a + 1 + qq;
+        const int slot = pid - lrel * P;
+        const int a = s_pa[slot], m = s_pm[slot];
         const int base = d == 0 ? N * line : (d == 1 ? (line % N) + N2 * (line / N) : line);
         const int na = base + a * stride, nm = base + m * stride;
         FastPrim<T> l, r;
         l.rho = s_w[0][na]; l.v1 = s_w[1][na]; l.v2 = s_w[2][na]; l.v3 = s_w[3][na];
-        l.p = s_w[4][na]; l.e = s_w[5][na]; l.phi = s_w[6][na]; l.lrho = s_w[7][na]; l.lrp = s_w[8][na];
+        l.p = s_w[4][na]; l.e = s_w[5][na]; l.phi = s_w[6][na]; l.lrho = s_w[7][na];
+        if constexpr (VOL == 0) l.lrp = s_w[8][na];
         r.rho = s_w[0][nm]; r.v1 = s_w[1][nm]; r.v2 = s_w[2][nm]; r.v3 = s_w[3][nm];
-        r.p = s_w[4][nm]; r.e = s_w[5][nm]; r.phi = s_w[6][nm]; r.lrho = s_w[7][nm]; r.lrp = s_w[8][nm];
+        r.p = s_w[4][nm]; r.e = s_w[5][nm]; r.phi = s_w[6][nm]; r.lrho = s_w[7][nm];
+        if constexpr (VOL == 0) r.lrp = s_w[8][nm];
         T dphi;
         if constexpr (MIXED) {
           dphi = T(s_phi[nm] - s_phi[na]);
@@ -489,17 +513,23 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
         const T n0 = T(0.5) * (s_ja[0][na] + s_ja[0][nm]);
         const T n1 = T(0.5) * (s_ja[1][na] + s_ja[1][nm]);
         const T n2 = T(0.5) * (s_ja[2][na] + s_ja[2][nm]);
-        T f[5], g[4];
+        T f[5], g[5];
         pair_flux<VOL, NC, T>(q, l, r, n0, n1, n2, dphi, bg, f, g);
 #pragma unroll
         for (int v = 0; v < 5; ++v) s_fl[v * C::NPC + pid] = f[v];
         if constexpr (NC != 0) {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) s_fl[(5 + v) * C::NPC + pid] = g[v];
+          s_fl[5 * C::NPC + pid] = g[4];  // sm
+          s_fl[6 * C::NPC + pid] = g[3];  // energy entry
         }
       }
       __syncthreads();
       if (active && own_line >= line0 && own_line < line0 + n_lines) {
+        T jo[3] = {T(0), T(0), T(0)};
+        if constexpr (NC != 0) {
+          jo[0] = s_ja[0][t];
+          jo[1] = s_ja[1][t];
+          jo[2] = s_ja[2][t];
+        }
 #pragma unroll
         for (int m = 0; m < N; ++m) {
           if (m == own_d) continue;
@@ -510,8 +540,12 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
           for (int v = 0; v < 5; ++v) acc[v].add(ds * s_fl[v * C::NPC + pid]);
           if constexpr (NC != 0) {
             const T dsg = own_d < m ? ds : -ds;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc_nc[v].add(dsg * s_fl[(5 + v) * C::NPC + pid]);
+            const int pn = t + (m - own_d) * stride;  // the partner node
+            const T sm = s_fl[5 * C::NPC + pid];
+            acc_nc[0].add(dsg * ((T(0.5) * (jo[0] + s_ja[0][pn])) * sm));
+            acc_nc[1].add(dsg * ((T(0.5) * (jo[1] + s_ja[1][pn])) * sm));
+            acc_nc[2].add(dsg * ((T(0.5) * (jo[2] + s_ja[2][pn])) * sm));
+            acc_nc[3].add(dsg * s_fl[6 * C::NPC + pid]);
           }
         }
       }
@@ -544,6 +578,9 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
   const double fac = -invJ[gnode];
 #pragma unroll
   for (int v = 0; v < NV; ++v) du[v] *= fac;
+  double uo[NV];  // read again rather than kept live through the pair phase
+#pragma unroll
+  for (int v = 0; v < NV; ++v) uo[v] = u[gnode * NV + v];
   if (bg.on) {  // tendency of the hydrostatic background, computed in FP64
     du[1] += double(bg.B[gnode * 3 + 0]);
     du[2] += double(bg.B[gnode * 3 + 1]);
