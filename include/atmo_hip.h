@@ -212,8 +212,20 @@ typedef struct {
 int atmo_swe_create(const atmo_swe_config* config, const atmo_geometry2d* geometry,
                     atmo_handle** handle);
 
+/* Autotuner of the covariant stage kernel (no counterpart in the reference):
+ * times every workgroup shape (1, 2 or 4 waves per workgroup, each with a
+ * full or a persistent grid) on the resident state and keeps the fastest;
+ * `ms` (may be NULL) receives the time of each of the ATMO_SWE_VARIANTS
+ * shapes.  atmo_step tunes by itself on its first call; every shape gives
+ * bit-identical results. */
+#define ATMO_SWE_VARIANTS 6
+int atmo_autotune(atmo_handle* handle, int32_t* variant, double* ms);
+/* pin the workgroup shape of the covariant stage kernel (0..ATMO_SWE_VARIANTS-1) */
+int atmo_set_kernel_variant(atmo_handle* handle, int32_t variant);
+
 /* ensemble variants: per-member max_dt (dt[n_members]) and RK steps with
- * per-member time steps */
+ * per-member time steps (two steps at a time replayed from a captured
+ * hipGraph) */
 int atmo_max_dt_members(atmo_handle* handle, double cfl, double* dt);
 int atmo_step_members(atmo_handle* handle, const double* dt, int32_t n_steps);
 

@@ -181,3 +181,36 @@ def test_ensemble_conserves_mass_per_member():
         m0 = (wJ[..., sl] * u0[0][..., sl]).sum()
         m1 = (wJ[..., sl] * u[0][..., sl]).sum()
         assert abs(m1 - m0) <= 1e-13 * m0
+
+
+def test_autotuner_shapes_and_graph_replay_are_bit_identical():
+    """every workgroup shape gives the same bits, and steps replayed from the
+    captured graph equal steps launched one by one"""
+    case = W.zonal_jet(6, 3, seed=9)
+    semi = case.semi()
+    u0 = perturbed(case, amp=0.002)
+    semi.set_state(u0)
+    chosen, ms = semi.autotune()
+    print("autotune", chosen, ms, flush=True)
+    assert 0 <= chosen < 6 and np.all(ms > 0) and ms[chosen] == ms.min()
+    outs = []
+    for v in range(6):
+        semi.set_kernel_variant(v)
+        outs.append(semi.rhs(None, u0))
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
+    semi.set_kernel_variant(chosen)
+    dt = semi.max_dt(0.4)
+    semi.set_state(u0)
+    semi.step(dt, 4)            # two graph replays
+    a = semi.get_state()
+    semi.set_state(u0)
+    for _ in range(4):
+        semi.step(dt, 1)        # launched one by one
+    b = semi.get_state()
+    np.testing.assert_array_equal(a, b)
+    for v in (3, 5):            # persistent grids inside the graph as well
+        semi.set_kernel_variant(v)
+        semi.set_state(u0)
+        semi.step(dt, 4)
+        np.testing.assert_array_equal(semi.get_state(), a)

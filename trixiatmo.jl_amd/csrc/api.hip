@@ -422,6 +422,7 @@ void atmo_destroy(atmo_handle* h) {
   if (h->d_member) hipFree(h->d_member);
   if (h->d_member_first) hipFree(h->d_member_first);
   if (h->d_face_info) hipFree(h->d_face_info);
+  if (h->swe_graph) hipGraphExecDestroy(h->swe_graph);
   rainy_free(h);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;

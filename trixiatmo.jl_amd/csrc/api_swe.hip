@@ -63,35 +63,120 @@ int swe_rhs_device(atmo_handle* h, double* du, const double* u) {
   return launch(h, L);
 }
 
-int swe_step(atmo_handle* h, const double* dt_members, double dt, int n_steps) {
-  static const double A[5] = {0.0, -567301805773.0 / 1357537059087.0,
-                              -2404267990393.0 / 2016746695238.0,
-                              -3550918686646.0 / 2091501179385.0,
-                              -1275806237668.0 / 842570457699.0};
-  static const double Bc[5] = {1432997174477.0 / 9575080441755.0,
-                               5161836677717.0 / 13612068292357.0,
-                               1720146321549.0 / 2090206949498.0,
-                               3134564353537.0 / 4481467310338.0,
-                               2277821191437.0 / 14882151754819.0};
+namespace {
+
+const double kA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                      -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+const double kB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                      1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                      2277821191437.0 / 14882151754819.0};
+
+// the 5 stage launches of n_steps CarpenterKennedy2N54 steps, time steps read
+// from d_dt_member
+int enqueue_steps(atmo_handle* h, int n_steps) {
   SweLaunch L = h->swe_base;
   L.mode = 1;
   L.k = h->d_k;
-  L.rk_dt = dt;
-  if (dt_members) {
-    HIP_TRY(h, hipMemcpyAsync(h->d_dt_member, dt_members, sizeof(double) * h->n_members,
-                              hipMemcpyHostToDevice, h->stream));
-    L.dt_member = h->d_dt_member;
-  }
+  L.dt_member = h->d_dt_member;
   for (int s = 0; s < n_steps; ++s)
     for (int st = 0; st < 5; ++st) {
       L.u = h->d_u;
       L.out = h->d_u2;
-      L.rk_a = A[st];
-      L.rk_b = Bc[st];
+      L.rk_a = kA[st];
+      L.rk_b = kB[st];
       const int rc = launch(h, L);
       if (rc != ATMO_OK) return rc;
       std::swap(h->d_u, h->d_u2);
     }
+  return ATMO_OK;
+}
+
+// capture two steps (ten launches; the ping-pong buffers come back to their
+// places) into a graph for replay
+int ensure_graph(atmo_handle* h) {
+  if (h->swe_graph && h->swe_graph_variant == h->swe_base.variant) return ATMO_OK;
+  if (h->swe_graph) {
+    HIP_TRY(h, hipGraphExecDestroy(h->swe_graph));
+    h->swe_graph = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_steps(h, 2);
+  const hipError_t ec = hipStreamEndCapture(h->stream, &g);
+  if (rc != ATMO_OK) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  HIP_TRY(h, ec);
+  const hipError_t ei = hipGraphInstantiate(&h->swe_graph, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  HIP_TRY(h, ei);
+  h->swe_graph_variant = h->swe_base.variant;
+  return ATMO_OK;
+}
+
+}  // namespace
+
+int swe_autotune(atmo_handle* h, double* ms_per_variant) {
+  hipEvent_t a, b;
+  HIP_TRY(h, hipEventCreate(&a));
+  HIP_TRY(h, hipEventCreate(&b));
+  SweLaunch L = h->swe_base;
+  L.mode = 0;
+  L.u = h->d_u;
+  L.out = h->d_du;
+  int best = 0;
+  float best_ms = 1e30f;
+  for (int v = 0; v < kSweVariants; ++v) {
+    L.variant = v;
+    float ms = 1e30f;
+    if (swe_launch_stage(L, h->stream)) {
+      HIP_TRY(h, hipEventRecord(a, h->stream));
+      for (int r = 0; r < 3; ++r) swe_launch_stage(L, h->stream);
+      HIP_TRY(h, hipEventRecord(b, h->stream));
+      HIP_TRY(h, hipEventSynchronize(b));
+      HIP_TRY(h, hipEventElapsedTime(&ms, a, b));
+      ms /= 3.0f;
+    }
+    if (ms_per_variant) ms_per_variant[v] = ms;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = v;
+    }
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  HIP_TRY(h, hipGetLastError());
+  h->swe_base.variant = best;
+  h->swe_tuned = true;
+  h->du_valid = false;
+  return ATMO_OK;
+}
+
+int swe_step(atmo_handle* h, const double* dt_members, double dt, int n_steps) {
+  std::vector<double> dts;
+  if (!dt_members) {
+    dts.assign(h->n_members, dt);
+    dt_members = dts.data();
+  }
+  HIP_TRY(h, hipMemcpyAsync(h->d_dt_member, dt_members, sizeof(double) * h->n_members,
+                            hipMemcpyHostToDevice, h->stream));
+  if (!h->swe_tuned) {
+    const int rc = swe_autotune(h, nullptr);
+    if (rc != ATMO_OK) return rc;
+  }
+  int done = 0;
+  if (!h->timing && n_steps >= 2) {
+    int rc = ensure_graph(h);
+    if (rc != ATMO_OK) return rc;
+    for (; done + 2 <= n_steps; done += 2) HIP_TRY(h, hipGraphLaunch(h->swe_graph, h->stream));
+  }
+  if (done < n_steps) {
+    const int rc = enqueue_steps(h, n_steps - done);
+    if (rc != ATMO_OK) return rc;
+  }
+  // the host copy of the time steps may go out of scope: finish the upload
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
   h->du_valid = false;
   return ATMO_OK;
 }
@@ -295,7 +380,31 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
   L.nbr = h->d_nbr;
   L.member = h->d_member;
   L.dt_member = nullptr;
+  L.variant = 0;
   *out = h;
+  return ATMO_OK;
+}
+
+int atmo_autotune(atmo_handle* h, int32_t* variant, double* ms) {
+  if (!h) return fail(nullptr, ATMO_ERR_INVALID, "null handle");
+  if (h->eq != ATMO_EQ_COVARIANT_SWE_2D) {
+    if (variant) *variant = 0;
+    return ATMO_OK;  // the other stage kernels have one shape
+  }
+  const int rc = swe_autotune(h, ms);
+  if (rc != ATMO_OK) return rc;
+  if (variant) *variant = h->swe_base.variant;
+  return ATMO_OK;
+}
+
+int atmo_set_kernel_variant(atmo_handle* h, int32_t variant) {
+  if (!h) return fail(nullptr, ATMO_ERR_INVALID, "null handle");
+  if (h->eq != ATMO_EQ_COVARIANT_SWE_2D)
+    return variant == 0 ? ATMO_OK : fail(&h->err, ATMO_ERR_INVALID, "one shape only");
+  if (variant < 0 || variant >= kSweVariants)
+    return fail(&h->err, ATMO_ERR_INVALID, "no such kernel variant");
+  h->swe_base.variant = variant;
+  h->swe_tuned = true;
   return ATMO_OK;
 }
 

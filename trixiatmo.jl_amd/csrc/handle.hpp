@@ -52,6 +52,9 @@ struct atmo_handle {
   double *d_aux = nullptr, *d_xyz = nullptr, *d_dt_member = nullptr;
   int *d_member = nullptr, *d_member_first = nullptr;
   int n_members = 1;
+  bool swe_tuned = false;                  // workgroup shape chosen by the autotuner
+  hipGraphExec_t swe_graph = nullptr;      // two RK steps captured for replay
+  int swe_graph_variant = -1;
   // rainy Euler
   atmo_rainy_config rainy{};
   atmo::RainyLaunch rainy_base{};
@@ -82,6 +85,7 @@ int swe_rhs_device(atmo_handle* h, double* du, const double* u);
 int swe_step(atmo_handle* h, const double* dt_members, double dt, int n_steps);
 int swe_max_dt(atmo_handle* h, double cfl, double* dt_members);
 int swe_integrals(atmo_handle* h, int fresh_rhs, double* out);
+int swe_autotune(atmo_handle* h, double* ms_per_variant);
 // rainy Euler paths of the generic entry points
 int rainy_rhs_device(atmo_handle* h, double* du, const double* u, double t);
 int rainy_step(atmo_handle* h, double dt, int n_steps);

@@ -3,11 +3,13 @@
 //
 // Layout: u [3, n, n, E] (h, h v^1, h v^2), aux [26, n, n, E] as Trixi's
 // aux_node_vars (src/equations/equations.jl:178-250), node coordinates
-// [3, n, n, E].  A workgroup of 64 lanes holds EPB = max(1, 64 / n^2) elements
-// (one lane per node); each element's state and auxiliary block is staged in
-// LDS.  Volume terms are gathered per node: sum_m Dsplit[own, m] (f#(own, m) +
-// g(own, m) / 2) along both directions (the symmetric f# is evaluated for each
-// ordering, the nonconservative g is non-symmetric anyway).  Face nodes are
+// [3, n, n, E].  A workgroup of 64, 128 or 256 lanes holds EPB = lanes / n^2
+// elements (one lane per node); the per-node fields the two-point fluxes need
+// (state, velocity v = hv / h and covariant velocity G v formed once per node,
+// J, G, G^-1, h_s) are staged in LDS.  Volume terms are gathered per node:
+// sum_m Dsplit[own, m] (f#(own, m) + g(own, m) / 2) along both directions (the
+// symmetric f# is evaluated for each ordering, the nonconservative g is
+// non-symmetric anyway).  Face nodes are
 // then mapped to lanes: each element evaluates the interface flux of its own
 // faces in its own contravariant basis with its own auxiliary variables,
 // exactly as the reference does for each side
@@ -146,131 +148,181 @@ __device__ __forceinline__ void source_split(double omega, const double* u, cons
   s[2] = -(sg1 + f * a.J() * (a.Gcon(1, 1) * hv0 - a.Gcon(1, 0) * hv1));
 }
 
-template <int N>
+// Workgroup shape: WAVES waves of 64 lanes, one lane per node, EPB = WAVES *
+// 64 / n^2 elements (n^2 <= 64); with PERSIST the grid is capped and every
+// workgroup loops over element groups with a grid stride.
+template <int N, int WAVES>
 struct SweShape {
   static constexpr int N2 = N * N;
-  static constexpr int EPB = N2 >= 64 ? 1 : 64 / N2;  // elements per workgroup
-  static constexpr int BLOCK = ((EPB * N2 + 63) / 64) * 64;
+  static constexpr int BLOCK = 64 * WAVES;
+  static constexpr int EPB = BLOCK / N2;
   static constexpr int NF = 4 * N;  // face nodes per element
 };
 
+// per-node fields staged in LDS: h, hv^1, hv^2, v^1, v^2 (v = hv / h), the
+// covariant velocity G v, J, G_11, G_12, G_22, G^11, G^12, G^22, h_s
+constexpr int kSweFields = 15;
+
 }  // namespace
 
-template <int N>
-__global__ __launch_bounds__(SweShape<N>::BLOCK) void swe_stage(SweLaunch L) {
-  using S = SweShape<N>;
+template <int N, int WAVES, bool PERSIST>
+__global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
+  using S = SweShape<N, WAVES>;
   constexpr int N2 = S::N2, EPB = S::EPB, B = S::BLOCK;
-  __shared__ double s_u[EPB][NV][N2];
-  __shared__ double s_a[EPB][NA][N2];
+  __shared__ double s_n[EPB][kSweFields][N2];
   __shared__ double s_sf[EPB][NV][S::NF];
-  __shared__ double s_ds[N * N];
 
   const int t = threadIdx.x;
-  const int e0 = blockIdx.x * EPB;
-  if (t < N * N) s_ds[t] = L.dsplit[t];
-  // stage the elements' state and auxiliary variables (contiguous per element)
-  for (int idx = t; idx < EPB * N2 * NV; idx += B) {
-    const int es = idx / (N2 * NV), r = idx - es * N2 * NV;
-    if (e0 + es < L.n_elements) s_u[es][r % NV][r / NV] = L.u[(size_t)(e0 + es) * N2 * NV + r];
-  }
-  for (int idx = t; idx < EPB * N2 * NA; idx += B) {
-    const int es = idx / (N2 * NA), r = idx - es * N2 * NA;
-    if (e0 + es < L.n_elements) s_a[es][r % NA][r / NA] = L.aux[(size_t)(e0 + es) * N2 * NA + r];
-  }
-  __syncthreads();
-
   const int es = t / N2, q = t - es * N2;
-  const int e = e0 + es;
-  const bool active = es < EPB && e < L.n_elements;
   const int ti = q % N, tj = q / N;
-  double du[NV] = {0, 0, 0};
-  if (active) {
-    const double uo[NV] = {s_u[es][0][q], s_u[es][1][q], s_u[es][2][q]};
-    const AuxV ao{&s_a[es][0][q], N2};
+  const int groups = (L.n_elements + EPB - 1) / EPB;
+  for (int grp = blockIdx.x; grp < groups; grp += PERSIST ? gridDim.x : groups) {
+    const int e0 = grp * EPB;
+    const int e = e0 + es;
+    const bool active = es < EPB && e < L.n_elements;
+    if (PERSIST) __syncthreads();  // the previous group's LDS reads are done
+    if (active) {
+      const size_t g = (size_t)e * N2 + q;
+      const double* un = L.u + g * NV;
+      const double* an = L.aux + g * NA;
+      const double h = un[0], hv1 = un[1], hv2 = un[2];
+      const double v1 = hv1 / h, v2 = hv2 / h;
+      const double G11 = an[13], G12 = an[14], G22 = an[15];
+      s_n[es][0][q] = h;
+      s_n[es][1][q] = hv1;
+      s_n[es][2][q] = hv2;
+      s_n[es][3][q] = v1;
+      s_n[es][4][q] = v2;
+      s_n[es][5][q] = G11 * v1 + G12 * v2;
+      s_n[es][6][q] = G12 * v1 + G22 * v2;
+      s_n[es][7][q] = an[12];
+      s_n[es][8][q] = G11;
+      s_n[es][9][q] = G12;
+      s_n[es][10][q] = G22;
+      s_n[es][11][q] = an[16];
+      s_n[es][12][q] = an[17];
+      s_n[es][13][q] = an[18];
+      s_n[es][14][q] = an[19];
+    }
+    __syncthreads();
+
+    double du[NV] = {0, 0, 0};
+    if (active) {
+      const double hl = s_n[es][0][q], hvl[2] = {s_n[es][1][q], s_n[es][2][q]};
+      const double vl[2] = {s_n[es][3][q], s_n[es][4][q]};
+      const double Jl = s_n[es][7][q], hsl = s_n[es][14][q];
+      const double Gi[3] = {s_n[es][11][q], s_n[es][12][q], s_n[es][13][q]};
 #pragma unroll
-    for (int o = 0; o < 2; ++o) {
-      const int own = o == 0 ? ti : tj;
+      for (int o = 0; o < 2; ++o) {
+        const int own = o == 0 ? ti : tj;
 #pragma unroll
-      for (int m = 0; m < N; ++m) {
-        if (m == own) continue;
-        const int pq = o == 0 ? m + N * tj : ti + N * m;
-        const double up[NV] = {s_u[es][0][pq], s_u[es][1][pq], s_u[es][2][pq]};
-        const AuxV ap{&s_a[es][0][pq], N2};
-        double f[NV], g[NV];
-        if (L.volume_flux == 0) flux_ec(uo, up, ao, ap, o, f);
-        else flux_central(uo, up, ao, ap, o, f);
-        noncons_ec(L.gravity, uo, up, ao, ap, o, g);
-        const double ds = s_ds[own * N + m];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) du[v] += ds * f[v] + 0.5 * ds * g[v];
+        for (int m = 0; m < N; ++m) {
+          if (m == own) continue;
+          const int pq = o == 0 ? m + N * tj : ti + N * m;
+          const double hr = s_n[es][0][pq], hvr_o = s_n[es][1 + o][pq];
+          const double vr[2] = {s_n[es][3][pq], s_n[es][4][pq]};
+          const double vc0 = s_n[es][5][pq], vc1 = s_n[es][6][pq];
+          const double Jr = s_n[es][7][pq];
+          double f[NV];
+          if (L.volume_flux == 0) {  // flux_ec (:100-122)
+            const double mass = 0.5 * (Jl * hvl[o] + Jr * hvr_o);
+            f[0] = mass;
+            f[1] = 0.5 * (vl[0] + vr[0]) * mass;
+            f[2] = 0.5 * (vl[1] + vr[1]) * mass;
+          } else {  // flux_central of the split flux (:66-80)
+            const double ml = Jl * hvl[o], mr = Jr * hvr_o;
+            f[0] = 0.5 * (ml + mr);
+            f[1] = 0.5 * (ml * vl[0] + mr * vr[0]);
+            f[2] = 0.5 * (ml * vl[1] + mr * vr[1]);
+          }
+          // flux_nonconservative_ec (:157-187) of (own, partner)
+          const double hs_jump = s_n[es][14][pq] - hsl;
+          const double geo0 = 0.5 * hvl[o] * ((Gi[0] * vc0 + Gi[1] * vc1) - vr[0]);
+          const double geo1 = 0.5 * hvl[o] * ((Gi[1] * vc0 + Gi[2] * vc1) - vr[1]);
+          const double Gio0 = o == 0 ? Gi[0] : Gi[1], Gio1 = o == 0 ? Gi[1] : Gi[2];
+          const double pr0 = L.gravity * Gio0 * hl * (hr + hs_jump);
+          const double pr1 = L.gravity * Gio1 * hl * (hr + hs_jump);
+          const double g1 = Jl * (geo0 + pr0), g2 = Jl * (geo1 + pr1);
+          const double ds = L.dsplit[own * N + m];
+          du[0] += ds * f[0];
+          du[1] += ds * f[1] + 0.5 * ds * g1;
+          du[2] += ds * f[2] + 0.5 * ds * g2;
+        }
       }
     }
-  }
-  // face nodes of the workgroup's elements, one lane each
-  const double inv_w = L.inv_w;
-  for (int fid = t; fid < EPB * S::NF; fid += B) {
-    const int fe = fid / S::NF, r = fid - fe * S::NF;
-    const int f = r / N, k = r - f * N;
-    if (e0 + fe >= L.n_elements) continue;
-    const int i = f == 0 ? 0 : (f == 1 ? N - 1 : k);
-    const int j = f == 2 ? 0 : (f == 3 ? N - 1 : k);
-    const int qn = i + N * j;
-    const double ul[NV] = {s_u[fe][0][qn], s_u[fe][1][qn], s_u[fe][2][qn]};
-    const AuxV ax{&s_a[fe][0][qn], N2};
-    const int link = L.nbr[((size_t)(e0 + fe) * 4 + f) * N + k];
-    const double* ur = L.u + (size_t)link * NV;
-    const double* ar = L.aux + (size_t)link * NA;  // neighbour basis_covariant = ar[0..5]
-    // neighbour state in this element's contravariant basis (contravariant2global,
-    // then global2contravariant: covariant_shallow_water.jl:140-153)
-    double gl[3];
+    // face nodes of the group's elements, one lane each
+    const double inv_w = L.inv_w;
+    for (int fid = t; fid < EPB * S::NF; fid += B) {
+      const int fe = fid / S::NF, r = fid - fe * S::NF;
+      const int f = r / N, k = r - f * N;
+      if (e0 + fe >= L.n_elements) continue;
+      const int i = f == 0 ? 0 : (f == 1 ? N - 1 : k);
+      const int j = f == 2 ? 0 : (f == 3 ? N - 1 : k);
+      const int qn = i + N * j;
+      const double ul[NV] = {s_n[fe][0][qn], s_n[fe][1][qn], s_n[fe][2][qn]};
+      const double* ao = L.aux + ((size_t)(e0 + fe) * N2 + qn) * NA;
+      double xa[NA];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) gl[c] = ar[c] * ur[1] + ar[3 + c] * ur[2];
-    const double tr[NV] = {ur[0], ax.B(0, 0) * gl[0] + ax.B(0, 1) * gl[1] + ax.B(0, 2) * gl[2],
-                           ax.B(1, 0) * gl[0] + ax.B(1, 1) * gl[1] + ax.B(1, 2) * gl[2]};
-    const int o = f / 2;
-    double fs[NV];
-    if (f % 2 == 0) {  // negative-side face: -(f*(u_nbr, u_own) + g/2)
-      surface_terms(L, tr, ul, ax, o, fs);
+      for (int c = 0; c < 20; ++c) xa[c] = ao[c];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) fs[v] = -fs[v];
-    } else {
-      surface_terms(L, ul, tr, ax, o, fs);
+      for (int c = 20; c < NA; ++c) xa[c] = 0.0;
+      const AuxV ax{xa, 1};
+      const int link = L.nbr[((size_t)(e0 + fe) * 4 + f) * N + k];
+      const double* ur = L.u + (size_t)link * NV;
+      const double* ar = L.aux + (size_t)link * NA;  // neighbour basis_covariant = ar[0..5]
+      // neighbour state in this element's contravariant basis (contravariant2global,
+      // then global2contravariant: covariant_shallow_water.jl:140-153)
+      double gl[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) gl[c] = ar[c] * ur[1] + ar[3 + c] * ur[2];
+      const double tr[NV] = {ur[0], ax.B(0, 0) * gl[0] + ax.B(0, 1) * gl[1] + ax.B(0, 2) * gl[2],
+                             ax.B(1, 0) * gl[0] + ax.B(1, 1) * gl[1] + ax.B(1, 2) * gl[2]};
+      const int o = f / 2;
+      double fs[NV];
+      if (f % 2 == 0) {  // negative-side face: -(f*(u_nbr, u_own) + g/2)
+        surface_terms(L, tr, ul, ax, o, fs);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) fs[v] = -fs[v];
+      } else {
+        surface_terms(L, ul, tr, ax, o, fs);
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) s_sf[fe][v][r] = fs[v] * inv_w;
     }
+    __syncthreads();
+    if (active) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) s_sf[fe][v][r] = fs[v] * inv_w;
-  }
-  __syncthreads();
-  if (!active) return;
+      for (int f = 0; f < 4; ++f) {
+        const bool on = f == 0 ? ti == 0 : f == 1 ? ti == N - 1 : f == 2 ? tj == 0 : tj == N - 1;
+        if (!on) continue;
+        const int k = f < 2 ? tj : ti;
 #pragma unroll
-  for (int f = 0; f < 4; ++f) {
-    const bool on = f == 0 ? ti == 0 : f == 1 ? ti == N - 1 : f == 2 ? tj == 0 : tj == N - 1;
-    if (!on) continue;
-    const int k = f < 2 ? tj : ti;
+        for (int v = 0; v < NV; ++v) du[v] += s_sf[es][v][f * N + k];
+      }
+      const size_t g = (size_t)e * N2 + q;
+      const double fac = -1.0 / s_n[es][7][q];
+      const double uo[NV] = {s_n[es][0][q], s_n[es][1][q], s_n[es][2][q]};
 #pragma unroll
-    for (int v = 0; v < NV; ++v) du[v] += s_sf[es][v][f * N + k];
-  }
-  const AuxV ao{&s_a[es][0][q], N2};
-  const double fac = -1.0 / ao.J();
-  const size_t g = (size_t)e * N2 + q;
-  const double uo[NV] = {s_u[es][0][q], s_u[es][1][q], s_u[es][2][q]};
+      for (int v = 0; v < NV; ++v) du[v] *= fac;
+      if (L.source_terms) {
+        const AuxV ao{L.aux + g * NA, 1};
+        double sv[NV];
+        source_split(L.rotation_rate, uo, L.xyz + g * 3, ao, sv);
 #pragma unroll
-  for (int v = 0; v < NV; ++v) du[v] *= fac;
-  if (L.source_terms) {
-    double sv[NV];
-    source_split(L.rotation_rate, uo, L.xyz + g * 3, ao, sv);
+        for (int v = 0; v < NV; ++v) du[v] += sv[v];
+      }
+      if (L.mode == 0) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) du[v] += sv[v];
-  }
-  if (L.mode == 0) {
+        for (int v = 0; v < NV; ++v) L.out[g * NV + v] = du[v];
+      } else {
+        const double dt = L.dt_member ? L.dt_member[L.member[e]] : L.rk_dt;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) L.out[g * NV + v] = du[v];
-  } else {
-    const double dt = L.dt_member ? L.dt_member[L.member[e]] : L.rk_dt;
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      const double kn = L.rk_a * L.k[g * NV + v] + dt * du[v];
-      L.k[g * NV + v] = kn;
-      L.out[g * NV + v] = uo[v] + L.rk_b * kn;
+        for (int v = 0; v < NV; ++v) {
+          const double kn = L.rk_a * L.k[g * NV + v] + dt * du[v];
+          L.k[g * NV + v] = kn;
+          L.out[g * NV + v] = uo[v] + L.rk_b * kn;
+        }
+      }
     }
   }
 }
@@ -318,23 +370,48 @@ __global__ void swe_member_max(const double* partial, const int* member_first, i
   if (t == 0) out[m] = s[0];
 }
 
+namespace {
+
+template <int N, int WAVES, bool PERSIST>
+void launch_variant(const SweLaunch& L, hipStream_t s) {
+  using S = SweShape<N, WAVES>;
+  const int groups = (L.n_elements + S::EPB - 1) / S::EPB;
+  // persistent: a few workgroups per CU (256 CUs), each looping over groups
+  const int grid = PERSIST ? (groups < 256 * 8 ? groups : 256 * 8) : groups;
+  hipLaunchKernelGGL((swe_stage<N, WAVES, PERSIST>), dim3(grid), dim3(S::BLOCK), 0, s, L);
+}
+
+template <int N>
+bool launch_n(const SweLaunch& L, int variant, hipStream_t s) {
+  // variants: 0..2 = 1, 2, 4 waves per workgroup; +3 = persistent grid
+  if constexpr (N * N > 64) {
+    return false;
+  } else {
+    switch (variant) {
+      case 0: launch_variant<N, 1, false>(L, s); return true;
+      case 1: launch_variant<N, 2, false>(L, s); return true;
+      case 2: launch_variant<N, 4, false>(L, s); return true;
+      case 3: launch_variant<N, 1, true>(L, s); return true;
+      case 4: launch_variant<N, 2, true>(L, s); return true;
+      case 5: launch_variant<N, 4, true>(L, s); return true;
+    }
+    return false;
+  }
+}
+
+}  // namespace
+
 bool swe_launch_stage(const SweLaunch& L, hipStream_t s) {
-#define ATMO_SWE_CASE(NN)                                                                      \
-  case NN: {                                                                                   \
-    const int grid = (L.n_elements + SweShape<NN>::EPB - 1) / SweShape<NN>::EPB;               \
-    hipLaunchKernelGGL(swe_stage<NN>, dim3(grid), dim3(SweShape<NN>::BLOCK), 0, s, L);         \
-    return true;                                                                               \
-  }
+  const int variant = L.variant;
   switch (L.n) {
-    ATMO_SWE_CASE(2)
-    ATMO_SWE_CASE(3)
-    ATMO_SWE_CASE(4)
-    ATMO_SWE_CASE(5)
-    ATMO_SWE_CASE(6)
-    ATMO_SWE_CASE(7)
-    ATMO_SWE_CASE(8)
+    case 2: return launch_n<2>(L, variant, s);
+    case 3: return launch_n<3>(L, variant, s);
+    case 4: return launch_n<4>(L, variant, s);
+    case 5: return launch_n<5>(L, variant, s);
+    case 6: return launch_n<6>(L, variant, s);
+    case 7: return launch_n<7>(L, variant, s);
+    case 8: return launch_n<8>(L, variant, s);
   }
-#undef ATMO_SWE_CASE
   return false;
 }
 

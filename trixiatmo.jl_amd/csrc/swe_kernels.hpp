@@ -25,7 +25,9 @@ struct SweLaunch {
   double rk_a, rk_dt, rk_b;
   const double* dt_member;  // per-member time steps (NULL: rk_dt for all)
   const int* member;        // [E] member of each element
+  int variant;              // workgroup shape: 0..2 = 1, 2, 4 waves; +3 persistent grid
 };
+constexpr int kSweVariants = 6;
 
 bool swe_launch_stage(const SweLaunch& L, hipStream_t s);
 // per-member max over elements of the summed scaled wave speeds -> out[n_members]

@@ -133,6 +133,8 @@ SIGNATURES = [
      [ctypes.POINTER(AtmoSweConfig), ctypes.POINTER(AtmoGeometry2D), ctypes.POINTER(ctypes.c_void_p)]),
     ("atmo_max_dt_members", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, c_double_p]),
     ("atmo_step_members", ctypes.c_int, [ctypes.c_void_p, c_double_p, ctypes.c_int32]),
+    ("atmo_autotune", ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_double_p]),
+    ("atmo_set_kernel_variant", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("atmo_rainy_create", ctypes.c_int,
      [ctypes.POINTER(AtmoRainyConfig), ctypes.POINTER(ctypes.c_void_p)]),
     ("atmo_set_time", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),

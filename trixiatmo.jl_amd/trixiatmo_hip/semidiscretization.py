@@ -210,6 +210,18 @@ class SemidiscretizationHyperbolic:
     def synchronize(self):
         check(lib().atmo_synchronize(self._h), self._h)
 
+    def autotune(self):
+        """time every workgroup shape of the stage kernel on the resident
+        state, keep the fastest; returns (chosen shape, ms per shape)"""
+        v = ctypes.c_int32()
+        ms = np.zeros(6)
+        check(lib().atmo_autotune(self._h, ctypes.byref(v), dptr(ms)), self._h)
+        return v.value, ms
+
+    def set_kernel_variant(self, variant):
+        """pin the workgroup shape of the stage kernel"""
+        check(lib().atmo_set_kernel_variant(self._h, int(variant)), self._h)
+
     def enable_timing(self, on=True):
         check(lib().atmo_enable_timing(self._h, 1 if on else 0), self._h)
 
