@@ -168,6 +168,61 @@ __device__ __forceinline__ void flux_ec_rain(const RNode& l, const RNode& r, dou
   f[6] = f[7] = f[8] = 0.0;
 }
 
+// the per-node quantities flux_ec_rain reads, formed once per node
+struct ECNode {
+  double rd, rv, rc, rr, v1, v2, it, vr;  // it = 1 / T, vr from rho_vapour + rho_cloud
+};
+
+__device__ __forceinline__ void flux_ec_rain(const ECNode& l, const ECNode& r, double n1,
+                                             double n2, double* f) {
+  const double itl = l.it, itr = r.it;
+  const double v1a = 0.5 * (l.v1 + r.v1), v2a = 0.5 * (l.v2 + r.v2);
+  const double v1sq = 0.5 * (l.v1 * l.v1 + r.v1 * r.v1), v2sq = 0.5 * (l.v2 * l.v2 + r.v2 * r.v2);
+  const double K = 0.5 * (v1sq + v2sq);
+  const double vdn = n1 * v1a + n2 * v2a;
+  const double vrn = 0.5 * n2 * (l.vr + r.vr);
+  const double dry_a = 0.5 * (l.rd + r.rd);
+  const double vap_a = 0.5 * (l.rv + r.rv);
+  const double cld_a = 0.5 * (l.rc + r.rc);
+  const double rain_a = 0.5 * (l.rr + r.rr);
+  double dry_log = 0.0, vap_log = 0.0;
+  if (l.rd != 0.0 && r.rd != 0.0) dry_log = ln_mean(l.rd, r.rd);
+  if (l.rv != 0.0 && r.rv != 0.0) vap_log = ln_mean(l.rv, r.rv);
+  const double it_avg = 0.5 * (itl + itr);
+  const double it_log = inv_ln_mean(itl, itr);
+  const double p_int = (1.0 / it_avg) * (R_d * dry_a + R_v * vap_a);
+  const double f_vap = vap_log * vdn;
+  const double f_cld = cld_a * vdn;
+  const double f_dry = dry_log * vdn;
+  const double f_rain = rain_a * (vdn - vrn);
+  const double f_moist = f_vap + f_cld;
+  const double f_rho = f_dry + f_moist + f_rain;
+  const double f_m1 = f_rho * v1a + p_int * n1;
+  const double f_m2 = f_rho * v2a + p_int * n2;
+  f[0] = f_dry;
+  f[1] = f_moist;
+  f[2] = f_rain;
+  f[3] = f_m1;
+  f[4] = f_m2;
+  f[5] = (c_vd * it_log - K) * f_dry + (c_vv * it_log - K + L_ref) * f_vap +
+         (c_l * it_log - K) * (f_cld + f_rain) + (v1a * f_m1 + v2a * f_m2);
+}
+
+__device__ __forceinline__ ECNode ecnode(const double* u) {
+  ECNode a;
+  const double rho = u[0] + u[1] + u[2];
+  const double inv = 1.0 / rho;
+  double T;
+  nonlinear_sol(u, a.rv, a.rc, T);
+  a.rd = u[0];
+  a.rr = u[2];
+  a.v1 = u[3] * inv;
+  a.v2 = u[4] * inv;
+  a.it = 1.0 / T;
+  a.vr = terminal_velocity(a.rv + a.rc, a.rr);
+  return a;
+}
+
 // flux_lax_friedrichs with max_abs_speed_naive (:607-624) on all nine entries
 __device__ __forceinline__ void flux_llf(const RNode& l, const RNode& r, const double* ul,
                                          const double* ur, double n1, double n2, double* f) {
@@ -209,7 +264,7 @@ __device__ __forceinline__ void source_rainy(const double* u, double* s) {
 }
 
 // examples/euler/precipitation/tests/convergence_test.jl:45-139
-__device__ void source_convergence(double x, double y, double t, double* s) {
+__device__ __forceinline__ void source_convergence(double x, double y, double t, double* s) {
   const double c = 2, A = 0.1, Lw = 2, fr = 1 / Lw, w = 2 * kPi * fr;
   const double c_help = v_0 * 1.9386213994279082 * pow(kPi * N_0, -0.125);
   const double si = sin(w * (x + y - t)), co = cos(w * (x + y - t));
@@ -286,7 +341,23 @@ __device__ __forceinline__ void jacobian(const double* u, const double* x, doubl
   J[8] = 0.0;
 }
 
-// 3x3 Gaussian elimination with partial pivoting
+// 3x3 Gaussian elimination with partial pivoting (the first largest pivot of
+// each column, one row swap); rows are swapped with selects so the system
+// stays in registers
+__device__ __forceinline__ void swap_rows(double (&a)[3][4], int c, int p) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    if (r <= c) continue;
+    const bool sw = p == r;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const double tc = a[c][j], tr = a[r][j];
+      a[c][j] = sw ? tr : tc;
+      a[r][j] = sw ? tc : tr;
+    }
+  }
+}
+
 __device__ __forceinline__ void solve3(const double* J, const double* r, double* dx) {
   double a[3][4];
 #pragma unroll
@@ -298,37 +369,40 @@ __device__ __forceinline__ void solve3(const double* J, const double* r, double*
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     int p = c;
+    double best = fabs(a[c][c]);
 #pragma unroll
-    for (int i = c + 1; i < 3; ++i)
-      if (fabs(a[i][c]) > fabs(a[p][c])) p = i;
-    if (p != c) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const double tmp = a[c][j];
-        a[c][j] = a[p][j];
-        a[p][j] = tmp;
+    for (int i = 0; i < 3; ++i) {
+      if (i <= c) continue;
+      const double v = fabs(a[i][c]);
+      if (v > best) {
+        best = v;
+        p = i;
       }
     }
+    swap_rows(a, c, p);
 #pragma unroll
-    for (int i = c + 1; i < 3; ++i) {
+    for (int i = 0; i < 3; ++i) {
+      if (i <= c) continue;
       const double f = a[i][c] / a[c][c];
 #pragma unroll
-      for (int j = c; j < 4; ++j) a[i][j] -= f * a[c][j];
+      for (int j = 0; j < 4; ++j)
+        if (j >= c) a[i][j] -= f * a[c][j];
     }
   }
 #pragma unroll
   for (int i = 2; i >= 0; --i) {
-    double s = a[i][3];
+    double sacc = a[i][3];
 #pragma unroll
-    for (int j = i + 1; j < 3; ++j) s -= a[i][j] * dx[j];
-    dx[i] = s / a[i][i];
+    for (int j = 0; j < 3; ++j)
+      if (j > i) sacc -= a[i][j] * dx[j];
+    dx[i] = sacc / a[i][i];
   }
 }
 
 // one node of nonlinear_solve_dg_2d!: solve from guess x (updated in place),
 // the residual reading `un` as it was before the rain clamp; returns the
 // number of updates or -1 when max_iter updates did not reach the tolerance
-__device__ int newton_node(const double* un, double* x, double tol, int max_iter) {
+__device__ __forceinline__ int newton_node(const double* un, double* x, double tol, int max_iter) {
   for (int it = 1; it <= max_iter; ++it) {
     double r[3];
     residual(un, x, r);
@@ -432,14 +506,20 @@ __device__ __forceinline__ bool mlp_guess(const float* P, const double* u, doubl
   return x[2] > 0.0;
 }
 
-template <int N>
+template <int N, bool SEEDED>
 __global__ __launch_bounds__(RShape<N>::BLOCK) void rainy_stage(RainyLaunch L) {
   using S = RShape<N>;
   constexpr int N2 = S::N2, EPB = S::EPB, B = S::BLOCK;
+  constexpr int VOLW = 12 * N2 > NV * S::NF ? 12 * N2 : NV * S::NF;
   __shared__ double s_u[EPB][NV][N2];
-  __shared__ double s_f[EPB][2][6][N2];  // weak form: contravariant fluxes of the 6 flux rows
-  __shared__ double s_sf[EPB][NV][S::NF];
+  // one buffer per element, used in turn for the volume terms (weak form: the
+  // contravariant fluxes of the 6 flux rows [2][6][N2]; flux differencing: the
+  // per-node quantities of flux_ec_rain [8][N2]) and for the face fluxes [9][4 n]
+  __shared__ double s_buf[EPB][VOLW];
   extern __shared__ float s_mlp[];  // network parameters + normalisation when seeding
+  auto s_f = [&](int es, int o, int v, int q) -> double& { return s_buf[es][(o * 6 + v) * N2 + q]; };
+  auto s_ec = [&](int es, int k, int q) -> double& { return s_buf[es][k * N2 + q]; };
+  auto s_sf = [&](int es, int v, int r) -> double& { return s_buf[es][v * S::NF + r]; };
 
   const int t = threadIdx.x;
   const int e0 = blockIdx.x * EPB;
@@ -447,7 +527,7 @@ __global__ __launch_bounds__(RShape<N>::BLOCK) void rainy_stage(RainyLaunch L) {
     const int es = idx / (N2 * NV), r = idx - es * N2 * NV;
     if (e0 + es < L.n_elements) s_u[es][r % NV][r / NV] = L.u[(size_t)(e0 + es) * N2 * NV + r];
   }
-  const bool seeded = L.mlp.params != nullptr && L.mode == 1 && L.newton_max_iter > 0;
+  const bool seeded = SEEDED && L.mode == 1 && L.newton_max_iter > 0;
   if (seeded) {
     for (int i = t; i < kMlpParams; i += B) s_mlp[i] = L.mlp.params[i];
     if (t < 10) s_mlp[kMlpParams + t] = L.mlp.norm[t];
@@ -474,8 +554,8 @@ __global__ __launch_bounds__(RShape<N>::BLOCK) void rainy_stage(RainyLaunch L) {
       flux_o(a, 2, f2);
 #pragma unroll
       for (int v = 0; v < 6; ++v) {
-        s_f[es][0][v][q] = L.Ja11 * f1[v] + 0.0 * f2[v];
-        s_f[es][1][v][q] = 0.0 * f1[v] + L.Ja22 * f2[v];
+        s_f(es, 0, v, q) = L.Ja11 * f1[v] + 0.0 * f2[v];
+        s_f(es, 1, v, q) = 0.0 * f1[v] + L.Ja22 * f2[v];
       }
     }
     __syncthreads();
@@ -485,39 +565,51 @@ __global__ __launch_bounds__(RShape<N>::BLOCK) void rainy_stage(RainyLaunch L) {
         const double d1 = L.dhat[ti * N + m], d2 = L.dhat[tj * N + m];
 #pragma unroll
         for (int v = 0; v < 6; ++v)
-          du[v] += d1 * s_f[es][0][v][m + N * tj] + d2 * s_f[es][1][v][ti + N * m];
+          du[v] += d1 * s_f(es, 0, v, m + N * tj) + d2 * s_f(es, 1, v, ti + N * m);
       }
     }
-  } else if (active) {
+  } else {
     // flux differencing with flux_ec_rain, gathered per node; each pair is
     // evaluated with the lower node index on the left as the scatter form does
-    double uo[NV];
+    if (active) {
+      double uo[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) uo[v] = s_u[es][v][q];
-    const RNode a = rnode(uo);
+      for (int v = 0; v < NV; ++v) uo[v] = s_u[es][v][q];
+      const ECNode a = ecnode(uo);
+      s_ec(es, 0, q) = a.rd; s_ec(es, 1, q) = a.rv; s_ec(es, 2, q) = a.rc; s_ec(es, 3, q) = a.rr;
+      s_ec(es, 4, q) = a.v1; s_ec(es, 5, q) = a.v2; s_ec(es, 6, q) = a.it; s_ec(es, 7, q) = a.vr;
+    }
+    __syncthreads();
+    if (active) {
+      ECNode a;
+      a.rd = s_ec(es, 0, q); a.rv = s_ec(es, 1, q); a.rc = s_ec(es, 2, q); a.rr = s_ec(es, 3, q);
+      a.v1 = s_ec(es, 4, q); a.v2 = s_ec(es, 5, q); a.it = s_ec(es, 6, q); a.vr = s_ec(es, 7, q);
 #pragma unroll
-    for (int o = 0; o < 2; ++o) {
-      const int own = o == 0 ? ti : tj;
-      const double n1 = o == 0 ? L.Ja11 : 0.0, n2 = o == 0 ? 0.0 : L.Ja22;
+      for (int o = 0; o < 2; ++o) {
+        const int own = o == 0 ? ti : tj;
+        const double n1 = o == 0 ? L.Ja11 : 0.0, n2 = o == 0 ? 0.0 : L.Ja22;
 #pragma unroll
-      for (int m = 0; m < N; ++m) {
-        if (m == own) continue;
-        const int pq = o == 0 ? m + N * tj : ti + N * m;
-        double up[NV];
+        for (int m = 0; m < N; ++m) {
+          if (m == own) continue;
+          const int pq = o == 0 ? m + N * tj : ti + N * m;
+          ECNode b;
+          b.rd = s_ec(es, 0, pq); b.rv = s_ec(es, 1, pq); b.rc = s_ec(es, 2, pq);
+          b.rr = s_ec(es, 3, pq); b.v1 = s_ec(es, 4, pq); b.v2 = s_ec(es, 5, pq);
+          b.it = s_ec(es, 6, pq); b.vr = s_ec(es, 7, pq);
+          double f[6];
+          if (own < m) flux_ec_rain(a, b, n1, n2, f);
+          else flux_ec_rain(b, a, n1, n2, f);
+          const double ds = L.dsplit[own * N + m];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) up[v] = s_u[es][v][pq];
-        const RNode b = rnode(up);
-        double f[NV];
-        if (own < m) flux_ec_rain(a, b, n1, n2, f);
-        else flux_ec_rain(b, a, n1, n2, f);
-        const double ds = L.dsplit[own * N + m];
-#pragma unroll
-        for (int v = 0; v < 6; ++v) du[v] += ds * f[v];
+          for (int v = 0; v < 6; ++v) du[v] += ds * f[v];
+        }
       }
     }
   }
 
-  // surface fluxes, one lane per face node of the workgroup's elements
+  // surface fluxes, one lane per face node of the workgroup's elements; the
+  // buffer of the volume terms is free once every lane has gathered
+  __syncthreads();
   for (int fid = t; fid < EPB * S::NF; fid += B) {
     const int fe = fid / S::NF, r = fid - fe * S::NF;
     const int f = r / N, l = r - f * N;
@@ -566,7 +658,7 @@ __global__ __launch_bounds__(RShape<N>::BLOCK) void rainy_stage(RainyLaunch L) {
     surface_flux(L, ul, ur, n1, n2, fl);
     const double sgn = (f % 2 == 0) ? -L.inv_w0 : L.inv_wn;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) s_sf[fe][v][r] = sgn * fl[v];
+    for (int v = 0; v < NV; ++v) s_sf(fe, v, r) = sgn * fl[v];
   }
   __syncthreads();
   int it_stat = 0;
@@ -577,7 +669,7 @@ __global__ __launch_bounds__(RShape<N>::BLOCK) void rainy_stage(RainyLaunch L) {
       if (!on) continue;
       const int l = f < 2 ? tj : ti;
 #pragma unroll
-      for (int v = 0; v < NV; ++v) du[v] += s_sf[es][v][f * N + l];
+      for (int v = 0; v < NV; ++v) du[v] += s_sf(es, v, f * N + l);
     }
     double uo[NV];
 #pragma unroll
@@ -618,7 +710,7 @@ __global__ __launch_bounds__(RShape<N>::BLOCK) void rainy_stage(RainyLaunch L) {
       // the stage limiter: NonlinearSolveDG on the updated state
       if (L.newton_max_iter > 0) {
         double x[3] = {un[6], un[7], un[8]};
-        if (seeded) {
+        if constexpr (SEEDED) {
           // network seed; a failed seeded solve is redone from u[7:9]
           const bool ok = mlp_guess(s_mlp, un, x);
           it_stat = ok ? newton_node(un, x, L.newton_tol, L.newton_max_iter) : -1;
@@ -812,8 +904,11 @@ bool rainy_launch_stage(const RainyLaunch& L, hipStream_t s) {
 #define ATMO_CASE(NN)                                                                         \
   case NN: {                                                                                  \
     const int grid = (L.n_elements + RShape<NN>::EPB - 1) / RShape<NN>::EPB;                  \
-    const size_t lds = L.mlp.params ? sizeof(float) * (kMlpParams + 10) : 0;                 \
-    hipLaunchKernelGGL(rainy_stage<NN>, dim3(grid), dim3(RShape<NN>::BLOCK), lds, s, L);      \
+    if (L.mlp.params && L.mode == 1 && L.newton_max_iter > 0)                                 \
+      hipLaunchKernelGGL((rainy_stage<NN, true>), dim3(grid), dim3(RShape<NN>::BLOCK),          \
+                         sizeof(float) * (kMlpParams + 10), s, L);                              \
+    else                                                                                        \
+      hipLaunchKernelGGL((rainy_stage<NN, false>), dim3(grid), dim3(RShape<NN>::BLOCK), 0, s, L); \
     return true;                                                                              \
   }
   ATMO_RAINY_SWITCH(ATMO_CASE)
