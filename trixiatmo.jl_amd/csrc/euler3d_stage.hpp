@@ -295,13 +295,51 @@ __device__ __forceinline__ void surface_momentum_split(const E3Params& q, int su
   }
 }
 
+// face-node passes of one workgroup and the neighbour states they need,
+// loaded at the start of the kernel so that the dependent loads (link, then
+// the neighbour's state) overlap the volume phase
+template <int N>
+struct FacePrefetch {
+  static constexpr int PASSES = (6 * Shape<N>::N2 + Shape<N>::BLOCK - 1) / Shape<N>::BLOCK;
+  int link[PASSES], info[PASSES];
+  double ur[PASSES][6];
+};
+
+template <int N>
+__device__ __forceinline__ void face_prefetch(FacePrefetch<N>& pf, int e, int t,
+                                              const double* __restrict__ u,
+                                              const int* __restrict__ nbr,
+                                              const int* __restrict__ face_info) {
+  constexpr int N2 = Shape<N>::N2, NF = 6 * N2;
+#pragma unroll
+  for (int ps = 0; ps < FacePrefetch<N>::PASSES; ++ps) {
+    const int fid = t + ps * Shape<N>::BLOCK;
+    pf.link[ps] = -1;
+    pf.info[ps] = 0;
+    if (fid < NF) {
+      const int f = fid / N2, ab = fid - f * N2;
+      pf.link[ps] = nbr[((size_t)e * 6 + f) * N2 + ab];
+      pf.info[ps] = face_info[(size_t)e * 6 + f];
+    }
+  }
+#pragma unroll
+  for (int ps = 0; ps < FacePrefetch<N>::PASSES; ++ps) {
+    const int link = pf.link[ps];
+#pragma unroll
+    for (int v = 0; v < 6; ++v) pf.ur[ps][v] = link >= 0 ? u[(size_t)link * 6 + v] : 0.0;
+  }
+}
+
 template <int N, typename T>
 __device__ __forceinline__ void face_fluxes_to_lds(
     const E3Params& q, int e, int t, double inv_w, const double* __restrict__ u,
-    const T* __restrict__ cv, const int* __restrict__ nbr, const int* __restrict__ face_info,
-    int surf, int snc, const Background& bg, const double* s_pb, T* s_sf) {
+    const T* __restrict__ cv, const FacePrefetch<N>& pf, int surf, int snc,
+    const Background& bg, const double* s_pb, T* s_sf) {
   constexpr int N2 = Shape<N>::N2, N3 = Shape<N>::N3, NF = 6 * N2;
-  for (int fid = t; fid < NF; fid += Shape<N>::BLOCK) {
+#pragma unroll
+  for (int ps = 0; ps < FacePrefetch<N>::PASSES; ++ps) {
+    const int fid = t + ps * Shape<N>::BLOCK;
+    if (fid >= NF) continue;
     const int f = fid / N2, ab = fid - f * N2;
     const int a = ab % N, b = ab / N;
     const int d = f / 2, fixed = (f % 2 == 0) ? 0 : N - 1;
@@ -311,10 +349,10 @@ __device__ __forceinline__ void face_fluxes_to_lds(
     double uo[6];
 #pragma unroll
     for (int v = 0; v < 6; ++v) uo[v] = u[gn * 6 + v];
-    const int link = nbr[((size_t)e * 6 + f) * N2 + ab];
+    const int link = pf.link[ps];
     // face_info: 0 = primary side or boundary (own outward normal);
     // 8 | dp = secondary side of an interface whose primary face direction is dp
-    const int info = face_info[(size_t)e * 6 + f];
+    const int info = pf.info[ps];
     T n0, n1, n2;
     if (info == 0) {
       const T sgn = (f % 2 == 0) ? T(-1) : T(1);
@@ -333,7 +371,7 @@ __device__ __forceinline__ void face_fluxes_to_lds(
     double ur[6];
     if (link >= 0) {
 #pragma unroll
-      for (int v = 0; v < 6; ++v) ur[v] = u[(size_t)link * 6 + v];
+      for (int v = 0; v < 6; ++v) ur[v] = pf.ur[ps][v];
     } else {
       // boundary_condition_slip_wall (:93-116): mirror the normal momentum
       const double m0 = n0, m1 = n1, m2 = n2;
@@ -471,6 +509,9 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
     }
   }
 
+  FacePrefetch<N> pf;
+  face_prefetch<N>(pf, e, t, u, nbr, face_info);
+
   Acc<T, KAHAN> acc[5], acc_nc[4];
   for (int d = 0; d < 3; ++d) {
     __syncthreads();
@@ -552,7 +593,7 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
     }
   }
   __syncthreads();  // pair terms consumed; the buffer now takes the face fluxes
-  face_fluxes_to_lds<N, T>(q, e, t, inv_w, u, cv, nbr, face_info, surf, snc, bg, s_pb, s_fl);
+  face_fluxes_to_lds<N, T>(q, e, t, inv_w, u, cv, pf, surf, snc, bg, s_pb, s_fl);
   __syncthreads();
   if (!active) return;
 
