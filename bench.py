@@ -30,12 +30,12 @@ METRIC = "GDOF\u00b7RHS/s, mixed vs FP64 on 1 GPU at stated rel. err; % of VALU 
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch); FETCH_SIZE
 # reads half the bytes of a plain device copy of known size on this GPU, hence
 # the factor 2 (profiles/e3_stage_shell16x8_p4_pmc.txt).
-STAGE_FETCH_KIB, STAGE_WRITE_KIB, FETCH_FACTOR = 1.621e5, 1.363e5, 2.0
+STAGE_FETCH_KIB, STAGE_WRITE_KIB, FETCH_FACTOR = 1.822e5, 1.321e5, 2.0
 # Executed VALU flops per node and stage (SQ_INSTS_VALU_{ADD,MUL,TRANS}_F64 x 64
 # + SQ_INSTS_VALU_FMA_F64 x 128 per dispatch / nodes, same profile file); the
 # mixed form executes mostly FP32.
-EXEC_FP64_FLOPS_PER_NODE = 1477.0
-EXEC_MIXED_FLOPS_PER_NODE = {"fp32": 1753.0, "fp64": 354.0}
+EXEC_FP64_FLOPS_PER_NODE = 1556.0
+EXEC_MIXED_FLOPS_PER_NODE = {"fp32": 1876.0, "fp64": 352.0}
 
 
 def log(msg):
