@@ -47,9 +47,10 @@ __device__ __forceinline__ T inv_ln_mean(T x, T y) {
 }
 
 // primitive quantities a two-point flux needs from one node
+// (e = rho_e / rho, inv_rho = 1 / rho)
 template <typename T>
 struct Prim {
-  T rho, v1, v2, v3, p, rho_e, phi;
+  T rho, v1, v2, v3, p, rho_e, phi, e, inv_rho;
 };
 
 template <typename T>
@@ -63,6 +64,8 @@ __device__ __forceinline__ Prim<T> cons2prim(const E3Params& q, const T* u) {
   w.p = T(q.gm1) * (u[4] - T(0.5) * (u[1] * w.v1 + u[2] * w.v2 + u[3] * w.v3));
   w.rho_e = u[4];
   w.phi = u[5];
+  w.e = u[4] * inv;
+  w.inv_rho = inv;
   return w;
 }
 
@@ -92,7 +95,7 @@ __device__ __forceinline__ void flux_kennedy_gruber(const Prim<T>& l, const Prim
   const T ra = T(0.5) * (l.rho + r.rho);
   const T v1a = T(0.5) * (l.v1 + r.v1), v2a = T(0.5) * (l.v2 + r.v2), v3a = T(0.5) * (l.v3 + r.v3);
   const T pa = T(0.5) * (l.p + r.p);
-  const T ea = T(0.5) * (div_(l.rho_e, l.rho) + div_(r.rho_e, r.rho));
+  const T ea = T(0.5) * (l.e + r.e);
   const T vna = v1a * n0 + v2a * n1 + v3a * n2;
   const T f1 = ra * vna;
   f[0] = f1;
@@ -151,8 +154,8 @@ __device__ __forceinline__ T max_abs_speed(const E3Params& q, const Prim<T>& l, 
                                            const T n0, const T n1, const T n2) {
   const T vl = l.v1 * n0 + l.v2 * n1 + l.v3 * n2;
   const T vr = r.v1 * n0 + r.v2 * n1 + r.v3 * n2;
-  const T cl = sqrt_(div_(T(q.gamma) * l.p, l.rho));
-  const T cr = sqrt_(div_(T(q.gamma) * r.p, r.rho));
+  const T cl = sqrt_(T(q.gamma) * l.p * l.inv_rho);
+  const T cr = sqrt_(T(q.gamma) * r.p * r.inv_rho);
   const T nn = sqrt_(n0 * n0 + n1 * n1 + n2 * n2);
   return fmax(fabs(vl) + cl * nn, fabs(vr) + cr * nn);
 }

@@ -247,6 +247,7 @@ __device__ __forceinline__ Prim<T> to_t(const Prim<double>& w) {
   Prim<T> o;
   o.rho = T(w.rho); o.v1 = T(w.v1); o.v2 = T(w.v2); o.v3 = T(w.v3);
   o.p = T(w.p); o.rho_e = T(w.rho_e); o.phi = T(w.phi);
+  o.e = T(w.e); o.inv_rho = T(w.inv_rho);
   return o;
 }
 
@@ -330,44 +331,90 @@ __device__ __forceinline__ void face_prefetch(FacePrefetch<N>& pf, int e, int t,
   }
 }
 
+// face-node geometry of pass ps: face f, direction d, own node; false when the
+// lane has no face node in that pass
+template <int N>
+__device__ __forceinline__ bool face_node_of(int t, int ps, int& fid, int& f, int& d, int& node) {
+  constexpr int N2 = Shape<N>::N2, NF = 6 * N2;
+  fid = t + ps * Shape<N>::BLOCK;
+  if (fid >= NF) return false;
+  f = fid / N2;
+  const int ab = fid - f * N2;
+  const int a = ab % N, b = ab / N;
+  d = f / 2;
+  const int fixed = (f % 2 == 0) ? 0 : N - 1;
+  node = d == 0 ? fixed + N * (a + N * b) : (d == 1 ? a + N * (fixed + N * b) : a + N * (b + N * fixed));
+  return true;
+}
+
+// normal of every face pass: the own outward contravariant vector on a primary
+// or boundary face, minus the primary's on a secondary face (Trixi evaluates
+// an interface with the primary element's normal).  Read before the face
+// fluxes overwrite the LDS copy of the own vectors (s_cv = [9][n^3], or null
+// to read them from memory).
 template <int N, typename T>
-__device__ __forceinline__ void face_fluxes_to_lds(
-    const E3Params& q, int e, int t, double inv_w, const double* __restrict__ u,
-    const T* __restrict__ cv, const FacePrefetch<N>& pf, int surf, int snc,
-    const Background& bg, const double* s_pb, T* s_sf) {
-  constexpr int N2 = Shape<N>::N2, N3 = Shape<N>::N3, NF = 6 * N2;
+struct FaceNormals {
+  T n[FacePrefetch<N>::PASSES][3];
+};
+
+template <int N, typename T>
+__device__ __forceinline__ void face_normals(FaceNormals<N, T>& fn, int e, int t,
+                                             const T* __restrict__ cv, const T* s_cv,
+                                             const FacePrefetch<N>& pf) {
+  constexpr int N3 = Shape<N>::N3;
 #pragma unroll
   for (int ps = 0; ps < FacePrefetch<N>::PASSES; ++ps) {
-    const int fid = t + ps * Shape<N>::BLOCK;
-    if (fid >= NF) continue;
-    const int f = fid / N2, ab = fid - f * N2;
-    const int a = ab % N, b = ab / N;
-    const int d = f / 2, fixed = (f % 2 == 0) ? 0 : N - 1;
-    const int node = d == 0 ? fixed + N * (a + N * b)
-                            : (d == 1 ? a + N * (fixed + N * b) : a + N * (b + N * fixed));
-    const size_t gn = (size_t)e * N3 + node;
-    double uo[6];
-#pragma unroll
-    for (int v = 0; v < 6; ++v) uo[v] = u[gn * 6 + v];
-    const int link = pf.link[ps];
-    // face_info: 0 = primary side or boundary (own outward normal);
-    // 8 | dp = secondary side of an interface whose primary face direction is dp
+    int fid, f, d, node;
+    fn.n[ps][0] = fn.n[ps][1] = fn.n[ps][2] = T(0);
+    if (!face_node_of<N>(t, ps, fid, f, d, node)) continue;
+    // face_info: 0 = primary side or boundary; 8 | dp = secondary side of an
+    // interface whose primary face direction is dp
     const int info = pf.info[ps];
-    T n0, n1, n2;
     if (info == 0) {
       const T sgn = (f % 2 == 0) ? T(-1) : T(1);
-      const T* jo = cv + gn * 9 + 3 * d;
-      n0 = sgn * jo[0];
-      n1 = sgn * jo[1];
-      n2 = sgn * jo[2];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        fn.n[ps][c] = sgn * (s_cv ? s_cv[(3 * d + c) * N3 + node]
+                                  : cv[((size_t)e * N3 + node) * 9 + 3 * d + c]);
     } else {
       const int dp = info & 7;
       const T sgn = (dp % 2 == 0) ? T(1) : T(-1);  // minus the primary's outward normal
-      const T* jp = cv + (size_t)link * 9 + 3 * (dp / 2);
-      n0 = sgn * jp[0];
-      n1 = sgn * jp[1];
-      n2 = sgn * jp[2];
+      const T* jp = cv + (size_t)pf.link[ps] * 9 + 3 * (dp / 2);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) fn.n[ps][c] = sgn * jp[c];
     }
+  }
+}
+
+template <int N, typename T>
+__device__ __forceinline__ void face_fluxes_to_lds(
+    const E3Params& q, int e, int t, double inv_w, const double* __restrict__ u,
+    const FacePrefetch<N>& pf, const FaceNormals<N, T>& fn, int surf, int snc,
+    const Background& bg, const T* s_w, const double* s_pb, T* s_sf) {
+  constexpr int N2 = Shape<N>::N2, N3 = Shape<N>::N3, NF = 6 * N2;
+#pragma unroll
+  for (int ps = 0; ps < FacePrefetch<N>::PASSES; ++ps) {
+    int fid, f, d, node;
+    if (!face_node_of<N>(t, ps, fid, f, d, node)) continue;
+    double uo[6];
+    if constexpr (sizeof(T) == 8) {
+      // the own state from the primitives in LDS (rounding-level differences
+      // only; it enters the dissipation and the wall mirror)
+      const double r = s_w[0 * N3 + node];
+      uo[0] = r;
+      uo[1] = r * s_w[1 * N3 + node];
+      uo[2] = r * s_w[2 * N3 + node];
+      uo[3] = r * s_w[3 * N3 + node];
+      uo[4] = r * s_w[5 * N3 + node];
+      uo[5] = s_w[6 * N3 + node];
+    } else {
+      // the mixed path forms p' = p - p_bg from the exact FP64 state
+      const size_t gn = (size_t)e * N3 + node;
+#pragma unroll
+      for (int v = 0; v < 6; ++v) uo[v] = u[gn * 6 + v];
+    }
+    const int link = pf.link[ps];
+    const T n0 = fn.n[ps][0], n1 = fn.n[ps][1], n2 = fn.n[ps][2];
     double ur[6];
     if (link >= 0) {
 #pragma unroll
@@ -385,7 +432,13 @@ __device__ __forceinline__ void face_fluxes_to_lds(
       ur[4] = uo[4];
       ur[5] = uo[5];
     }
-    const Prim<T> own = to_t<T>(cons2prim(q, uo));
+    // own primitives as the prologue formed them (rho, v, p, e per node in LDS)
+    Prim<T> own;
+    own.rho = s_w[0 * N3 + node]; own.v1 = s_w[1 * N3 + node]; own.v2 = s_w[2 * N3 + node];
+    own.v3 = s_w[3 * N3 + node]; own.p = s_w[4 * N3 + node]; own.e = s_w[5 * N3 + node];
+    own.rho_e = T(uo[4]);
+    own.phi = T(uo[5]);
+    own.inv_rho = div_(T(1), own.rho);
     const Prim<T> pr = to_t<T>(cons2prim(q, ur));
     T ut[6], urt[6];
 #pragma unroll
@@ -444,6 +497,120 @@ __device__ __forceinline__ int xcd_element(int b, int n_elements) {
   return (b & 7) * per + (b >> 3);
 }
 
+// per-node fields of the two-point fluxes, formed in FP64 from the node state:
+// rho v1 v2 v3 p e phi [log(rho)] [log(rho / p)] [4 of the background split]
+template <int VOL, int NC, typename T>
+struct NodeFields {
+  static constexpr bool MIXED = sizeof(T) == 4;
+  static constexpr int NW = MIXED ? 13 : (VOL == 0 ? 9 : 8);
+};
+
+template <int N, int VOL, int NC, typename T>
+__device__ __forceinline__ void node_fields_to_lds(const E3Params& q, const double* __restrict__ u,
+                                                   size_t gnode, int t, T* s_w, double* s_phi,
+                                                   double* s_pb, const Background& bg) {
+  constexpr int N3 = Shape<N>::N3;
+  constexpr bool MIXED = NodeFields<VOL, NC, T>::MIXED;
+  double uo[6];
+#pragma unroll
+  for (int v = 0; v < 6; ++v) uo[v] = u[gnode * 6 + v];
+  // primitives (pressure from total minus kinetic energy) in FP64
+  const Prim<double> w = cons2prim(q, uo);
+  s_w[0 * N3 + t] = T(w.rho); s_w[1 * N3 + t] = T(w.v1); s_w[2 * N3 + t] = T(w.v2);
+  s_w[3 * N3 + t] = T(w.v3); s_w[4 * N3 + t] = T(w.p);
+  s_w[5 * N3 + t] = T(w.rho_e / w.rho);
+  s_w[6 * N3 + t] = T(w.phi);
+  if constexpr (VOL == 0 || NC == 1) s_w[7 * N3 + t] = log_(T(w.rho));
+  if constexpr (VOL == 0) s_w[8 * N3 + t] = log_(T(w.rho / w.p));
+  if constexpr (MIXED) {
+    s_phi[t] = w.phi;
+    if (bg.on) {
+      // hydrostatic background in FP64; FP32 carries only the perturbation
+      const double pb = bg.p0 * exp(-w.phi * bg.inv_RT0);
+      const double rb = pb * bg.inv_RT0;
+      s_pb[t] = pb;
+      s_w[9 * N3 + t] = T(w.p - pb);
+      s_w[10 * N3 + t] = T(rb);
+      s_w[11 * N3 + t] = T(w.rho - rb);
+      s_w[12 * N3 + t] = log1pf(float((w.rho - rb) / rb));  // log(rho / rho_bg)
+    }
+  }
+}
+
+template <int VOL, int NC, int N3, typename T>
+__device__ __forceinline__ FastPrim<T> load_fast_prim(const T* s_w, int node, const Background& bg) {
+  FastPrim<T> l;
+  l.rho = s_w[0 * N3 + node]; l.v1 = s_w[1 * N3 + node]; l.v2 = s_w[2 * N3 + node];
+  l.v3 = s_w[3 * N3 + node]; l.p = s_w[4 * N3 + node]; l.e = s_w[5 * N3 + node];
+  l.phi = s_w[6 * N3 + node];
+  if constexpr (VOL == 0 || NC == 1) l.lrho = s_w[7 * N3 + node];
+  if constexpr (VOL == 0) l.lrp = s_w[8 * N3 + node];
+  if constexpr (sizeof(T) == 4) {
+    if (bg.on) {
+      l.pp = s_w[9 * N3 + node]; l.rb = s_w[10 * N3 + node];
+      l.rp = s_w[11 * N3 + node]; l.lam = s_w[12 * N3 + node];
+    }
+  }
+  return l;
+}
+
+// after the volume terms (acc, acc_nc) and the lifted face fluxes (s_sf) are
+// complete: surface integral, -1/J, sources, then du or the RK update
+template <int N, typename T, bool KAHAN>
+__device__ __forceinline__ void stage_epilogue(
+    const Acc<T, KAHAN>* acc, const Acc<T, KAHAN>* acc_nc, const T* s_sf, int ti, int tj, int tk,
+    size_t gnode, const double* __restrict__ u, double* __restrict__ out,
+    double* __restrict__ kreg, const double* __restrict__ invJ, int mode, double rk_a,
+    double rk_dt, double rk_b, double coriolis, const Background& bg) {
+  constexpr int NV = 6, N2 = Shape<N>::N2;
+  double du[NV];
+  du[0] = double(acc[0].value());
+#pragma unroll
+  for (int v = 1; v < 5; ++v)
+    du[v] = double(acc[v].value()) + 0.5 * double(acc_nc[v - 1].value());
+  du[5] = 0.0;
+  constexpr int NF = 6 * N2;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const int d = f / 2;
+    const int od = d == 0 ? ti : (d == 1 ? tj : tk);
+    const bool on_face = (f % 2 == 0) ? (od == 0) : (od == N - 1);
+    if (!on_face) continue;
+    const int a = d == 0 ? tj : ti;
+    const int b = d == 2 ? tj : tk;
+    const int fid = f * N2 + a + N * b;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) du[v] += double(s_sf[v * NF + fid]);
+  }
+  const double fac = -invJ[gnode];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) du[v] *= fac;
+  double uo[NV];  // read again rather than kept live through the volume phase
+#pragma unroll
+  for (int v = 0; v < NV; ++v) uo[v] = u[gnode * NV + v];
+  if (bg.on) {  // tendency of the hydrostatic background, computed in FP64
+    du[1] += double(bg.B[gnode * 3 + 0]);
+    du[2] += double(bg.B[gnode * 3 + 1]);
+    du[3] += double(bg.B[gnode * 3 + 2]);
+  }
+  // calc_sources!: source_terms_coriolis (-2 Omega e_z x rho v), after the Jacobian
+  du[1] += coriolis * uo[2];
+  du[2] += -coriolis * uo[1];
+  if (mode == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) out[gnode * NV + v] = du[v];
+  } else {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double kn = rk_a * kreg[gnode * NV + v] + rk_dt * du[v];
+      kreg[gnode * NV + v] = kn;
+      out[gnode * NV + v] = uo[v] + rk_b * kn;
+    }
+  }
+}
+
+// Pair-record form: per direction the unordered pairs of every line are
+// evaluated once into LDS records, then gathered by both nodes.
 template <int N, int VOL, int NC, typename T, bool KAHAN>
 __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
     E3Params q, DsplitTable dst, double inv_w, int n_elements, const double* __restrict__ u,
@@ -454,11 +621,9 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
   using S = Shape<N>;
   using C = Chunking<N, T, NC>;
   constexpr int N2 = S::N2, N3 = S::N3, P = S::P, B = S::BLOCK;
-  constexpr int NV = 6;
   constexpr bool MIXED = sizeof(T) == 4;
-  // per-node fields: rho v1 v2 v3 p e phi log(rho) [log(rho/p)] [4 of the background split]
-  constexpr int NW = MIXED ? 13 : (VOL == 0 ? 9 : 8);
-  __shared__ T s_w[NW][N3];
+  constexpr int NW = NodeFields<VOL, NC, T>::NW;
+  __shared__ T s_w[NW * N3];
   __shared__ double s_phi[MIXED ? N3 : 1];  // FP64 phi for the jumps of the mixed path
   __shared__ double s_pb[MIXED ? N3 : 1];   // FP64 background pressure
   __shared__ T s_ja[3][N3];
@@ -481,33 +646,7 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
     s_pa[t] = (unsigned char)a;
     s_pm[t] = (unsigned char)(a + 1 + qq);
   }
-
-  if (active) {
-    double uo[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) uo[v] = u[gnode * NV + v];
-    // primitives (pressure from total minus kinetic energy) in FP64
-    const Prim<double> w = cons2prim(q, uo);
-    s_w[0][t] = T(w.rho); s_w[1][t] = T(w.v1); s_w[2][t] = T(w.v2); s_w[3][t] = T(w.v3);
-    s_w[4][t] = T(w.p);
-    s_w[5][t] = T(w.rho_e / w.rho);
-    s_w[6][t] = T(w.phi);
-    if constexpr (VOL == 0 || NC == 1) s_w[7][t] = log_(T(w.rho));
-    if constexpr (VOL == 0) s_w[8][t] = log_(T(w.rho / w.p));
-    if constexpr (MIXED) {
-      s_phi[t] = w.phi;
-      if (bg.on) {
-        // hydrostatic background in FP64; FP32 carries only the perturbation
-        const double pb = bg.p0 * exp(-w.phi * bg.inv_RT0);
-        const double rb = pb * bg.inv_RT0;
-        s_pb[t] = pb;
-        s_w[9][t] = T(w.p - pb);
-        s_w[10][t] = T(rb);
-        s_w[11][t] = T(w.rho - rb);
-        s_w[12][t] = log1pf(float((w.rho - rb) / rb));  // log(rho / rho_bg)
-      }
-    }
-  }
+  if (active) node_fields_to_lds<N, VOL, NC, T>(q, u, gnode, t, s_w, s_phi, s_pb, bg);
 
   FacePrefetch<N> pf;
   face_prefetch<N>(pf, e, t, u, nbr, face_info);
@@ -534,23 +673,11 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
         const int a = s_pa[slot], m = s_pm[slot];
         const int base = d == 0 ? N * line : (d == 1 ? (line % N) + N2 * (line / N) : line);
         const int na = base + a * stride, nm = base + m * stride;
-        FastPrim<T> l, r;
-        l.rho = s_w[0][na]; l.v1 = s_w[1][na]; l.v2 = s_w[2][na]; l.v3 = s_w[3][na];
-        l.p = s_w[4][na]; l.e = s_w[5][na]; l.phi = s_w[6][na]; l.lrho = s_w[7][na];
-        if constexpr (VOL == 0) l.lrp = s_w[8][na];
-        r.rho = s_w[0][nm]; r.v1 = s_w[1][nm]; r.v2 = s_w[2][nm]; r.v3 = s_w[3][nm];
-        r.p = s_w[4][nm]; r.e = s_w[5][nm]; r.phi = s_w[6][nm]; r.lrho = s_w[7][nm];
-        if constexpr (VOL == 0) r.lrp = s_w[8][nm];
+        const FastPrim<T> l = load_fast_prim<VOL, NC, N3, T>(s_w, na, bg);
+        const FastPrim<T> r = load_fast_prim<VOL, NC, N3, T>(s_w, nm, bg);
         T dphi;
-        if constexpr (MIXED) {
-          dphi = T(s_phi[nm] - s_phi[na]);
-          if (bg.on) {
-            l.pp = s_w[9][na]; l.rb = s_w[10][na]; l.rp = s_w[11][na]; l.lam = s_w[12][na];
-            r.pp = s_w[9][nm]; r.rb = s_w[10][nm]; r.rp = s_w[11][nm]; r.lam = s_w[12][nm];
-          }
-        } else {
-          dphi = r.phi - l.phi;
-        }
+        if constexpr (MIXED) dphi = T(s_phi[nm] - s_phi[na]);
+        else dphi = r.phi - l.phi;
         const T n0 = T(0.5) * (s_ja[0][na] + s_ja[0][nm]);
         const T n1 = T(0.5) * (s_ja[1][na] + s_ja[1][nm]);
         const T n2 = T(0.5) * (s_ja[2][na] + s_ja[2][nm]);
@@ -592,55 +719,111 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
       }
     }
   }
+  FaceNormals<N, T> fn;
+  face_normals<N, T>(fn, e, t, cv, nullptr, pf);
   __syncthreads();  // pair terms consumed; the buffer now takes the face fluxes
-  face_fluxes_to_lds<N, T>(q, e, t, inv_w, u, cv, pf, surf, snc, bg, s_pb, s_fl);
+  face_fluxes_to_lds<N, T>(q, e, t, inv_w, u, pf, fn, surf, snc, bg, s_w, s_pb, s_fl);
   __syncthreads();
   if (!active) return;
+  stage_epilogue<N, T, KAHAN>(acc, acc_nc, s_fl, ti, tj, tk, gnode, u, out, kreg, invJ, mode,
+                              rk_a, rk_dt, rk_b, coriolis, bg);
+}
 
-  double du[NV];
-  du[0] = double(acc[0].value());
+// Direct form: every node evaluates the two-point flux with each of its n-1
+// partners on each line itself (each unordered pair is evaluated by both of
+// its nodes).  The fluxes are symmetric and the gravity terms antisymmetric
+// under exchanging the two states bit for bit, so the result equals the
+// pair-record form's exactly; in exchange for twice the flux arithmetic the
+// volume phase needs no pair buffer and no barrier, which leaves LDS for more
+// resident workgroups.
+template <int N, int VOL, int NC, typename T, bool KAHAN>
+__global__ __launch_bounds__(Shape<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+void e3_stage_direct(
+    E3Params q, DsplitTable dst, double inv_w, int n_elements, const double* __restrict__ u,
+    double* __restrict__ out, double* __restrict__ kreg, const T* __restrict__ cv,
+    const double* __restrict__ invJ, const int* __restrict__ nbr,
+    const int* __restrict__ face_info, int surf, int snc, int mode, double rk_a, double rk_dt,
+    double rk_b, double coriolis, Background bg) {
+  using S = Shape<N>;
+  constexpr int N2 = S::N2, N3 = S::N3;
+  constexpr bool MIXED = sizeof(T) == 4;
+  constexpr int NW = NodeFields<VOL, NC, T>::NW;
+  constexpr int BUF = 9 * N3 > 36 * N2 ? 9 * N3 : 36 * N2;
+  __shared__ T s_w[NW * N3];
+  __shared__ double s_phi[MIXED ? N3 : 1];
+  __shared__ double s_pb[MIXED ? N3 : 1];
+  __shared__ T s_buf[BUF];  // contravariant vectors [9][n^3], later face fluxes [6][6 n^2]
+  __shared__ T s_ds[N * N];
+
+  if constexpr (!MIXED) bg.on = 0;
+  const int e = xcd_element(blockIdx.x, n_elements);
+  if (e >= n_elements) return;
+  const int t = threadIdx.x;
+  const bool active = t < N3;
+  const int ti = t % N, tj = (t / N) % N, tk = t / N2;
+  const size_t gnode = (size_t)e * N3 + t;
+  if (t < N * N) s_ds[t] = T(dst.ds[t]);
+  if (active) {
+    node_fields_to_lds<N, VOL, NC, T>(q, u, gnode, t, s_w, s_phi, s_pb, bg);
 #pragma unroll
-  for (int v = 1; v < 5; ++v)
-    du[v] = double(acc[v].value()) + 0.5 * double(acc_nc[v - 1].value());
-  du[5] = 0.0;
-  constexpr int NF = 6 * N2;
-#pragma unroll
-  for (int f = 0; f < 6; ++f) {
-    const int d = f / 2;
-    const int od = d == 0 ? ti : (d == 1 ? tj : tk);
-    const bool on_face = (f % 2 == 0) ? (od == 0) : (od == N - 1);
-    if (!on_face) continue;
-    const int a = d == 0 ? tj : ti;
-    const int b = d == 2 ? tj : tk;
-    const int fid = f * N2 + a + N * b;
-#pragma unroll
-    for (int v = 0; v < NV; ++v) du[v] += double(s_fl[v * NF + fid]);
+    for (int c = 0; c < 9; ++c) s_buf[c * N3 + t] = cv[gnode * 9 + c];
   }
-  const double fac = -invJ[gnode];
+  FacePrefetch<N> pf;
+  face_prefetch<N>(pf, e, t, u, nbr, face_info);
+  __syncthreads();
+
+  Acc<T, KAHAN> acc[5], acc_nc[4];
+  if (active) {
+    const FastPrim<T> l = load_fast_prim<VOL, NC, N3, T>(s_w, t, bg);
+    double phi_own = 0.0;
+    if constexpr (MIXED) phi_own = s_phi[t];
+    for (int d = 0; d < 3; ++d) {
+      const int stride = d == 0 ? 1 : (d == 1 ? N : N2);
+      const int own_d = d == 0 ? ti : (d == 1 ? tj : tk);
+      const T* ja = s_buf + 3 * d * N3;
+      const T jo0 = ja[t], jo1 = ja[N3 + t], jo2 = ja[2 * N3 + t];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) du[v] *= fac;
-  double uo[NV];  // read again rather than kept live through the pair phase
+      for (int m = 0; m < N; ++m) {
+        if (m == own_d) continue;
+        const int pn = t + (m - own_d) * stride;  // the partner node
+        const FastPrim<T> r = load_fast_prim<VOL, NC, N3, T>(s_w, pn, bg);
+        T dphi;
+        if constexpr (MIXED) dphi = T(s_phi[pn] - phi_own);
+        else dphi = r.phi - l.phi;
+        const T n0 = T(0.5) * (jo0 + ja[pn]);
+        const T n1 = T(0.5) * (jo1 + ja[N3 + pn]);
+        const T n2 = T(0.5) * (jo2 + ja[2 * N3 + pn]);
+        T f[5], g[5];
+        pair_flux<VOL, NC, T>(q, l, r, n0, n1, n2, dphi, bg, f, g);
+        const T ds = s_ds[own_d * N + m];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) uo[v] = u[gnode * NV + v];
-  if (bg.on) {  // tendency of the hydrostatic background, computed in FP64
-    du[1] += double(bg.B[gnode * 3 + 0]);
-    du[2] += double(bg.B[gnode * 3 + 1]);
-    du[3] += double(bg.B[gnode * 3 + 2]);
-  }
-  // calc_sources!: source_terms_coriolis (-2 Omega e_z x rho v), after the Jacobian
-  du[1] += coriolis * uo[2];
-  du[2] += -coriolis * uo[1];
-  if (mode == 0) {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) out[gnode * NV + v] = du[v];
-  } else {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      const double kn = rk_a * kreg[gnode * NV + v] + rk_dt * du[v];
-      kreg[gnode * NV + v] = kn;
-      out[gnode * NV + v] = uo[v] + rk_b * kn;
+        for (int v = 0; v < 5; ++v) acc[v].add(ds * f[v]);
+        if constexpr (NC != 0) {
+          acc_nc[0].add(ds * g[0]);
+          acc_nc[1].add(ds * g[1]);
+          acc_nc[2].add(ds * g[2]);
+          acc_nc[3].add(ds * g[3]);
+        }
+      }
     }
   }
+  FaceNormals<N, T> fn;
+  face_normals<N, T>(fn, e, t, cv, s_buf, pf);
+  __syncthreads();  // contravariant vectors consumed; the buffer takes the face fluxes
+  face_fluxes_to_lds<N, T>(q, e, t, inv_w, u, pf, fn, surf, snc, bg, s_w, s_pb, s_buf);
+  __syncthreads();
+  if (!active) return;
+  stage_epilogue<N, T, KAHAN>(acc, acc_nc, s_buf, ti, tj, tk, gnode, u, out, kreg, invJ, mode,
+                              rk_a, rk_dt, rk_b, coriolis, bg);
+}
+
+// which volume form a shape runs, from timings on MI355X: the direct form for
+// FP64 at every polydeg and for FP32 from polydeg 5 on (polydeg 7, EC: 1.8x
+// faster than the pair form); the pair form for FP32 up to polydeg 4, whose
+// background-split flux is costly enough that evaluating it twice loses
+template <int N, int VOL, typename T>
+constexpr bool use_direct_form() {
+  return sizeof(T) == 8 || N >= 6;
 }
 
 template <int N, int VOL, int NC, typename T, bool KAHAN>
@@ -648,11 +831,18 @@ void launch_stage_inst(const E3Launch& L, hipStream_t s) {
   DsplitTable dst;
   for (int q = 0; q < N * N; ++q) dst.ds[q] = L.dsplit[q];
   const int grid = 8 * ((L.n_elements + 7) / 8);
-  hipLaunchKernelGGL((e3_stage<N, VOL, NC, T, KAHAN>), dim3(grid), dim3(Shape<N>::BLOCK), 0, s,
-                     L.q, dst, L.inv_w, L.n_elements, L.u, L.out, L.k,
-                     reinterpret_cast<const T*>(L.cv_t), L.invJ, L.nbr, L.face_info,
-                     L.surface_flux, L.surface_noncons, L.mode, L.rk_a, L.rk_dt, L.rk_b,
-                     L.coriolis, L.bg);
+  if constexpr (use_direct_form<N, VOL, T>())
+    hipLaunchKernelGGL((e3_stage_direct<N, VOL, NC, T, KAHAN>), dim3(grid),
+                       dim3(Shape<N>::BLOCK), 0, s, L.q, dst, L.inv_w, L.n_elements, L.u, L.out,
+                       L.k, reinterpret_cast<const T*>(L.cv_t), L.invJ, L.nbr, L.face_info,
+                       L.surface_flux, L.surface_noncons, L.mode, L.rk_a, L.rk_dt, L.rk_b,
+                       L.coriolis, L.bg);
+  else
+    hipLaunchKernelGGL((e3_stage<N, VOL, NC, T, KAHAN>), dim3(grid), dim3(Shape<N>::BLOCK), 0, s,
+                       L.q, dst, L.inv_w, L.n_elements, L.u, L.out, L.k,
+                       reinterpret_cast<const T*>(L.cv_t), L.invJ, L.nbr, L.face_info,
+                       L.surface_flux, L.surface_noncons, L.mode, L.rk_a, L.rk_dt, L.rk_b,
+                       L.coriolis, L.bg);
 }
 
 template <typename T, bool KAHAN, int N, int VOL>
