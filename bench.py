@@ -91,9 +91,10 @@ def bench_covariant(steps, warmup, trees=32, polydeg=3, n_members=32):
     nodes = semi.ndofs
     avg_s = total_ms / launches / 1e3
     # algorithmic bytes per node and stage: u, k read, k and u_out written
-    # (3 FP64 each), the 17 auxiliary values the volume and sources use
-    # (J, G, G^-1, h_s, Christoffel) plus the node position (3)
-    bpn = 8 * (4 * 3 + 17 + 3)
+    # (3 FP64 each), the 15 geometry values the stage reads (J, G, G^-1, h_s,
+    # the Christoffel symbols, z / r), and per face node (4 n of them per n^2
+    # nodes) the 2 x 2 basis map (FP64) and the neighbour link (int32)
+    bpn = 8 * (4 * 3 + 15) + (8 * 4 + 4) * 4 / (polydeg + 1)
     out["single"] = {"workload": f"6x{trees}^2 elements, polydeg {polydeg}", "nodes": nodes,
                      "value": nodes * 5 * steps / el / 1e9, "unit": "GDOF*RHS/s",
                      "ms_per_step": el / steps * 1e3, "avg_launch_us": avg_s * 1e6,

@@ -416,7 +416,7 @@ void atmo_destroy(atmo_handle* h) {
   if (h->d_ubg) hipFree(h->d_ubg);
   if (h->d_dubg) hipFree(h->d_dubg);
   if (h->d_B) hipFree(h->d_B);
-  double* swe_bufs[] = {h->d_aux, h->d_xyz, h->d_dt_member};
+  double* swe_bufs[] = {h->d_aux, h->d_xyz, h->d_dt_member, h->d_geo, h->d_fxf};
   for (double* p : swe_bufs)
     if (p) hipFree(p);
   if (h->d_member) hipFree(h->d_member);

@@ -322,6 +322,31 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
   for (auto v : nbr)
     if (v == INT32_MIN) return fail(nullptr, ATMO_ERR_INVALID, "element face without interface");
 
+  // compact geometry of the stage kernel (structure of arrays, only what the
+  // stage reads) and the per-face-node basis maps B_own A_nbr, in which the
+  // neighbour's momentum enters (contravariant2global then global2contravariant,
+  // covariant_shallow_water.jl:140-153, as one 2 x 2 product)
+  const size_t NN = (size_t)n2 * E;
+  std::vector<double> geo(kSweGeo * NN), fxf((size_t)16 * n * E);
+  for (size_t gq = 0; gq < NN; ++gq) {
+    const double* a = g->aux_node_vars + gq * 26;
+    for (int c = 0; c < 8; ++c) geo[c * NN + gq] = a[12 + c];
+    for (int c = 0; c < 6; ++c) geo[(8 + c) * NN + gq] = a[20 + c];
+    const double* x = g->node_coordinates + gq * 3;
+    geo[14 * NN + gq] = x[2] / std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  }
+  for (int e = 0; e < E; ++e)
+    for (int f = 0; f < 4; ++f)
+      for (int k = 0; k < n; ++k) {
+        const int i = f == 0 ? 0 : (f == 1 ? n - 1 : k), j = f == 2 ? 0 : (f == 3 ? n - 1 : k);
+        const double* bo = g->aux_node_vars + ((size_t)e * n2 + i + n * j) * 26 + 6;  // B (2 x 3)
+        const double* an = g->aux_node_vars + (size_t)nbr[((size_t)e * 4 + f) * n + k] * 26;  // A (3 x 2)
+        double* M = &fxf[(((size_t)e * 4 + f) * n + k) * 4];
+        for (int r = 0; r < 2; ++r)
+          for (int c = 0; c < 2; ++c)
+            M[2 * r + c] = bo[r] * an[3 * c] + bo[r + 2] * an[1 + 3 * c] + bo[r + 4] * an[2 + 3 * c];
+      }
+
   auto* h = new atmo_handle;
   h->eq = ATMO_EQ_COVARIANT_SWE_2D;
   h->swe = *cfg;
@@ -347,6 +372,8 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
       hipMalloc(&h->d_uio, sb) != hipSuccess || hipMalloc(&h->d_duio, sb) != hipSuccess ||
       hipMalloc(&h->d_aux, sizeof(double) * 26 * n2 * (size_t)E) != hipSuccess ||
       hipMalloc(&h->d_xyz, sizeof(double) * 3 * n2 * (size_t)E) != hipSuccess ||
+      hipMalloc(&h->d_geo, sizeof(double) * kSweGeo * n2 * (size_t)E) != hipSuccess ||
+      hipMalloc(&h->d_fxf, sizeof(double) * 16 * n * (size_t)E) != hipSuccess ||
       hipMalloc(&h->d_nbr, sizeof(int32_t) * nbr.size()) != hipSuccess ||
       hipMalloc(&h->d_member, sizeof(int32_t) * E) != hipSuccess ||
       hipMalloc(&h->d_member_first, sizeof(int32_t) * (M + 1)) != hipSuccess ||
@@ -358,6 +385,10 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
                 hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->d_xyz, g->node_coordinates, sizeof(double) * 3 * n2 * (size_t)E,
                 hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->d_geo, geo.data(), sizeof(double) * geo.size(), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(h->d_fxf, fxf.data(), sizeof(double) * fxf.size(), hipMemcpyHostToDevice) !=
+          hipSuccess ||
       hipMemcpy(h->d_nbr, nbr.data(), sizeof(int32_t) * nbr.size(), hipMemcpyHostToDevice) !=
           hipSuccess ||
       hipMemcpy(h->d_member, member.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice) !=
@@ -381,6 +412,9 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
   L.inv_w = 1.0 / b.w[0];
   L.aux = h->d_aux;
   L.xyz = h->d_xyz;
+  L.geo = h->d_geo;
+  L.fxf = h->d_fxf;
+  L.n_nodes = (long long)NN;
   L.nbr = h->d_nbr;
   L.member = h->d_member;
   L.dt_member = nullptr;

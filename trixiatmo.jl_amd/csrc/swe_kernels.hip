@@ -3,7 +3,11 @@
 //
 // Layout: u [3, n, n, E] (h, h v^1, h v^2), aux [26, n, n, E] as Trixi's
 // aux_node_vars (src/equations/equations.jl:178-250), node coordinates
-// [3, n, n, E].  A workgroup of 64, 128 or 256 lanes holds EPB = lanes / n^2
+// [3, n, n, E].  The stage kernel reads a compact structure-of-arrays copy of
+// the 14 auxiliary rows it needs plus z / r (Trixi's node-major aux rows span
+// 208 bytes, so reading a few of them fetches the whole row), and one 2 x 2
+// basis map per face node instead of the 12 basis entries of both sides.
+// A workgroup of 64, 128 or 256 lanes holds EPB = lanes / n^2
 // elements (one lane per node); the per-node fields the two-point fluxes need
 // (state, velocity v = hv / h and covariant velocity G v formed once per node,
 // J, G, G^-1, h_s) are staged in LDS.  Volume terms are gathered per node:
@@ -119,8 +123,8 @@ __device__ __forceinline__ void surface_terms(const SweLaunch& L, const double* 
   for (int v = 0; v < NV; ++v) f[v] += 0.5 * g[v];
 }
 
-// covariant_shallow_water_split.jl:273-302
-__device__ __forceinline__ void source_split(double omega, const double* u, const double* x,
+// covariant_shallow_water_split.jl:273-302 (sin_lat = z / r of the node)
+__device__ __forceinline__ void source_split(double omega, const double* u, double sin_lat,
                                              const AuxV& a, double* s) {
   const double hv0 = u[1], hv1 = u[2];
   const double v0 = u[1] / u[0], v1 = u[2] / u[0];
@@ -130,8 +134,7 @@ __device__ __forceinline__ void source_split(double omega, const double* u, cons
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) hcv[i][j] = a.Gcov(i, 0) * hvv[0][j] + a.Gcov(i, 1) * hvv[1][j];
-  const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
-  const double f = 2 * omega * x[2] / r;
+  const double f = 2 * omega * sin_lat;
   double t1[2], t2[2];
 #pragma unroll
   for (int k = 0; k < 2; ++k)
@@ -184,10 +187,11 @@ __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
     if (active) {
       const size_t g = (size_t)e * N2 + q;
       const double* un = L.u + g * NV;
-      const double* an = L.aux + g * NA;
+      const double* gg = L.geo + g;
+      const long long NN = L.n_nodes;
       const double h = un[0], hv1 = un[1], hv2 = un[2];
       const double v1 = hv1 / h, v2 = hv2 / h;
-      const double G11 = an[13], G12 = an[14], G22 = an[15];
+      const double G11 = gg[NN], G12 = gg[2 * NN], G22 = gg[3 * NN];
       s_n[es][0][q] = h;
       s_n[es][1][q] = hv1;
       s_n[es][2][q] = hv2;
@@ -195,14 +199,14 @@ __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
       s_n[es][4][q] = v2;
       s_n[es][5][q] = G11 * v1 + G12 * v2;
       s_n[es][6][q] = G12 * v1 + G22 * v2;
-      s_n[es][7][q] = an[12];
+      s_n[es][7][q] = gg[0];
       s_n[es][8][q] = G11;
       s_n[es][9][q] = G12;
       s_n[es][10][q] = G22;
-      s_n[es][11][q] = an[16];
-      s_n[es][12][q] = an[17];
-      s_n[es][13][q] = an[18];
-      s_n[es][14][q] = an[19];
+      s_n[es][11][q] = gg[4 * NN];
+      s_n[es][12][q] = gg[5 * NN];
+      s_n[es][13][q] = gg[6 * NN];
+      s_n[es][14][q] = gg[7 * NN];
     }
     __syncthreads();
 
@@ -260,23 +264,20 @@ __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
       const int j = f == 2 ? 0 : (f == 3 ? N - 1 : k);
       const int qn = i + N * j;
       const double ul[NV] = {s_n[fe][0][qn], s_n[fe][1][qn], s_n[fe][2][qn]};
-      const double* ao = L.aux + ((size_t)(e0 + fe) * N2 + qn) * NA;
-      double xa[NA];
+      // the own node's J, G, G^-1, h_s (aux rows 12-19) from LDS
+      double xa[NA] = {};
+      xa[12] = s_n[fe][7][qn];
 #pragma unroll
-      for (int c = 0; c < 20; ++c) xa[c] = ao[c];
-#pragma unroll
-      for (int c = 20; c < NA; ++c) xa[c] = 0.0;
+      for (int c = 0; c < 6; ++c) xa[13 + c] = s_n[fe][8 + c][qn];
+      xa[19] = s_n[fe][14][qn];
       const AuxV ax{xa, 1};
-      const int link = L.nbr[((size_t)(e0 + fe) * 4 + f) * N + k];
+      const size_t fnode = ((size_t)(e0 + fe) * 4 + f) * N + k;
+      const int link = L.nbr[fnode];
       const double* ur = L.u + (size_t)link * NV;
-      const double* ar = L.aux + (size_t)link * NA;  // neighbour basis_covariant = ar[0..5]
       // neighbour state in this element's contravariant basis (contravariant2global,
       // then global2contravariant: covariant_shallow_water.jl:140-153)
-      double gl[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) gl[c] = ar[c] * ur[1] + ar[3 + c] * ur[2];
-      const double tr[NV] = {ur[0], ax.B(0, 0) * gl[0] + ax.B(0, 1) * gl[1] + ax.B(0, 2) * gl[2],
-                             ax.B(1, 0) * gl[0] + ax.B(1, 1) * gl[1] + ax.B(1, 2) * gl[2]};
+      const double* M = L.fxf + fnode * 4;
+      const double tr[NV] = {ur[0], M[0] * ur[1] + M[1] * ur[2], M[2] * ur[1] + M[3] * ur[2]};
       const int o = f / 2;
       double fs[NV];
       if (f % 2 == 0) {  // negative-side face: -(f*(u_nbr, u_own) + g/2)
@@ -305,9 +306,16 @@ __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) du[v] *= fac;
       if (L.source_terms) {
-        const AuxV ao{L.aux + g * NA, 1};
+        const double* gg = L.geo + g;
+        const long long NN = L.n_nodes;
+        double xa[NA] = {};
+        xa[12] = s_n[es][7][q];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) xa[13 + c] = s_n[es][8 + c][q];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) xa[20 + c] = gg[(8 + c) * NN];
         double sv[NV];
-        source_split(L.rotation_rate, uo, L.xyz + g * 3, ao, sv);
+        source_split(L.rotation_rate, uo, gg[14 * NN], AuxV{xa, 1}, sv);
 #pragma unroll
         for (int v = 0; v < NV; ++v) du[v] += sv[v];
       }

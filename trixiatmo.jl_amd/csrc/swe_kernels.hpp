@@ -20,6 +20,14 @@ struct SweLaunch {
   double* k;
   const double* aux;    // [26, n, n, E]
   const double* xyz;    // [3, n, n, E]
+  // the stage kernel's geometry: geo[c * n_nodes + node] for c = J, G_11, G_12,
+  // G_22, G^11, G^12, G^22, h_s, the six Christoffel symbols (aux rows 12-25)
+  // and z / r; fxf[((e * 4 + f) * n + k) * 4 + 2 r + s] = (B_own A_nbr)_rs, the
+  // 2 x 2 map of a neighbour's contravariant momentum into this element's
+  // contravariant basis at face node k of face f
+  const double* geo;
+  const double* fxf;
+  long long n_nodes;
   const int* nbr;       // [E, 4, n] neighbour global node of each face node
   int mode;             // 0 rhs, 1 RK stage
   double rk_a, rk_dt, rk_b;
@@ -28,6 +36,7 @@ struct SweLaunch {
   int variant;              // workgroup shape: 0..2 = 1, 2, 4 waves; +3 persistent grid
 };
 constexpr int kSweVariants = 6;
+constexpr int kSweGeo = 15;
 
 bool swe_launch_stage(const SweLaunch& L, hipStream_t s);
 // per-member max over elements of the summed scaled wave speeds -> out[n_members]
