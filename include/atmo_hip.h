@@ -285,6 +285,11 @@ int atmo_saturation_newton(atmo_handle* handle, int32_t seeded, int32_t* iterati
 int atmo_enable_newton_stats(atmo_handle* handle, int32_t enable);
 int atmo_newton_stats_read(atmo_handle* handle, atmo_newton_stats* stats);
 
+/* Entry points that take device pointers (x_device, iterations_device, ...)
+ * run on the library's own stream and read them at once: the caller finishes
+ * whatever writes them on other streams first.  They synchronise the
+ * library's stream before returning, so their results are complete. */
+
 /* features of the resident state for the saturation seed (device pointers):
  * x[3 * node] = (rho, internal energy density, rho_moist + rho_rain),
  * y[2 * node] = (T, rho_vapour / rho) */

@@ -134,7 +134,11 @@ def test_steps_and_max_dt(oracle):
     u_gpu = semi.get_state()
     u_ref = ro.ck2n54(u, 0.0, dt_ref, 3, 1e-9, 20)
     a, b = u_gpu.reshape(9, -1, order="F"), u_ref.reshape(9, -1, order="F")
-    rel = np.abs(a - b).max(axis=1) / np.abs(b).max(axis=1)
+    scale = np.abs(b).max(axis=1)
+    # vapour and cloud split rho_moist: unsaturated nodes carry a cloud density
+    # at the solver's rounding level, so both are measured on rho_moist's scale
+    scale[6:8] = np.maximum(scale[6:8], scale[1])
+    rel = np.abs(a - b).max(axis=1) / scale
     print("3-step state rel err", rel, flush=True)
     assert np.all(rel <= 1e-11), rel
     ig = semi.integrals()

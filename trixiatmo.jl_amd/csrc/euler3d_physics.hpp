@@ -46,6 +46,36 @@ __device__ __forceinline__ T inv_ln_mean(T x, T y) {
   return div_(log_(div_(y, x)), y - x);
 }
 
+// ln_mean / inv_ln_mean from node logarithms: (y - x) / (log y - log x) away
+// from x = y, the series of Ranocha et al. (2023) where (x-y)^2/(x+y)^2 < 1e-4.
+// Both branches end in one division of a selected numerator and denominator,
+// so lanes of a wave that take different branches do not serialise; f^2 uses
+// a reciprocal refined by one Newton step (it only enters the small series
+// terms).
+__device__ __forceinline__ double rcp_refined(double x) {
+  const double r = __builtin_amdgcn_rcp(x);
+  return r * fma(-x, r, 2.0);
+}
+__device__ __forceinline__ float rcp_refined(float x) { return __builtin_amdgcn_rcpf(x); }
+
+template <typename T>
+__device__ __forceinline__ T ln_mean_logs(T x, T y, T lx, T ly) {
+  const T d = y - x, s = x + y;
+  const bool ser = d * d < T(1.0e-4) * (s * s);
+  const T f2 = ser ? (d * d) * rcp_refined(s * s) : T(0);
+  const T q = T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0)));
+  return div_(ser ? s : d, ser ? q : ly - lx);
+}
+
+template <typename T>
+__device__ __forceinline__ T inv_ln_mean_logs(T x, T y, T lx_minus_ly) {
+  const T d = y - x, s = x + y;
+  const bool ser = d * d < T(1.0e-4) * (s * s);
+  const T f2 = ser ? (d * d) * rcp_refined(s * s) : T(0);
+  const T q = T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0)));
+  return div_(ser ? q : -lx_minus_ly, ser ? s : d);
+}
+
 // primitive quantities a two-point flux needs from one node
 // (e = rho_e / rho, inv_rho = 1 / rho)
 template <typename T>

@@ -147,36 +147,6 @@ __device__ __forceinline__ T delta_ln_mean(T A, T s, T Ab, T sb, T Ap, T sp, T d
   return div_(A, dL) - T(0.5) * sb;  // flat background: ln_mean(xb, yb) = (xb + yb) / 2
 }
 
-// ln_mean / inv_ln_mean from node logarithms: (y - x) / (log y - log x) away
-// from x = y, the series of Ranocha et al. (2023) where (x-y)^2/(x+y)^2 < 1e-4.
-// Both branches end in one division of a selected numerator and denominator,
-// so lanes of a wave that take different branches do not serialise; f^2 uses
-// a reciprocal refined by one Newton step (it only enters the small series
-// terms).
-__device__ __forceinline__ double rcp_refined(double x) {
-  const double r = __builtin_amdgcn_rcp(x);
-  return r * fma(-x, r, 2.0);
-}
-__device__ __forceinline__ float rcp_refined(float x) { return __builtin_amdgcn_rcpf(x); }
-
-template <typename T>
-__device__ __forceinline__ T ln_mean_logs(T x, T y, T lx, T ly) {
-  const T d = y - x, s = x + y;
-  const bool ser = d * d < T(1.0e-4) * (s * s);
-  const T f2 = ser ? (d * d) * rcp_refined(s * s) : T(0);
-  const T q = T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0)));
-  return div_(ser ? s : d, ser ? q : ly - lx);
-}
-
-template <typename T>
-__device__ __forceinline__ T inv_ln_mean_logs(T x, T y, T lx_minus_ly) {
-  const T d = y - x, s = x + y;
-  const bool ser = d * d < T(1.0e-4) * (s * s);
-  const T f2 = ser ? (d * d) * rcp_refined(s * s) : T(0);
-  const T q = T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0)));
-  return div_(ser ? q : -lx_minus_ly, ser ? s : d);
-}
-
 // the volume flux f (5 entries) and the gravity term g (4 entries) of one pair
 template <int VOL, int NC, typename T>
 __device__ __forceinline__ void pair_flux(const E3Params& q, const FastPrim<T>& l,

@@ -47,27 +47,29 @@ __device__ __forceinline__ void nonlinear_sol(const double* u, double& rv, doubl
   T = u[8];
 }
 
+// x^(1/8) by three correctly rounded square roots (pow(x, 0.125) of the
+// reference to a few ulp, at a fraction of pow's cost)
+__device__ __forceinline__ double root8(double x) { return sqrt(sqrt(sqrt(x))); }
+
 // terminal_velocity_rain (:336-351)
 __device__ __forceinline__ double terminal_velocity(double rho_moist, double rho_rain) {
   if (rho_rain > 0.0)
-    return v_0 * 1.9386213994279082 * pow(rho_rain / (kPi * (rho_moist + rho_rain) * N_0), 0.125);
+    return v_0 * 1.9386213994279082 * root8(rho_rain / (kPi * (rho_moist + rho_rain) * N_0));
   return 0.0;
 }
 
-// saturation_vapour_pressure and its derivative (:353-398)
+// saturation_vapour_pressure (:353-375), ref_sp (T / ref_T)^k1 exp(k2 (1 / ref_T
+// - 1 / T)), formed as one exponential of the summed exponents
+constexpr double kSvp1 = (c_pv - c_l) / R_v;
+constexpr double kSvp2 = (L_ref - (c_pv - c_l) * ref_T) / R_v;
 __device__ __forceinline__ double svp(double T) {
-  constexpr double k1 = (c_pv - c_l) / R_v;
-  constexpr double k2 = (L_ref - (c_pv - c_l) * ref_T) / R_v;
-  return ref_sp * pow(T / ref_T, k1) * exp(k2 * (1 / ref_T - 1 / T));
+  return ref_sp * exp(kSvp1 * log(T / ref_T) + kSvp2 * (1 / ref_T - 1 / T));
 }
 
-__device__ __forceinline__ double svp_deriv(double T) {
-  constexpr double k1 = (c_pv - c_l) / R_v;
-  constexpr double k2 = (L_ref - (c_pv - c_l) * ref_T) / R_v;
-  double d = ref_sp / pow(ref_T, k1);
-  d *= k1 * pow(T, k1 - 1) + k2 * pow(T, k1 - 2);
-  d *= exp(k2 * (1 / ref_T - 1 / T));
-  return d;
+// its derivative (:377-398), ref_sp / ref_T^k1 (k1 T^(k1-1) + k2 T^(k1-2))
+// exp(...), which equals svp(T) (k1 + k2 / T) / T
+__device__ __forceinline__ double svp_deriv_from(double sv, double T) {
+  return sv * (kSvp1 + kSvp2 / T) / T;
 }
 
 // quantities of one node used by the two-point and surface fluxes
@@ -171,6 +173,7 @@ __device__ __forceinline__ void flux_ec_rain(const RNode& l, const RNode& r, dou
 // the per-node quantities flux_ec_rain reads, formed once per node
 struct ECNode {
   double rd, rv, rc, rr, v1, v2, it, vr;  // it = 1 / T, vr from rho_vapour + rho_cloud
+  double lrd, lrv, lit;                   // log(rho_dry), log(rho_vapour), log(1 / T)
 };
 
 __device__ __forceinline__ void flux_ec_rain(const ECNode& l, const ECNode& r, double n1,
@@ -185,11 +188,13 @@ __device__ __forceinline__ void flux_ec_rain(const ECNode& l, const ECNode& r, d
   const double vap_a = 0.5 * (l.rv + r.rv);
   const double cld_a = 0.5 * (l.rc + r.rc);
   const double rain_a = 0.5 * (l.rr + r.rr);
+  // logarithmic means from the node logarithms (the series branch where the
+  // two values are close), no logarithm per pair
   double dry_log = 0.0, vap_log = 0.0;
-  if (l.rd != 0.0 && r.rd != 0.0) dry_log = ln_mean(l.rd, r.rd);
-  if (l.rv != 0.0 && r.rv != 0.0) vap_log = ln_mean(l.rv, r.rv);
+  if (l.rd != 0.0 && r.rd != 0.0) dry_log = ln_mean_logs(l.rd, r.rd, l.lrd, r.lrd);
+  if (l.rv != 0.0 && r.rv != 0.0) vap_log = ln_mean_logs(l.rv, r.rv, l.lrv, r.lrv);
   const double it_avg = 0.5 * (itl + itr);
-  const double it_log = inv_ln_mean(itl, itr);
+  const double it_log = inv_ln_mean_logs(itl, itr, l.lit - r.lit);
   const double p_int = (1.0 / it_avg) * (R_d * dry_a + R_v * vap_a);
   const double f_vap = vap_log * vdn;
   const double f_cld = cld_a * vdn;
@@ -220,6 +225,9 @@ __device__ __forceinline__ ECNode ecnode(const double* u) {
   a.v2 = u[4] * inv;
   a.it = 1.0 / T;
   a.vr = terminal_velocity(a.rv + a.rc, a.rr);
+  a.lrd = log(a.rd);
+  a.lrv = log(a.rv);
+  a.lit = -log(T);
   return a;
 }
 
@@ -252,9 +260,11 @@ __device__ __forceinline__ void source_rainy(const double* u, double* s) {
   nonlinear_sol(u, rv, rc, T);
   const double rho_vs = svp(T) / (R_v * T);
   const double rr = u[2];
-  double S_ev = (3.86e-3 - 9.41e-5 * (T - ref_T)) * (1 + 9.1 * pow(rr, 0.1875));
-  S_ev *= (rho_vs - rv) * pow(rr, 0.5);
-  const double S = 0.001 * rc + 1.72 * rc * pow(rr, 0.875) - S_ev;
+  // rr^(3/16), rr^(1/2), rr^(7/8) from the chain of square roots
+  const double r2 = sqrt(rr), r4 = sqrt(r2), r8 = sqrt(r4), r16 = sqrt(r8);
+  double S_ev = (3.86e-3 - 9.41e-5 * (T - ref_T)) * (1 + 9.1 * (r16 * r8));
+  S_ev *= (rho_vs - rv) * r2;
+  const double S = 0.001 * rc + 1.72 * rc * (r2 * r4 * r8) - S_ev;
 #pragma unroll
   for (int v = 0; v < NV; ++v) s[v] = 0.0;
   s[1] = -S;
@@ -272,7 +282,7 @@ __device__ __forceinline__ void source_convergence(double x, double y, double t,
   const double rho_x = w * A * co;
   const double rho_t = -rho_x;
   const double T = rho + 250.0;
-  const double sv = svp(T), dsv = svp_deriv(T);
+  const double sv = svp(T), dsv = svp_deriv_from(sv, T);
   const double rv = sv / (R_v * T);
   const double rc = rho / c_l * 3000;
   const double rr = rho / c_l * 1000;
@@ -309,15 +319,16 @@ __device__ __forceinline__ void source_convergence(double x, double y, double t,
   s[5] = sE;
 }
 
-// saturation_residual and its Jacobian (:738-809); J[row + 3 col]
-__device__ __forceinline__ void residual(const double* u, const double* x, double* r) {
+// saturation_residual and its Jacobian (:738-809); J[row + 3 col]; both take
+// sv = saturation_vapour_pressure(x[2])
+__device__ __forceinline__ void residual(const double* u, const double* x, double sv, double* r) {
   const double rho = u[0] + u[1] + u[2];
   const double inv = 1.0 / rho;
   const double v1 = u[3] * inv, v2 = u[4] * inv;
   double r1 = (c_vd * u[0] + c_vv * x[0] + c_l * (x[1] + u[2])) * x[2];
   r1 += x[0] * L_ref;
   r1 -= (u[5] - rho * 0.5 * (v1 * v1 + v2 * v2));
-  double r2 = fmin(svp(x[2]) / (R_v * x[2]), u[1]);
+  double r2 = fmin(sv / (R_v * x[2]), u[1]);
   r2 -= x[0];
   r2 *= 1.0e7;
   double r3 = u[1];
@@ -328,8 +339,8 @@ __device__ __forceinline__ void residual(const double* u, const double* x, doubl
   r[2] = r3;
 }
 
-__device__ __forceinline__ void jacobian(const double* u, const double* x, double* J) {
-  const double sv = svp(x[2]), svt = svp_deriv(x[2]);
+__device__ __forceinline__ void jacobian(const double* u, const double* x, double sv, double* J) {
+  const double svt = svp_deriv_from(sv, x[2]);
   J[0] = c_vv * x[2] + L_ref;
   J[3] = c_l * x[2];
   J[6] = c_vd * u[0] + c_vv * x[0] + c_l * (x[1] + u[2]);
@@ -405,10 +416,11 @@ __device__ __forceinline__ void solve3(const double* J, const double* r, double*
 __device__ __forceinline__ int newton_node(const double* un, double* x, double tol, int max_iter) {
   for (int it = 1; it <= max_iter; ++it) {
     double r[3];
-    residual(un, x, r);
+    const double sv = svp(x[2]);
+    residual(un, x, sv, r);
     if (fmax(fabs(r[0]), fmax(fabs(r[1]), fabs(r[2]))) < tol) return it - 1;
     double J[9], dx[3];
-    jacobian(un, x, J);
+    jacobian(un, x, sv, J);
     solve3(J, r, dx);
 #pragma unroll
     for (int k = 0; k < 3; ++k) x[k] += -dx[k];
@@ -507,14 +519,15 @@ __device__ __forceinline__ bool mlp_guess(const float* P, const double* u, doubl
 }
 
 template <int N, bool SEEDED>
-__global__ __launch_bounds__(RShape<N>::BLOCK) void rainy_stage(RainyLaunch L) {
+__global__ __launch_bounds__(RShape<N>::BLOCK)
+void rainy_stage(RainyLaunch L) {
   using S = RShape<N>;
   constexpr int N2 = S::N2, EPB = S::EPB, B = S::BLOCK;
   constexpr int VOLW = 12 * N2 > NV * S::NF ? 12 * N2 : NV * S::NF;
   __shared__ double s_u[EPB][NV][N2];
   // one buffer per element, used in turn for the volume terms (weak form: the
   // contravariant fluxes of the 6 flux rows [2][6][N2]; flux differencing: the
-  // per-node quantities of flux_ec_rain [8][N2]) and for the face fluxes [9][4 n]
+  // per-node quantities of flux_ec_rain [11][N2]) and for the face fluxes [9][4 n]
   __shared__ double s_buf[EPB][VOLW];
   extern __shared__ float s_mlp[];  // network parameters + normalisation when seeding
   auto s_f = [&](int es, int o, int v, int q) -> double& { return s_buf[es][(o * 6 + v) * N2 + q]; };
@@ -578,12 +591,14 @@ __global__ __launch_bounds__(RShape<N>::BLOCK) void rainy_stage(RainyLaunch L) {
       const ECNode a = ecnode(uo);
       s_ec(es, 0, q) = a.rd; s_ec(es, 1, q) = a.rv; s_ec(es, 2, q) = a.rc; s_ec(es, 3, q) = a.rr;
       s_ec(es, 4, q) = a.v1; s_ec(es, 5, q) = a.v2; s_ec(es, 6, q) = a.it; s_ec(es, 7, q) = a.vr;
+      s_ec(es, 8, q) = a.lrd; s_ec(es, 9, q) = a.lrv; s_ec(es, 10, q) = a.lit;
     }
     __syncthreads();
     if (active) {
       ECNode a;
       a.rd = s_ec(es, 0, q); a.rv = s_ec(es, 1, q); a.rc = s_ec(es, 2, q); a.rr = s_ec(es, 3, q);
       a.v1 = s_ec(es, 4, q); a.v2 = s_ec(es, 5, q); a.it = s_ec(es, 6, q); a.vr = s_ec(es, 7, q);
+      a.lrd = s_ec(es, 8, q); a.lrv = s_ec(es, 9, q); a.lit = s_ec(es, 10, q);
 #pragma unroll
       for (int o = 0; o < 2; ++o) {
         const int own = o == 0 ? ti : tj;
@@ -596,6 +611,7 @@ __global__ __launch_bounds__(RShape<N>::BLOCK) void rainy_stage(RainyLaunch L) {
           b.rd = s_ec(es, 0, pq); b.rv = s_ec(es, 1, pq); b.rc = s_ec(es, 2, pq);
           b.rr = s_ec(es, 3, pq); b.v1 = s_ec(es, 4, pq); b.v2 = s_ec(es, 5, pq);
           b.it = s_ec(es, 6, pq); b.vr = s_ec(es, 7, pq);
+          b.lrd = s_ec(es, 8, pq); b.lrv = s_ec(es, 9, pq); b.lit = s_ec(es, 10, pq);
           double f[6];
           if (own < m) flux_ec_rain(a, b, n1, n2, f);
           else flux_ec_rain(b, a, n1, n2, f);

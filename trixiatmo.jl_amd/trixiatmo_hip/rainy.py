@@ -30,7 +30,16 @@ class NonlinearSolveDG:
 
 
 def _ptr(t):
-    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+    """device pointer of a torch tensor for the library, which runs on its own
+    stream and reads the memory at once: work that torch has queued on the
+    tensor's stream is finished first (the library's entry points synchronise
+    their own stream before they return, so results are complete for torch)"""
+    if t is None:
+        return None
+    if t.is_cuda:
+        import torch
+        torch.cuda.current_stream(t.device).synchronize()
+    return ctypes.c_void_p(t.data_ptr())
 
 
 class SemidiscretizationRainy:
