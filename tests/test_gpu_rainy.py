@@ -230,7 +230,9 @@ def test_seeded_newton_converges_to_the_same_states():
     xm, xs = x.mean(0).cpu().numpy(), x.std(0).cpu().numpy() + 1e-12
     ym, ys = y.mean(0).cpu().numpy(), y.std(0).cpu().numpy() + 1e-12
     mlp.set_normalization(xm, xs, ym, ys)
-    perm = torch.randperm(n, device="cuda").to(torch.int32)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0)
+    perm = torch.randperm(n, device="cuda", generator=g).to(torch.int32)
     for _ in range(20):
         mlp.train(x, y, perm, 256, n // 256, 3e-3)
     semi.attach_mlp(mlp)
@@ -249,7 +251,8 @@ def test_seeded_newton_converges_to_the_same_states():
     np.testing.assert_allclose(a[:, 8], b[:, 8], rtol=1e-11)
     np.testing.assert_allclose(a[:, 6], b[:, 6], rtol=1e-9)
     np.testing.assert_allclose(a[:, 8], ref.reshape(-1, 9)[:, 8], rtol=1e-11)
-    assert st_seeded["updates"] <= st_default["updates"]
+    # what the trained network gains over the default guess on a cold solve
+    assert 2 * st_seeded["updates"] <= st_default["updates"], (st_seeded, st_default)
 
 
 def test_seeded_stage_limiter_matches_unseeded():

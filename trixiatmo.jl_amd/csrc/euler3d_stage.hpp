@@ -788,12 +788,13 @@ void e3_stage_direct(
 }
 
 // which volume form a shape runs, from timings on MI355X: the direct form for
-// FP64 at every polydeg and for FP32 from polydeg 5 on (polydeg 7, EC: 1.8x
-// faster than the pair form); the pair form for FP32 up to polydeg 4, whose
-// background-split flux is costly enough that evaluating it twice loses
+// FP64 at every polydeg and for FP32 at polydeg 3 (6 % faster) and 7 (EC:
+// 1.8x faster than the pair form); the pair form for FP32 at polydeg 4 (2 %
+// faster: its background-split flux is costly enough that evaluating every
+// pair twice loses)
 template <int N, int VOL, typename T>
 constexpr bool use_direct_form() {
-  return sizeof(T) == 8 || N >= 6;
+  return sizeof(T) == 8 || N != 5;
 }
 
 template <int N, int VOL, int NC, typename T, bool KAHAN>

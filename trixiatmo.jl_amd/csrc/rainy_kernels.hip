@@ -503,18 +503,26 @@ __device__ __forceinline__ void mlp_inputs(const double* u, float* x) {
   x[2] = (float)(u[1] + u[2]);
 }
 
-// seeded guess of one node from the network's (T, q_v); returns false when
-// the prediction is unusable
+// seeded guess of one node: the network's q_v splits rho_moist into vapour
+// and cloud (all vapour when the prediction is within 1 % of it: the
+// unsaturated branch), and T then follows from the energy row of the
+// saturation residual, which is linear in T, so the guess satisfies the
+// energy and total-water rows exactly and Newton only has the saturation row
+// left to correct; returns false when the guess is unusable
 __device__ __forceinline__ bool mlp_guess(const float* P, const double* u, double* x) {
   float xi[kMlpIn], y[kMlpOut];
   mlp_inputs(u, xi);
   mlp_eval(P, P + kMlpParams, xi, y);
   const double rho = u[0] + u[1] + u[2];
-  double rv = (double)y[1] * rho;
-  rv = fmin(fmax(rv, 0.0), u[1]);
+  const double inv = 1.0 / rho;
+  const double v1 = u[3] * inv, v2 = u[4] * inv;
+  const double e_int = u[5] - rho * 0.5 * (v1 * v1 + v2 * v2);
+  double rv = fmin(fmax((double)y[1] * rho, 0.0), u[1]);
+  if (rv >= 0.99 * u[1]) rv = u[1];
+  const double rc = u[1] - rv;
   x[0] = rv;
-  x[1] = u[1] - rv;
-  x[2] = (double)y[0];
+  x[1] = rc;
+  x[2] = (e_int - rv * L_ref) / (c_vd * u[0] + c_vv * rv + c_l * (rc + u[2]));
   return x[2] > 0.0;
 }
 

@@ -426,15 +426,19 @@ def moist_hydrostatic_column(z, equations, theta=300.0, rel_humidity=0.5, p_surf
 
 
 def initial_condition_rainy_bubble(mesh, equations, seed=0, theta=300.0, radius=2000.0,
-                                   amplitude=2.0):
+                                   amplitude=2.0, relative_humidity=None):
     """Seeded moist warm bubble: a hydrostatic background with dry potential
-    temperature 300 K and a relative humidity drawn from U[0.2, 0.95], plus a
-    temperature bump amplitude cos^2(pi r / (2 R)) of R = 2 km at a random
-    centre, at the background pressure and relative humidity; at rest, no
-    cloud or rain, the saturation variables (rho_vapour, 0, T) consistent."""
+    temperature 300 K and a relative humidity drawn from U[0.2, 0.95] (or the
+    given one), plus a temperature bump amplitude cos^2(pi r / (2 R)) of
+    R = 2 km at a random centre, at the background pressure and relative
+    humidity; at rest, no cloud or rain, the saturation variables
+    (rho_vapour, 0, T) as given (a relative humidity above 1 leaves the
+    state supersaturated: the saturation solve then condenses cloud)."""
     eq = equations
     rng = np.random.default_rng(seed)
     rh = rng.uniform(0.2, 0.95)
+    if relative_humidity is not None:
+        rh = float(relative_humidity)
     xc = rng.uniform(mesh.x0 + radius, mesh.x1 - radius)
     zc = rng.uniform(mesh.y0 + radius, mesh.y0 + 0.5 * (mesh.y1 - mesh.y0))
     x, z = mesh.node_coordinates

@@ -216,15 +216,16 @@ def zonal_jet_ensemble(n_members=32, polydeg=3, seed=0, choices=(8, 16, 24, 32))
 # rainy Euler in 2D
 
 def rainy_bubble(cells=(256, 128), polydeg=3, seed=0, domain=(20000.0, 10000.0),
-                 newton_tolerance=1e-9, newton_max_iterations=20, periodic_y=False):
+                 newton_tolerance=1e-9, newton_max_iterations=20, periodic_y=False,
+                 relative_humidity=None):
     """Moist warm bubble on a 20 km x 10 km StructuredMesh, 256 x 128 cells,
     polydeg 3: CompressibleRainyEulerEquations2D, flux_ec_rain volume,
     flux_lax_friedrichs surface, source_terms_rainy, NonlinearSolveDG stage
     limiter; periodic in x and boundary_condition_simple_slip_wall in y (a
     hydrostatic column is not periodic in height); a hydrostatic background of
     dry potential temperature 300 K and seeded relative humidity in
-    [0.2, 0.95] with a 2 K bubble of 2 km radius at a seeded position.
-    Returns (mesh, equations, solver, source_terms, stage_limiter, u0, info)."""
+    [0.2, 0.95] (or the given one) with a 2 K bubble of 2 km radius at a
+    seeded position.  Returns (mesh, equations, solver, source_terms, stage_limiter, u0, info)."""
     from .equations import CompressibleRainyEulerEquations2D
     from .initial_conditions import initial_condition_rainy_bubble
     from .meshes import StructuredMesh2D
@@ -234,7 +235,8 @@ def rainy_bubble(cells=(256, 128), polydeg=3, seed=0, domain=(20000.0, 10000.0),
                             periodicity=(True, periodic_y))
     solver = DGSEM(polydeg=polydeg, surface_flux="flux_lax_friedrichs",
                    volume_integral=VolumeIntegralFluxDifferencing("flux_ec_rain"))
-    u0, info = initial_condition_rainy_bubble(mesh, eq, seed=seed)
+    u0, info = initial_condition_rainy_bubble(mesh, eq, seed=seed,
+                                              relative_humidity=relative_humidity)
     lim = NonlinearSolveDG(newton_tolerance, newton_max_iterations)
     return mesh, eq, solver, "source_terms_rainy", lim, u0, info
 
