@@ -73,6 +73,11 @@ struct Acc {
     }
   }
   __device__ __forceinline__ T value() const { return s; }
+  // multiply by a power of two (exact)
+  __device__ __forceinline__ void scale(T f) {
+    s *= f;
+    c *= f;
+  }
 };
 
 template <typename T>
@@ -268,13 +273,39 @@ __device__ __forceinline__ void surface_momentum_split(const E3Params& q, int su
 
 // face-node passes of one workgroup and the neighbour states they need,
 // loaded at the start of the kernel so that the dependent loads (link, then
-// the neighbour's state) overlap the volume phase
+// the neighbour's state) overlap the volume phase; from polydeg 4 on only the
+// first pass (every lane of the workgroup) is: polydeg 4's partial second
+// pass then loads in place, which keeps 14 registers free during the volume
+// phase (5 % faster there; polydeg 3, one wave per element, prefetches both)
 template <int N>
 struct FacePrefetch {
   static constexpr int PASSES = (6 * Shape<N>::N2 + Shape<N>::BLOCK - 1) / Shape<N>::BLOCK;
-  int link[PASSES], info[PASSES];
-  double ur[PASSES][6];
+  static constexpr int PF = N >= 5 ? 1 : PASSES;
+  int link[PF], info[PF];
+  double ur[PF][6];
 };
+
+// link (neighbour's global node, < 0 on a boundary) and face_info of pass ps
+template <int N>
+__device__ __forceinline__ void face_link(const FacePrefetch<N>& pf, int e, int t, int ps,
+                                          const int* __restrict__ nbr,
+                                          const int* __restrict__ face_info, int& link,
+                                          int& info) {
+  constexpr int N2 = Shape<N>::N2, NF = 6 * N2;
+  if (ps < FacePrefetch<N>::PF) {
+    link = pf.link[ps];
+    info = pf.info[ps];
+    return;
+  }
+  const int fid = t + ps * Shape<N>::BLOCK;
+  link = -1;
+  info = 0;
+  if (fid < NF) {
+    const int f = fid / N2, ab = fid - f * N2;
+    link = nbr[((size_t)e * 6 + f) * N2 + ab];
+    info = face_info[(size_t)e * 6 + f];
+  }
+}
 
 template <int N>
 __device__ __forceinline__ void face_prefetch(FacePrefetch<N>& pf, int e, int t,
@@ -283,7 +314,7 @@ __device__ __forceinline__ void face_prefetch(FacePrefetch<N>& pf, int e, int t,
                                               const int* __restrict__ face_info) {
   constexpr int N2 = Shape<N>::N2, NF = 6 * N2;
 #pragma unroll
-  for (int ps = 0; ps < FacePrefetch<N>::PASSES; ++ps) {
+  for (int ps = 0; ps < FacePrefetch<N>::PF; ++ps) {
     const int fid = t + ps * Shape<N>::BLOCK;
     pf.link[ps] = -1;
     pf.info[ps] = 0;
@@ -294,7 +325,7 @@ __device__ __forceinline__ void face_prefetch(FacePrefetch<N>& pf, int e, int t,
     }
   }
 #pragma unroll
-  for (int ps = 0; ps < FacePrefetch<N>::PASSES; ++ps) {
+  for (int ps = 0; ps < FacePrefetch<N>::PF; ++ps) {
     const int link = pf.link[ps];
 #pragma unroll
     for (int v = 0; v < 6; ++v) pf.ur[ps][v] = link >= 0 ? u[(size_t)link * 6 + v] : 0.0;
@@ -330,7 +361,9 @@ struct FaceNormals {
 template <int N, typename T>
 __device__ __forceinline__ void face_normals(FaceNormals<N, T>& fn, int e, int t,
                                              const T* __restrict__ cv, const T* s_cv,
-                                             const FacePrefetch<N>& pf) {
+                                             const FacePrefetch<N>& pf,
+                                             const int* __restrict__ nbr,
+                                             const int* __restrict__ face_info) {
   constexpr int N3 = Shape<N>::N3;
 #pragma unroll
   for (int ps = 0; ps < FacePrefetch<N>::PASSES; ++ps) {
@@ -339,7 +372,8 @@ __device__ __forceinline__ void face_normals(FaceNormals<N, T>& fn, int e, int t
     if (!face_node_of<N>(t, ps, fid, f, d, node)) continue;
     // face_info: 0 = primary side or boundary; 8 | dp = secondary side of an
     // interface whose primary face direction is dp
-    const int info = pf.info[ps];
+    int link, info;
+    face_link<N>(pf, e, t, ps, nbr, face_info, link, info);
     if (info == 0) {
       const T sgn = (f % 2 == 0) ? T(-1) : T(1);
 #pragma unroll
@@ -349,7 +383,7 @@ __device__ __forceinline__ void face_normals(FaceNormals<N, T>& fn, int e, int t
     } else {
       const int dp = info & 7;
       const T sgn = (dp % 2 == 0) ? T(1) : T(-1);  // minus the primary's outward normal
-      const T* jp = cv + (size_t)pf.link[ps] * 9 + 3 * (dp / 2);
+      const T* jp = cv + (size_t)link * 9 + 3 * (dp / 2);
 #pragma unroll
       for (int c = 0; c < 3; ++c) fn.n[ps][c] = sgn * jp[c];
     }
@@ -359,8 +393,9 @@ __device__ __forceinline__ void face_normals(FaceNormals<N, T>& fn, int e, int t
 template <int N, typename T>
 __device__ __forceinline__ void face_fluxes_to_lds(
     const E3Params& q, int e, int t, double inv_w, const double* __restrict__ u,
-    const FacePrefetch<N>& pf, const FaceNormals<N, T>& fn, int surf, int snc,
-    const Background& bg, const T* s_w, const double* s_pb, T* s_sf) {
+    const FacePrefetch<N>& pf, const FaceNormals<N, T>& fn, const int* __restrict__ nbr,
+    const int* __restrict__ face_info, int surf, int snc, const Background& bg, const T* s_w,
+    const double* s_pb, T* s_sf) {
   constexpr int N2 = Shape<N>::N2, N3 = Shape<N>::N3, NF = 6 * N2;
 #pragma unroll
   for (int ps = 0; ps < FacePrefetch<N>::PASSES; ++ps) {
@@ -383,12 +418,15 @@ __device__ __forceinline__ void face_fluxes_to_lds(
 #pragma unroll
       for (int v = 0; v < 6; ++v) uo[v] = u[gn * 6 + v];
     }
-    const int link = pf.link[ps];
+    int link, info;
+    face_link<N>(pf, e, t, ps, nbr, face_info, link, info);
     const T n0 = fn.n[ps][0], n1 = fn.n[ps][1], n2 = fn.n[ps][2];
     double ur[6];
     if (link >= 0) {
 #pragma unroll
-      for (int v = 0; v < 6; ++v) ur[v] = pf.ur[ps][v];
+      for (int v = 0; v < 6; ++v)
+        ur[v] = ps < FacePrefetch<N>::PF ? pf.ur[ps < FacePrefetch<N>::PF ? ps : 0][v]
+                                         : u[(size_t)link * 6 + v];
     } else {
       // boundary_condition_slip_wall (:93-116): mirror the normal momentum
       const double m0 = n0, m1 = n1, m2 = n2;
@@ -690,9 +728,10 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
     }
   }
   FaceNormals<N, T> fn;
-  face_normals<N, T>(fn, e, t, cv, nullptr, pf);
+  face_normals<N, T>(fn, e, t, cv, nullptr, pf, nbr, face_info);
   __syncthreads();  // pair terms consumed; the buffer now takes the face fluxes
-  face_fluxes_to_lds<N, T>(q, e, t, inv_w, u, pf, fn, surf, snc, bg, s_w, s_pb, s_fl);
+  face_fluxes_to_lds<N, T>(q, e, t, inv_w, u, pf, fn, nbr, face_info, surf, snc, bg, s_w, s_pb,
+                           s_fl);
   __syncthreads();
   if (!active) return;
   stage_epilogue<N, T, KAHAN>(acc, acc_nc, s_fl, ti, tj, tk, gnode, u, out, kreg, invJ, mode,
@@ -719,6 +758,9 @@ void e3_stage_direct(
   constexpr bool MIXED = sizeof(T) == 4;
   constexpr int NW = NodeFields<VOL, NC, T>::NW;
   constexpr int BUF = 9 * N3 > 36 * N2 ? 9 * N3 : 36 * N2;
+  // FP64 flux_kennedy_gruber (with or without a gravity term) takes the
+  // sum form below
+  constexpr bool KG_SUMS = !MIXED && VOL == 1;
   __shared__ T s_w[NW * N3];
   __shared__ double s_phi[MIXED ? N3 : 1];
   __shared__ double s_pb[MIXED ? N3 : 1];
@@ -760,27 +802,62 @@ void e3_stage_direct(
         T dphi;
         if constexpr (MIXED) dphi = T(s_phi[pn] - phi_own);
         else dphi = r.phi - l.phi;
-        const T n0 = T(0.5) * (jo0 + ja[pn]);
-        const T n1 = T(0.5) * (jo1 + ja[N3 + pn]);
-        const T n2 = T(0.5) * (jo2 + ja[2 * N3 + pn]);
-        T f[5], g[5];
-        pair_flux<VOL, NC, T>(q, l, r, n0, n1, n2, dphi, bg, f, g);
         const T ds = s_ds[own_d * N + m];
+        if constexpr (KG_SUMS) {
+          // flux_kennedy_gruber and the gravity term from sums of the two
+          // nodes' values: every average's factor 1/2 is a power of two, so
+          // accumulating 8 f_rho, 16 f_m, 16 f_E, 2 g_m and 4 g_E and scaling
+          // once at the end rounds exactly as the averaged form
+          const T rs = l.rho + r.rho, ps = l.p + r.p, es = l.e + r.e;
+          const T v1s = l.v1 + r.v1, v2s = l.v2 + r.v2, v3s = l.v3 + r.v3;
+          const T n0s = jo0 + ja[pn], n1s = jo1 + ja[N3 + pn], n2s = jo2 + ja[2 * N3 + pn];
+          const T vns = v1s * n0s + v2s * n1s + v3s * n2s;
+          const T F = rs * vns;
+          const T ps4 = T(4) * ps;
+          acc[0].add(ds * F);
+          acc[1].add(ds * (F * v1s + ps4 * n0s));
+          acc[2].add(ds * (F * v2s + ps4 * n1s));
+          acc[3].add(ds * (F * v3s + ps4 * n2s));
+          acc[4].add(ds * (F * es + (T(2) * ps) * vns));
+          if constexpr (NC != 0) {
+            const T ra = NC == 1 ? ln_mean_logs(l.rho, r.rho, l.lrho, r.lrho) : T(0.5) * rs;
+            const T sj = ra * dphi;
+            acc_nc[0].add(ds * (n0s * sj));
+            acc_nc[1].add(ds * (n1s * sj));
+            acc_nc[2].add(ds * (n2s * sj));
+            acc_nc[3].add(ds * (sj * vns));
+          }
+        } else {
+          const T n0 = T(0.5) * (jo0 + ja[pn]);
+          const T n1 = T(0.5) * (jo1 + ja[N3 + pn]);
+          const T n2 = T(0.5) * (jo2 + ja[2 * N3 + pn]);
+          T f[5], g[5];
+          pair_flux<VOL, NC, T>(q, l, r, n0, n1, n2, dphi, bg, f, g);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) acc[v].add(ds * f[v]);
-        if constexpr (NC != 0) {
-          acc_nc[0].add(ds * g[0]);
-          acc_nc[1].add(ds * g[1]);
-          acc_nc[2].add(ds * g[2]);
-          acc_nc[3].add(ds * g[3]);
+          for (int v = 0; v < 5; ++v) acc[v].add(ds * f[v]);
+          if constexpr (NC != 0) {
+            acc_nc[0].add(ds * g[0]);
+            acc_nc[1].add(ds * g[1]);
+            acc_nc[2].add(ds * g[2]);
+            acc_nc[3].add(ds * g[3]);
+          }
         }
       }
     }
+    if constexpr (KG_SUMS) {
+      constexpr T sc[5] = {T(0.125), T(0.0625), T(0.0625), T(0.0625), T(0.0625)};
+      constexpr T scn[4] = {T(0.5), T(0.5), T(0.5), T(0.25)};
+#pragma unroll
+      for (int v = 0; v < 5; ++v) acc[v].scale(sc[v]);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc_nc[v].scale(scn[v]);
+    }
   }
   FaceNormals<N, T> fn;
-  face_normals<N, T>(fn, e, t, cv, s_buf, pf);
+  face_normals<N, T>(fn, e, t, cv, s_buf, pf, nbr, face_info);
   __syncthreads();  // contravariant vectors consumed; the buffer takes the face fluxes
-  face_fluxes_to_lds<N, T>(q, e, t, inv_w, u, pf, fn, surf, snc, bg, s_w, s_pb, s_buf);
+  face_fluxes_to_lds<N, T>(q, e, t, inv_w, u, pf, fn, nbr, face_info, surf, snc, bg, s_w, s_pb,
+                           s_buf);
   __syncthreads();
   if (!active) return;
   stage_epilogue<N, T, KAHAN>(acc, acc_nc, s_buf, ti, tj, tk, gnode, u, out, kreg, invJ, mode,
