@@ -67,6 +67,18 @@ __device__ __forceinline__ T ln_mean_logs(T x, T y, T lx, T ly) {
   return div_(ser ? s : d, ser ? q : ly - lx);
 }
 
+// the same value with a branch instead of selects, for callers whose lanes
+// mostly agree on the branch (the pairs of one line direction of an element)
+template <typename T>
+__device__ __forceinline__ T ln_mean_logs_branch(T x, T y, T lx, T ly) {
+  const T d = y - x, s = x + y;
+  if (d * d < T(1.0e-4) * (s * s)) {
+    const T f2 = (d * d) * rcp_refined(s * s);
+    return div_(s, T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0))));
+  }
+  return div_(d, ly - lx);
+}
+
 template <typename T>
 __device__ __forceinline__ T inv_ln_mean_logs(T x, T y, T lx_minus_ly) {
   const T d = y - x, s = x + y;

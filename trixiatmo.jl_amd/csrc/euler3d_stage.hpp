@@ -820,7 +820,7 @@ void e3_stage_direct(
           acc[3].add(ds * (F * v3s + ps4 * n2s));
           acc[4].add(ds * (F * es + (T(2) * ps) * vns));
           if constexpr (NC != 0) {
-            const T ra = NC == 1 ? ln_mean_logs(l.rho, r.rho, l.lrho, r.lrho) : T(0.5) * rs;
+            const T ra = NC == 1 ? ln_mean_logs_branch(l.rho, r.rho, l.lrho, r.lrho) : T(0.5) * rs;
             const T sj = ra * dphi;
             acc_nc[0].add(ds * (n0s * sj));
             acc_nc[1].add(ds * (n1s * sj));
