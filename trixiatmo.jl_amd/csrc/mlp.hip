@@ -54,14 +54,14 @@ __global__ __launch_bounds__(64) void mlp_loss_grad_kernel(const float* params, 
     float a = sP[ob1 + j];
 #pragma unroll
     for (int k = 0; k < I; ++k) a += sP[oW1 + j * I + k] * xi[k];
-    h1[j] = tanhf(a);
+    h1[j] = mlp_tanh(a);
   }
 #pragma unroll 4
   for (int j = 0; j < H; ++j) {
     float a = sP[ob2 + j];
 #pragma unroll
     for (int k = 0; k < H; ++k) a += sP[oW2 + j * H + k] * h1[k];
-    h2[j] = tanhf(a);
+    h2[j] = mlp_tanh(a);
   }
   float dy[O], loss = 0.0f;
 #pragma unroll
@@ -188,14 +188,14 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const float* params, c
     float a = sP[ob1 + j];
 #pragma unroll
     for (int k = 0; k < I; ++k) a += sP[oW1 + j * I + k] * xi[k];
-    h1[j] = tanhf(a);
+    h1[j] = mlp_tanh(a);
   }
 #pragma unroll 4
   for (int j = 0; j < H; ++j) {
     float a = sP[ob2 + j];
 #pragma unroll
     for (int k = 0; k < H; ++k) a += sP[oW2 + j * H + k] * h1[k];
-    h2[j] = tanhf(a);
+    h2[j] = mlp_tanh(a);
   }
 #pragma unroll
   for (int o = 0; o < O; ++o) {

@@ -475,14 +475,14 @@ __device__ __forceinline__ void mlp_eval(const float* P, const float* norm, cons
     float a = b1[j];
 #pragma unroll
     for (int i = 0; i < kMlpIn; ++i) a += W1[j * kMlpIn + i] * x[i];
-    h1[j] = tanhf(a);
+    h1[j] = mlp_tanh(a);
   }
 #pragma unroll 4
   for (int j = 0; j < kMlpHidden; ++j) {
     float a = b2[j];
 #pragma unroll
     for (int i = 0; i < kMlpHidden; ++i) a += W2[j * kMlpHidden + i] * h1[i];
-    h2[j] = tanhf(a);
+    h2[j] = mlp_tanh(a);
   }
 #pragma unroll
   for (int o = 0; o < kMlpOut; ++o) {

@@ -13,6 +13,13 @@ constexpr int kMlpIn = 3, kMlpHidden = 32, kMlpOut = 2;
 constexpr int kMlpParams = kMlpHidden * kMlpIn + kMlpHidden + kMlpHidden * kMlpHidden +
                            kMlpHidden + kMlpOut * kMlpHidden + kMlpOut;  // 1250
 
+// tanh of the network's hidden layers: 1 - 2 / (1 + e^(2x)) from the
+// hardware base-2 exponential and reciprocal (absolute error ~1e-7, ample for
+// a seed; the limits +-1 come out exactly), used in training and inference
+__device__ __forceinline__ float mlp_tanh(float x) {
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(2.8853900817779268f * x));
+}
+
 struct MlpView {
   const float* params;  // kMlpParams
   const float* norm;    // in_mean[3], in_scale[3], out_mean[2], out_scale[2]
