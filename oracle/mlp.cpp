@@ -70,3 +70,28 @@ extern "C" void ora_mlp_loss_grad(const float* params, const float* norm, const 
   *loss = L;
   for (int i = 0; i < NP; ++i) grad[i] = G[i];
 }
+
+// the network's outputs y[2 n] in physical units (FP64 forward of FP32 parameters)
+extern "C" void ora_mlp_forward(const float* params, const float* norm, const float* x, long n,
+                                double* y) {
+  std::vector<double> P(params, params + NP);
+  for (long s = 0; s < n; ++s) {
+    double xi[I], h1[H], h2[H];
+    for (int k = 0; k < I; ++k) xi[k] = ((double)x[3 * s + k] - norm[k]) / norm[3 + k];
+    for (int j = 0; j < H; ++j) {
+      double a = P[ob1 + j];
+      for (int k = 0; k < I; ++k) a += P[oW1 + j * I + k] * xi[k];
+      h1[j] = std::tanh(a);
+    }
+    for (int j = 0; j < H; ++j) {
+      double a = P[ob2 + j];
+      for (int k = 0; k < H; ++k) a += P[oW2 + j * H + k] * h1[k];
+      h2[j] = std::tanh(a);
+    }
+    for (int o = 0; o < O; ++o) {
+      double a = P[ob3 + o];
+      for (int k = 0; k < H; ++k) a += P[oW3 + o * H + k] * h2[k];
+      y[2 * s + o] = a * norm[8 + o] + norm[6 + o];
+    }
+  }
+}

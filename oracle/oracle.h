@@ -186,6 +186,7 @@ void ora_rainy_integrals(const ora_rainy_params* p, const double* u, const doubl
  *      FP64 loss and gradient for FP32 parameters and samples ---- */
 void ora_mlp_loss_grad(const float* params, const float* norm, const float* x, const float* y,
                        long n, double* loss, double* grad);
+void ora_mlp_forward(const float* params, const float* norm, const float* x, long n, double* y);
 
 #ifdef __cplusplus
 }

@@ -212,6 +212,17 @@ def test_mlp_gradient_matches_oracle(oracle):
     o = out.cpu().numpy()
     mse = (((o - y) / np.array(norm_out[1], dtype=np.float32)) ** 2).sum(axis=1).mean()
     np.testing.assert_allclose(mse, loss2, rtol=1e-3)
+    # the forward pass (the seed's wave-wide evaluation, hidden layer on the
+    # matrix cores) per sample against the FP64 oracle's forward
+    p2 = mlp.params()
+    ref = np.zeros((x.shape[0], 2))
+    oracle.L.ora_mlp_forward(p2.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                             mlp.norm.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                             x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                             ctypes.c_long(x.shape[0]), _d(ref))
+    ferr = np.abs(o - ref).max(axis=0) / np.array(norm_out[1])
+    print("forward max error in output scales", ferr, flush=True)
+    assert np.all(ferr <= 1e-5), ferr
 
 
 def test_seeded_newton_converges_to_the_same_states():

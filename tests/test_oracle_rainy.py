@@ -284,3 +284,28 @@ def test_mlp_oracle_gradient_finite_differences(oracle):
         pm[idx] -= h
         fd = (loss_grad(pp)[0] - loss_grad(pm)[0]) / (float(pp[idx]) - float(pm[idx]))
         assert abs(fd - g[idx]) <= 2e-3 * abs(g).max(), (idx, fd, g[idx])
+
+
+def test_mlp_oracle_forward_matches_its_loss(oracle):
+    """the oracle's forward outputs give back the oracle's loss (the forward
+    that checks the GPU network is the one the gradient test pins)"""
+    import ctypes
+    from oracle_lib import _d
+    rng = np.random.default_rng(1)
+    p = (0.3 * rng.standard_normal(1250)).astype(np.float32)
+    norm = np.array([1.0, 2.3e5, 1e-2, 0.1, 2e4, 5e-3, 260.0, 5e-3, 20.0, 5e-3], dtype=np.float32)
+    x = np.stack([rng.uniform(0.8, 1.2, 200), rng.uniform(2e5, 2.6e5, 200),
+                  rng.uniform(1e-3, 2e-2, 200)], axis=1).astype(np.float32)
+    y = np.stack([rng.uniform(240, 290, 200), rng.uniform(1e-3, 1e-2, 200)], axis=1).astype(np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    L = ctypes.c_double()
+    oracle.L.ora_mlp_loss_grad(p.ctypes.data_as(fp), norm.ctypes.data_as(fp), x.ctypes.data_as(fp),
+                               y.ctypes.data_as(fp), ctypes.c_long(200), ctypes.byref(L),
+                               _d(np.zeros(1250)))
+    out = np.zeros((200, 2))
+    oracle.L.ora_mlp_forward(p.ctypes.data_as(fp), norm.ctypes.data_as(fp), x.ctypes.data_as(fp),
+                             ctypes.c_long(200), _d(out))
+    scale = norm[8:].astype(np.float64)
+    yt = (y.astype(np.float64) - norm[6:8]) / scale
+    yo = (out - norm[6:8]) / scale
+    np.testing.assert_allclose(((yo - yt) ** 2).sum(axis=1).mean(), L.value, rtol=1e-12)

@@ -172,37 +172,23 @@ __global__ void mlp_adam_kernel(float* params, float* m, float* v, const float* 
   params[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
 }
 
+// y[2 n] = network(x[3 n]) in physical units, by the same wave-wide
+// evaluation (hidden layer on the matrix cores) that seeds the saturation
+// solve; lanes past n evaluate the normalisation means and write nothing
 __global__ __launch_bounds__(256) void mlp_forward_kernel(const float* params, const float* norm,
                                                           const float* x, float* y, int64_t n) {
   __shared__ float sP[P];
   for (int i = threadIdx.x; i < P; i += blockDim.x) sP[i] = params[i];
   __syncthreads();
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  float xi[I];
+  const bool valid = q < n;
+  float xi[I], yo[O];
 #pragma unroll
-  for (int k = 0; k < I; ++k) xi[k] = (x[3 * q + k] - norm[k]) / norm[3 + k];
-  float h1[H], h2[H];
+  for (int k = 0; k < I; ++k) xi[k] = valid ? x[3 * q + k] : norm[k];
+  mlp_eval_wave(sP, norm, xi, yo);
+  if (valid) {
 #pragma unroll
-  for (int j = 0; j < H; ++j) {
-    float a = sP[ob1 + j];
-#pragma unroll
-    for (int k = 0; k < I; ++k) a += sP[oW1 + j * I + k] * xi[k];
-    h1[j] = mlp_tanh(a);
-  }
-#pragma unroll 4
-  for (int j = 0; j < H; ++j) {
-    float a = sP[ob2 + j];
-#pragma unroll
-    for (int k = 0; k < H; ++k) a += sP[oW2 + j * H + k] * h1[k];
-    h2[j] = mlp_tanh(a);
-  }
-#pragma unroll
-  for (int o = 0; o < O; ++o) {
-    float a = sP[ob3 + o];
-#pragma unroll
-    for (int k = 0; k < H; ++k) a += sP[oW3 + o * H + k] * h2[k];
-    y[2 * q + o] = a * norm[8 + o] + norm[6 + o];
+    for (int o = 0; o < O; ++o) y[2 * q + o] = yo[o];
   }
 }
 

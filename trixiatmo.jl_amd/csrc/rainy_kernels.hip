@@ -457,42 +457,6 @@ __device__ __forceinline__ void accumulate_stats(unsigned long long* stats, int 
 
 }  // namespace
 
-// MLP forward for one input (FP32, tanh hidden layers), normalised in/out
-__device__ __forceinline__ void mlp_eval(const float* P, const float* norm, const float* xin,
-                                         float* y) {
-  const float* W1 = P;
-  const float* b1 = W1 + kMlpHidden * kMlpIn;
-  const float* W2 = b1 + kMlpHidden;
-  const float* b2 = W2 + kMlpHidden * kMlpHidden;
-  const float* W3 = b2 + kMlpHidden;
-  const float* b3 = W3 + kMlpOut * kMlpHidden;
-  float x[kMlpIn];
-#pragma unroll
-  for (int i = 0; i < kMlpIn; ++i) x[i] = (xin[i] - norm[i]) / norm[3 + i];
-  float h1[kMlpHidden], h2[kMlpHidden];
-#pragma unroll
-  for (int j = 0; j < kMlpHidden; ++j) {
-    float a = b1[j];
-#pragma unroll
-    for (int i = 0; i < kMlpIn; ++i) a += W1[j * kMlpIn + i] * x[i];
-    h1[j] = mlp_tanh(a);
-  }
-#pragma unroll 4
-  for (int j = 0; j < kMlpHidden; ++j) {
-    float a = b2[j];
-#pragma unroll
-    for (int i = 0; i < kMlpHidden; ++i) a += W2[j * kMlpHidden + i] * h1[i];
-    h2[j] = mlp_tanh(a);
-  }
-#pragma unroll
-  for (int o = 0; o < kMlpOut; ++o) {
-    float a = b3[o];
-#pragma unroll
-    for (int i = 0; i < kMlpHidden; ++i) a += W3[o * kMlpHidden + i] * h2[i];
-    y[o] = a * norm[8 + o] + norm[6 + o];
-  }
-}
-
 // network inputs of one state: (rho, internal energy density, total water)
 __device__ __forceinline__ void mlp_inputs(const double* u, float* x) {
   const double rho = u[0] + u[1] + u[2];
@@ -503,16 +467,25 @@ __device__ __forceinline__ void mlp_inputs(const double* u, float* x) {
   x[2] = (float)(u[1] + u[2]);
 }
 
-// seeded guess of one node: the network's q_v splits rho_moist into vapour
-// and cloud (all vapour when the prediction is within 1 % of it: the
-// unsaturated branch), and T then follows from the energy row of the
-// saturation residual, which is linear in T, so the guess satisfies the
-// energy and total-water rows exactly and Newton only has the saturation row
-// left to correct; returns false when the guess is unusable
-__device__ __forceinline__ bool mlp_guess(const float* P, const double* u, double* x) {
-  float xi[kMlpIn], y[kMlpOut];
-  mlp_inputs(u, xi);
-  mlp_eval(P, P + kMlpParams, xi, y);
+// network inputs of the lanes of a wave: the node's state where `valid`,
+// else the normalisation means (a harmless input for lanes without a node)
+__device__ __forceinline__ void mlp_inputs_or_mean(const double* u, bool valid, const float* norm,
+                                                   float* x) {
+  if (valid) {
+    mlp_inputs(u, x);
+  } else {
+#pragma unroll
+    for (int i = 0; i < kMlpIn; ++i) x[i] = norm[i];
+  }
+}
+
+// seeded guess of one node from the network's prediction y = (T, q_v): q_v
+// splits rho_moist into vapour and cloud (all vapour when the prediction is
+// within 1 % of it: the unsaturated branch), and T then follows from the
+// energy row of the saturation residual, which is linear in T, so the guess
+// satisfies the energy and total-water rows exactly and Newton only has the
+// saturation row left to correct; returns false when the guess is unusable
+__device__ __forceinline__ bool mlp_seed(const double* u, const float* y, double* x) {
   const double rho = u[0] + u[1] + u[2];
   const double inv = 1.0 / rho;
   const double v1 = u[3] * inv, v2 = u[4] * inv;
@@ -537,7 +510,6 @@ void rainy_stage(RainyLaunch L) {
   // contravariant fluxes of the 6 flux rows [2][6][N2]; flux differencing: the
   // per-node quantities of flux_ec_rain [11][N2]) and for the face fluxes [9][4 n]
   __shared__ double s_buf[EPB][VOLW];
-  extern __shared__ float s_mlp[];  // network parameters + normalisation when seeding
   auto s_f = [&](int es, int o, int v, int q) -> double& { return s_buf[es][(o * 6 + v) * N2 + q]; };
   auto s_ec = [&](int es, int k, int q) -> double& { return s_buf[es][k * N2 + q]; };
   auto s_sf = [&](int es, int v, int r) -> double& { return s_buf[es][v * S::NF + r]; };
@@ -548,11 +520,9 @@ void rainy_stage(RainyLaunch L) {
     const int es = idx / (N2 * NV), r = idx - es * N2 * NV;
     if (e0 + es < L.n_elements) s_u[es][r % NV][r / NV] = L.u[(size_t)(e0 + es) * N2 * NV + r];
   }
+  // the network's parameters are read from memory where used: the reads are
+  // uniform over the wave (cached), cheaper than staging them per workgroup
   const bool seeded = SEEDED && L.mode == 1 && L.newton_max_iter > 0;
-  if (seeded) {
-    for (int i = t; i < kMlpParams; i += B) s_mlp[i] = L.mlp.params[i];
-    if (t < 10) s_mlp[kMlpParams + t] = L.mlp.norm[t];
-  }
   __syncthreads();
 
   const int es = t / N2, q = t - es * N2;
@@ -686,6 +656,9 @@ void rainy_stage(RainyLaunch L) {
   }
   __syncthreads();
   int it_stat = 0;
+  double un[NV];  // the RK-updated state (stage mode)
+#pragma unroll
+  for (int v = 0; v < NV; ++v) un[v] = 0.0;
   if (active) {
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
@@ -719,24 +692,38 @@ void rainy_stage(RainyLaunch L) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) du[v] += sv[v];
     }
-    const size_t g = ((size_t)e * N2 + q) * NV;
     if (L.mode == 0) {
+      const size_t g = ((size_t)e * N2 + q) * NV;
 #pragma unroll
       for (int v = 0; v < NV; ++v) L.out[g + v] = du[v];
     } else {
-      double un[NV];
+      const size_t g = ((size_t)e * N2 + q) * NV;
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const double kn = L.rk_a * L.k[g + v] + L.rk_dt * du[v];
         L.k[g + v] = kn;
         un[v] = uo[v] + L.rk_b * kn;
       }
-      // the stage limiter: NonlinearSolveDG on the updated state
+    }
+  }
+  // the network's prediction for the updated states: every lane of the wave
+  float ypred[kMlpOut] = {0.0f, 0.0f};
+  if constexpr (SEEDED) {
+    if (seeded) {
+      float xi[kMlpIn];
+      mlp_inputs_or_mean(un, active, L.mlp.norm, xi);
+      mlp_eval_wave(L.mlp.params, L.mlp.norm, xi, ypred);
+    }
+  }
+  if (active && L.mode != 0) {
+    const size_t g = ((size_t)e * N2 + q) * NV;
+    // the stage limiter: NonlinearSolveDG on the updated state
+    {
       if (L.newton_max_iter > 0) {
         double x[3] = {un[6], un[7], un[8]};
         if constexpr (SEEDED) {
           // network seed; a failed seeded solve is redone from u[7:9]
-          const bool ok = mlp_guess(s_mlp, un, x);
+          const bool ok = mlp_seed(un, ypred, x);
           it_stat = ok ? newton_node(un, x, L.newton_tol, L.newton_max_iter) : -1;
           if (it_stat < 0 || !(x[2] > 0.0)) {
             const int used = ok ? L.newton_max_iter : 0;
@@ -853,24 +840,26 @@ __global__ __launch_bounds__(256) void rainy_newton_kernel(RainyLaunch L, double
                                                            MlpView mlp, int use_mlp,
                                                            int* iterations,
                                                            unsigned long long* stats) {
-  __shared__ float sP[kMlpParams + 10];
-  if (use_mlp) {
-    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) sP[i] = mlp.params[i];
-    if (threadIdx.x < 10) sP[kMlpParams + threadIdx.x] = mlp.norm[threadIdx.x];
-    __syncthreads();
-  }
   const size_t nodes = (size_t)L.n_elements * L.n * L.n;
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   int it = 0;
-  if (q < nodes) {
-    double* un = u + q * NV;
-    double uc[NV];
+  const bool valid = q < nodes;
+  double uc[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) uc[v] = un[v];
+  for (int v = 0; v < NV; ++v) uc[v] = valid ? u[q * NV + v] : 0.0;
+  // the network's prediction: every lane of the wave takes part
+  float ypred[kMlpOut] = {0.0f, 0.0f};
+  if (use_mlp) {
+    float xi[kMlpIn];
+    mlp_inputs_or_mean(uc, valid, mlp.norm, xi);
+    mlp_eval_wave(mlp.params, mlp.norm, xi, ypred);
+  }
+  if (valid) {
+    double* un = u + q * NV;
     double x[3] = {uc[6], uc[7], uc[8]};
     int fallback = 0;
     if (use_mlp) {
-      const bool ok = mlp_guess(sP, uc, x);
+      const bool ok = mlp_seed(uc, ypred, x);
       it = ok ? newton_node(uc, x, L.newton_tol, L.newton_max_iter) : -1;
       if (it < 0 || !(x[2] > 0.0)) {
         // fall back to the default guess, as the unseeded solve
@@ -929,8 +918,7 @@ bool rainy_launch_stage(const RainyLaunch& L, hipStream_t s) {
   case NN: {                                                                                  \
     const int grid = (L.n_elements + RShape<NN>::EPB - 1) / RShape<NN>::EPB;                  \
     if (L.mlp.params && L.mode == 1 && L.newton_max_iter > 0)                                 \
-      hipLaunchKernelGGL((rainy_stage<NN, true>), dim3(grid), dim3(RShape<NN>::BLOCK),          \
-                         sizeof(float) * (kMlpParams + 10), s, L);                              \
+      hipLaunchKernelGGL((rainy_stage<NN, true>), dim3(grid), dim3(RShape<NN>::BLOCK), 0, s, L); \
     else                                                                                        \
       hipLaunchKernelGGL((rainy_stage<NN, false>), dim3(grid), dim3(RShape<NN>::BLOCK), 0, s, L); \
     return true;                                                                              \

@@ -20,6 +20,67 @@ __device__ __forceinline__ float mlp_tanh(float x) {
   return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(2.8853900817779268f * x));
 }
 
+typedef float mlp_floatx16 __attribute__((ext_vector_type(16)));
+
+// MLP forward for the 64 nodes of a wave, one per lane (FP32, tanh hidden
+// layers, normalised inputs and outputs); every lane of the wave must take
+// part.  The first and the output layer run on the VALU; the 32 x 32 hidden
+// layer runs on the matrix cores as Z = W2 H1 for two blocks of 32 nodes
+// (v_mfma_f32_32x32x2f32: A[i][k] from lane i + 32 k, B[k][j] from lane
+// j + 32 k, lane l receiving rows 8 (r / 4) + 4 (l / 32) + r % 4 of column
+// l % 32 in its 16 results r), each lane's h1 values reaching the other half
+// of the wave through one lane swap per step.
+__device__ __forceinline__ void mlp_eval_wave(const float* P, const float* norm, const float* xin,
+                                              float* y) {
+  const float* W1 = P;
+  const float* b1 = W1 + kMlpHidden * kMlpIn;
+  const float* W2 = b1 + kMlpHidden;
+  const float* b2 = W2 + kMlpHidden * kMlpHidden;
+  const float* W3 = b2 + kMlpHidden;
+  const float* b3 = W3 + kMlpOut * kMlpHidden;
+  const int l = threadIdx.x & 63, hi = l >> 5, lo = l & 31;
+  float x[kMlpIn];
+#pragma unroll
+  for (int i = 0; i < kMlpIn; ++i) x[i] = (xin[i] - norm[i]) / norm[3 + i];
+  float h1[kMlpHidden];
+#pragma unroll
+  for (int j = 0; j < kMlpHidden; ++j) {
+    float a = b1[j];
+#pragma unroll
+    for (int i = 0; i < kMlpIn; ++i) a += W1[j * kMlpIn + i] * x[i];
+    h1[j] = mlp_tanh(a);
+  }
+  mlp_floatx16 d0 = {}, d1 = {};
+#pragma unroll
+  for (int s = 0; s < kMlpHidden / 2; ++s) {
+    const float a = W2[lo * kMlpHidden + 2 * s + hi];
+    const float give = hi ? h1[2 * s] : h1[2 * s + 1];
+    const float recv = __shfl_xor(give, 32, 64);
+    const float bb0 = hi ? recv : h1[2 * s];       // h1[2 s + hi] of node lo
+    const float bb1 = hi ? h1[2 * s + 1] : recv;   // h1[2 s + hi] of node 32 + lo
+    d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bb0, d0, 0, 0, 0);
+    d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bb1, d1, 0, 0, 0);
+  }
+  // output layer: each lane sums its 16 hidden rows, the lane 32 apart the others
+  float p0[kMlpOut] = {0.0f, 0.0f}, p1[kMlpOut] = {0.0f, 0.0f};
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = 8 * (r / 4) + 4 * hi + r % 4;
+    const float g0 = mlp_tanh(d0[r] + b2[row]), g1 = mlp_tanh(d1[r] + b2[row]);
+#pragma unroll
+    for (int o = 0; o < kMlpOut; ++o) {
+      p0[o] += W3[o * kMlpHidden + row] * g0;
+      p1[o] += W3[o * kMlpHidden + row] * g1;
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < kMlpOut; ++o) {
+    const float t0 = p0[o] + __shfl_xor(p0[o], 32, 64);  // node lo
+    const float t1 = p1[o] + __shfl_xor(p1[o], 32, 64);  // node 32 + lo
+    y[o] = ((hi ? t1 : t0) + b3[o]) * norm[8 + o] + norm[6 + o];
+  }
+}
+
 struct MlpView {
   const float* params;  // kMlpParams
   const float* norm;    // in_mean[3], in_scale[3], out_mean[2], out_scale[2]
