@@ -758,9 +758,9 @@ void e3_stage_direct(
   constexpr bool MIXED = sizeof(T) == 4;
   constexpr int NW = NodeFields<VOL, NC, T>::NW;
   constexpr int BUF = 9 * N3 > 36 * N2 ? 9 * N3 : 36 * N2;
-  // FP64 flux_kennedy_gruber (with or without a gravity term) takes the
-  // sum form below
-  constexpr bool KG_SUMS = !MIXED && VOL == 1;
+  // flux_kennedy_gruber (with or without a gravity term) takes the sum form
+  // below
+  constexpr bool KG_SUMS = VOL == 1;
   __shared__ T s_w[NW * N3];
   __shared__ double s_phi[MIXED ? N3 : 1];
   __shared__ double s_pb[MIXED ? N3 : 1];
@@ -813,7 +813,12 @@ void e3_stage_direct(
           const T n0s = jo0 + ja[pn], n1s = jo1 + ja[N3 + pn], n2s = jo2 + ja[2 * N3 + pn];
           const T vns = v1s * n0s + v2s * n1s + v3s * n2s;
           const T F = rs * vns;
-          const T ps4 = T(4) * ps;
+          // the momentum rows carry p' = p - p_bg when the background is split off
+          T pms = ps;
+          if constexpr (MIXED) {
+            if (bg.on) pms = l.pp + r.pp;
+          }
+          const T ps4 = T(4) * pms;
           acc[0].add(ds * F);
           acc[1].add(ds * (F * v1s + ps4 * n0s));
           acc[2].add(ds * (F * v2s + ps4 * n1s));
@@ -822,9 +827,20 @@ void e3_stage_direct(
           if constexpr (NC != 0) {
             const T ra = NC == 1 ? ln_mean_logs_branch(l.rho, r.rho, l.lrho, r.lrho) : T(0.5) * rs;
             const T sj = ra * dphi;
-            acc_nc[0].add(ds * (n0s * sj));
-            acc_nc[1].add(ds * (n1s * sj));
-            acc_nc[2].add(ds * (n2s * sj));
+            T sm = sj;  // momentum part: (rho_avg - rho_avg of the background) [phi]
+            if constexpr (MIXED) {
+              if (bg.on) {
+                const T Ap = r.rp - l.rp, sp = l.rp + r.rp;
+                const T Ab = r.rb - l.rb, sb = l.rb + r.rb;
+                const T dmean = (NC == 1) ? delta_ln_mean(Ap + Ab, sp + sb, Ab, sb, Ap, sp,
+                                                          -dphi * T(bg.inv_RT0), r.lam - l.lam)
+                                          : T(0.5) * sp;
+                sm = dmean * dphi;
+              }
+            }
+            acc_nc[0].add(ds * (n0s * sm));
+            acc_nc[1].add(ds * (n1s * sm));
+            acc_nc[2].add(ds * (n2s * sm));
             acc_nc[3].add(ds * (sj * vns));
           }
         } else {
