@@ -173,12 +173,19 @@ typedef struct {
   const int32_t* boundary_condition;     /* [B] ATMO_BC_* */
 } atmo_geometry3d;
 
-/* covariant shallow water: fluxes */
+/* covariant shallow water: fluxes.  SplitCovariantShallowWaterEquations2D
+ * takes ATMO_SWE_VOL_EC / _CENTRAL with ATMO_SWE_SURF_EC_PLUS_LLF /
+ * _LAX_FRIEDRICHS / _EC and source_terms 0 or 1; the standard
+ * CovariantShallowWaterEquations2D (covariant_shallow_water.jl) takes
+ * ATMO_SWE_VOL_WEAK_FORM with ATMO_SWE_SURF_LAX_FRIEDRICHS_STANDARD and
+ * source_terms 0 or 2 (surface_noncons is then ignored). */
 #define ATMO_SWE_VOL_EC 0                  /* flux_ec */
 #define ATMO_SWE_VOL_CENTRAL 1             /* flux_central */
+#define ATMO_SWE_VOL_WEAK_FORM 2           /* VolumeIntegralWeakForm() of the standard form */
 #define ATMO_SWE_SURF_EC_PLUS_LLF 0        /* FluxPlusDissipation(flux_ec, DissipationLocalLaxFriedrichs()) */
 #define ATMO_SWE_SURF_LAX_FRIEDRICHS 1     /* flux_lax_friedrichs */
 #define ATMO_SWE_SURF_EC 2                 /* flux_ec */
+#define ATMO_SWE_SURF_LAX_FRIEDRICHS_STANDARD 3 /* flux_lax_friedrichs of the standard form */
 #define ATMO_SWE_NC_SURFACE_SIMPLIFIED 0   /* flux_nonconservative_surface_simplified */
 #define ATMO_SWE_NC_EC 1                   /* flux_nonconservative_ec */
 
@@ -188,9 +195,9 @@ typedef struct {
   int32_t volume_flux;      /* ATMO_SWE_VOL_*; the volume nonconservative term is flux_nonconservative_ec */
   int32_t surface_flux;     /* ATMO_SWE_SURF_* */
   int32_t surface_noncons;  /* ATMO_SWE_NC_* */
-  int32_t source_terms;     /* 1: source_terms_geometric_coriolis */
+  int32_t source_terms;     /* 1: source_terms_geometric_coriolis, 2: the same of the standard form */
   int32_t precision;        /* ATMO_PREC_FP64 */
-  double gravity, rotation_rate; /* SplitCovariantShallowWaterEquations2D(gravity, rotation_rate) */
+  double gravity, rotation_rate; /* (Split)CovariantShallowWaterEquations2D(gravity, rotation_rate) */
 } atmo_swe_config;
 
 typedef struct {

@@ -1,9 +1,11 @@
 // CPU ORACLE -- test infrastructure only (never linked by the product).
 //
 // FP64 restatement of the covariant `rhs_hyperbolic!` of TrixiAtmo for
-// SplitCovariantShallowWaterEquations2D on 2D P4est meshes of the sphere:
+// SplitCovariantShallowWaterEquations2D, and for the standard
+// CovariantShallowWaterEquations2D with the weak form (covariant_shallow_water.jl:166-238),
+// on 2D P4est meshes of the sphere:
 //   src/solvers/dgsem_p4est/dg_2d_manifold_in_3d_covariant.jl:8-71 (phases),
-//   :134-234 (flux differencing, symmetric + nonconservative),
+//   :100-130 (weak form), :134-234 (flux differencing, symmetric + nonconservative),
 //   :239-432 (interface flux in the local contravariant bases of both sides),
 //   :435-458 (sources with the auxiliary variables), :461-478 (exact J),
 // with the pointwise physics of src/equations/covariant_shallow_water_split.jl
@@ -92,11 +94,51 @@ double max_abs_speed(double g, const double* ul, const double* ur, const Aux& xl
                    std::fabs(ur[1 + o] / ur[0]) + std::sqrt(xr.Gcon(o, o) * ur[0] * g));
 }
 
+// CovariantShallowWaterEquations2D (the standard, non-split form):
+// covariant_shallow_water.jl:166-187, flux in orientation o with the pressure
+// term G^{io} g h^2 / 2 inside the momentum flux
+void flux_standard(double g, const double* u, const Aux& x, int o, double* f) {
+  const double h = u[0];
+  const double vcon = u[1 + o] / h;
+  const double grav = 0.5 * g * h * h;
+  const double m1 = u[1] * vcon + x.Gcon(0, o) * grav;
+  const double m2 = u[2] * vcon + x.Gcon(1, o) * grav;
+  f[0] = x.J() * u[1 + o];
+  f[1] = x.J() * m1;
+  f[2] = x.J() * m2;
+}
+
+// :215-238, source_terms_geometric_coriolis of the standard form
+void source_standard(double g, double omega, const double* u, const double* x, const Aux& a,
+                     double* s) {
+  const double h = u[0];
+  const double hv[2] = {u[1], u[2]};
+  const double v[2] = {u[1] / h, u[2] / h};
+  const double grav = 0.5 * g * h * h;
+  double tau[2][2];  // h v^i v^j + G^{ij} g h^2 / 2
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) tau[i][j] = hv[i] * v[j] + grav * a.Gcon(i, j);
+  const double r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  const double f = 2 * omega * x[2] / r;
+  double src[2];
+  for (int k = 0; k < 2; ++k) {
+    double sg = 0.0;
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) sg += a.Gam(k, i, j) * tau[i][j];
+    src[k] = sg + f * a.J() * (a.Gcon(k, 1) * hv[0] - a.Gcon(k, 0) * hv[1]);
+  }
+  s[0] = 0.0;
+  s[1] = -src[0];
+  s[2] = -src[1];
+}
+
 struct Fluxes {
   double g;
-  int volume_flux;      // 0 flux_ec, 1 flux_central
-  int surface_flux;     // 0 FluxPlusDissipation(flux_ec, LLF), 1 flux_lax_friedrichs, 2 flux_ec
+  int volume_flux;      // 0 flux_ec, 1 flux_central, 2 weak form of the standard form
+  int surface_flux;     // 0 FluxPlusDissipation(flux_ec, LLF), 1 flux_lax_friedrichs, 2 flux_ec,
+                        // 3 flux_lax_friedrichs of the standard form
   int surface_noncons;  // 0 flux_nonconservative_surface_simplified, 1 flux_nonconservative_ec
+  bool standard() const { return volume_flux == 2; }
 };
 
 void volume_flux(const Fluxes& F, const double* ul, const double* ur, const Aux& xl,
@@ -108,7 +150,12 @@ void volume_flux(const Fluxes& F, const double* ul, const double* ur, const Aux&
 // the surface flux called with (u_ll, u_rr, aux, aux): equations.jl:256-290
 void surface_flux(const Fluxes& F, const double* ul, const double* ur, const Aux& xl,
                   const Aux& xr, int o, double* f) {
-  if (F.surface_flux == 1) flux_central(ul, ur, xl, xr, o, f);
+  if (F.surface_flux == 3) {  // flux_central of the standard flux (equations.jl:273-280)
+    double fl[NV], fr[NV];
+    flux_standard(F.g, ul, xl, o, fl);
+    flux_standard(F.g, ur, xr, o, fr);
+    for (int v = 0; v < NV; ++v) f[v] = 0.5 * (fl[v] + fr[v]);
+  } else if (F.surface_flux == 1) flux_central(ul, ur, xl, xr, o, f);
   else flux_ec(ul, ur, xl, xr, o, f);
   if (F.surface_flux != 2) {
     const double lam = max_abs_speed(F.g, ul, ur, xl, xr, o);
@@ -118,7 +165,9 @@ void surface_flux(const Fluxes& F, const double* ul, const double* ur, const Aux
 
 void surface_noncons(const Fluxes& F, const double* ul, const double* ur, const Aux& xl,
                      const Aux& xr, int o, double* f) {
-  if (F.surface_noncons == 0) noncons_surface_simplified(F.g, ul, ur, xl, o, f);
+  if (F.standard()) {  // the standard form has no nonconservative terms
+    for (int v = 0; v < NV; ++v) f[v] = 0.0;
+  } else if (F.surface_noncons == 0) noncons_surface_simplified(F.g, ul, ur, xl, o, f);
   else noncons_ec(F.g, ul, ur, xl, xr, o, f);
 }
 
@@ -184,9 +233,13 @@ static int swe_rhs_impl(const ora_swe_params* p, const ora_connectivity2d* c, co
   auto M = [mag](double v) { return mag ? std::fabs(v) : v; };
   const int n = p->n, n2 = n * n, E = p->n_elements;
   const Fluxes F{p->gravity, p->volume_flux, p->surface_flux, p->surface_noncons};
-  std::vector<double> x(n), w(n), Ds(n * n);
+  std::vector<double> x(n), w(n), Ds(n * n), D(n * n), dhat(n * n);
   ora_lgl_nodes_weights(n, x.data(), w.data());
   ora_derivative_split(n, x.data(), w.data(), Ds.data());
+  ora_derivative_matrix(n, x.data(), D.data());
+  // Trixi's derivative_hat: dhat[j, m] = -D[m, j] w_m / w_j
+  for (int j = 0; j < n; ++j)
+    for (int m = 0; m < n; ++m) dhat[j * n + m] = -D[m * n + j] * w[m] / w[j];
   std::vector<double> sfv((size_t)NV * n * 4 * E, 0.0);
   auto U = [&](int e, int i, int j) { return u + (size_t)NV * (i + n * (j + (size_t)n * e)); };
   auto AX = [&](int e, int i, int j) { return Aux{aux + (size_t)NA * (i + n * (j + (size_t)n * e))}; };
@@ -195,6 +248,21 @@ static int swe_rhs_impl(const ora_swe_params* p, const ora_connectivity2d* c, co
   for (int e = 0; e < E; ++e) {
     double* due = du + (size_t)NV * n2 * e;
     std::memset(due, 0, sizeof(double) * NV * n2);
+    if (F.standard()) {
+      // weak form (dg_2d_manifold_in_3d_covariant.jl:100-130): node (i, j)'s
+      // contravariant fluxes enter node (ii, j) with dhat[ii, i], (i, jj) with dhat[jj, j]
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          double f1[NV], f2[NV];
+          flux_standard(F.g, U(e, i, j), AX(e, i, j), 0, f1);
+          flux_standard(F.g, U(e, i, j), AX(e, i, j), 1, f2);
+          for (int ii = 0; ii < n; ++ii)
+            for (int v = 0; v < NV; ++v) due[NV * (ii + n * j) + v] += M(dhat[ii * n + i] * f1[v]);
+          for (int jj = 0; jj < n; ++jj)
+            for (int v = 0; v < NV; ++v) due[NV * (i + n * jj) + v] += M(dhat[jj * n + j] * f2[v]);
+        }
+      continue;
+    }
     // symmetric part (dg_2d_manifold_in_3d_covariant.jl:134-183)
     for (int j = 0; j < n; ++j)
       for (int i = 0; i < n; ++i) {
@@ -304,7 +372,9 @@ static int swe_rhs_impl(const ora_swe_params* p, const ora_connectivity2d* c, co
         for (int v = 0; v < NV; ++v) d[v] *= mag ? std::fabs(fac) : fac;
         if (p->source_terms) {
           double sv[NV];
-          source_split(p->rotation_rate, U(e, i, j), xyz + 3 * (i + n * (j + (size_t)n * e)), a, sv);
+          const double* xn = xyz + 3 * (i + n * (j + (size_t)n * e));
+          if (p->source_terms == 2) source_standard(p->gravity, p->rotation_rate, U(e, i, j), xn, a, sv);
+          else source_split(p->rotation_rate, U(e, i, j), xn, a, sv);
           for (int v = 0; v < NV; ++v) d[v] += M(sv[v]);
         }
       }

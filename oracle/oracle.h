@@ -125,11 +125,14 @@ void ora_ck2n54_coefficients(double* A, double* B, double* c);
 typedef struct {
   int n, n_elements;
   double gravity, rotation_rate;
-  int volume_flux;     /* 0 flux_ec, 1 flux_central */
+  int volume_flux;     /* 0 flux_ec, 1 flux_central, 2 VolumeIntegralWeakForm() of
+                          CovariantShallowWaterEquations2D (the standard form) */
   int surface_flux;    /* 0 FluxPlusDissipation(flux_ec, DissipationLocalLaxFriedrichs()),
-                          1 flux_lax_friedrichs, 2 flux_ec */
-  int surface_noncons; /* 0 flux_nonconservative_surface_simplified, 1 flux_nonconservative_ec */
-  int source_terms;    /* 1: source_terms_geometric_coriolis */
+                          1 flux_lax_friedrichs, 2 flux_ec, 3 flux_lax_friedrichs of the
+                          standard form */
+  int surface_noncons; /* 0 flux_nonconservative_surface_simplified, 1 flux_nonconservative_ec
+                          (ignored by the standard form) */
+  int source_terms;    /* 1: source_terms_geometric_coriolis, 2: the same of the standard form */
 } ora_swe_params;
 
 typedef struct {

@@ -209,7 +209,7 @@ class SweOracle:
 
     @classmethod
     def from_parts(cls, oracle, mesh, aux, equations, solver, source_terms):
-        codes = swe_codes(solver, source_terms)
+        codes = swe_codes(equations, solver, source_terms)
         p = SweParams(mesh.nnodes, mesh.n_elements, equations.gravity, equations.rotation_rate,
                       *codes)
         return cls(oracle, p, aux, mesh.node_coordinates, mesh.interface_neighbor_ids,
@@ -263,15 +263,11 @@ class SweOracle:
         return u, steps
 
 
-def swe_codes(solver, source_terms):
+def swe_codes(equations, solver, source_terms):
     """(volume_flux, surface_flux, surface_noncons, source_terms) codes of a
-    covariant DGSEM solver"""
-    from trixiatmo_hip.equations import SWE_SURFACE_FLUXES, SWE_SURFACE_NONCONS, SWE_VOLUME_FLUXES
-    from trixiatmo_hip.semidiscretization import COVARIANT_SOURCE_TERMS
-    vol, _ = solver.volume_integral.volume_flux
-    sf, snc = solver.surface_flux
-    return (SWE_VOLUME_FLUXES[vol], SWE_SURFACE_FLUXES[sf], SWE_SURFACE_NONCONS[snc],
-            COVARIANT_SOURCE_TERMS[source_terms])
+    covariant semidiscretization (the oracle takes the library's codes)"""
+    from trixiatmo_hip.semidiscretization import swe_codes as codes
+    return codes(equations, solver, source_terms)
 
 
 def params_for(semi_args):

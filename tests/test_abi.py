@@ -126,7 +126,11 @@ def test_covariant_invalid_configurations_and_connectivity_rejected():
                 surface_noncons=0, source_terms=1, precision=0, gravity=9.81,
                 rotation_rate=7.292e-5)
     bad = [dict(polydeg=8), dict(volume_flux=2), dict(surface_flux=3), dict(surface_noncons=2),
-           dict(precision=1), dict(gravity=0.0), dict(n_elements=0), dict(source_terms=2)]
+           dict(precision=1), dict(gravity=0.0), dict(n_elements=0), dict(source_terms=2),
+           dict(volume_flux=3), dict(surface_flux=4), dict(source_terms=3),
+           # the standard form's weak form, flux and sources do not mix with the split form's
+           dict(volume_flux=2, surface_flux=3, source_terms=1),
+           dict(volume_flux=2, surface_flux=1, source_terms=2)]
     geo, keep = _swe_geometry(mesh, aux)
     for kw in bad:
         cfg = _lib.AtmoSweConfig(**{**good, **kw})

@@ -31,7 +31,7 @@ def perturbed(case, seed=5, amp=0.01):
 def check_rhs(oracle, case, u):
     semi = case.semi()
     print(f"case: E={semi.shape[-1]} n={semi.shape[1]} surface={case.solver.surface_flux} "
-          f"volume={case.solver.volume_integral.volume_flux}", flush=True)
+          f"volume={case.solver.volume_integral}", flush=True)
     so = SweOracle.for_semi(oracle, semi)
     du_gpu = semi.rhs(None, u)
     du_ref = so.rhs(u)
@@ -60,6 +60,14 @@ def test_rhs_fluxes(oracle, surface, noncons):
 @pytest.mark.parametrize("polydeg", [1, 2, 4, 5, 6, 7])
 def test_rhs_polydeg(oracle, polydeg):
     case = W.zonal_jet(3, polydeg, seed=2)
+    check_rhs(oracle, case, perturbed(case))
+
+
+@pytest.mark.parametrize("polydeg", [2, 3, 5])
+def test_rhs_standard_form_weak(oracle, polydeg):
+    """CovariantShallowWaterEquations2D: weak form, its flux_lax_friedrichs and
+    its geometric and Coriolis sources"""
+    case = W.covariant_elixir("geostrophic_balance", cells=3, polydeg=polydeg)
     check_rhs(oracle, case, perturbed(case))
 
 

@@ -1,9 +1,10 @@
 """The covariant shallow-water oracle against the reference's own regression
 values and against invariants the mathematics fixes.
 
-Pins: test/test_2d_shallow_water_covariant.jl:27-100 (five split-covariant
-elixirs on the 5x5-per-face cubed sphere, polydeg 3, one day with
-StepsizeCallback + CarpenterKennedy2N54).  Invariants: lake at rest over
+Pins: test/test_2d_shallow_water_covariant.jl:7-100 (geostrophic balance in
+the standard form with the weak form, and five split-covariant elixirs, on
+the 5x5-per-face cubed sphere, polydeg 3, one day with StepsizeCallback +
+CarpenterKennedy2N54).  Invariants: lake at rest over
 topography, mass conservation, entropy conservation of the EC fluxes, metric
 identities of the auxiliary variables.
 """
@@ -20,6 +21,10 @@ DAY = 86400.0
 
 # (elixir, overrides, l2, linf) from test/test_2d_shallow_water_covariant.jl
 PINS = [
+    # :7-26, the standard form with the weak form and flux_lax_friedrichs
+    ("geostrophic_balance", {},
+     [0.2782318946518096, 0.00021164119121947804, 8.588414721470694e-5],
+     [1.818810315677183, 0.0011402373824409423, 0.0010284231183849968]),
     ("unsteady_solid_body_rotation_EC",
      dict(volume_flux=("flux_central", "flux_nonconservative_ec"),
           surface_flux=("flux_lax_friedrichs", "flux_nonconservative_ec")),

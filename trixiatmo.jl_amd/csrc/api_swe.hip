@@ -272,9 +272,14 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
   if (cfg->polydeg < 1 || cfg->polydeg > 7)
     return fail(nullptr, ATMO_ERR_INVALID, "polydeg must be in 1..7");
   if (cfg->n_elements < 1) return fail(nullptr, ATMO_ERR_INVALID, "n_elements must be positive");
-  if (cfg->volume_flux < 0 || cfg->volume_flux > 1 || cfg->surface_flux < 0 ||
-      cfg->surface_flux > 2 || cfg->surface_noncons < 0 || cfg->surface_noncons > 1 ||
-      cfg->source_terms < 0 || cfg->source_terms > 1)
+  // the standard form (weak form, its flux_lax_friedrichs and sources) and the
+  // split form's fluxes do not mix
+  const bool standard = cfg->volume_flux == ATMO_SWE_VOL_WEAK_FORM;
+  if (cfg->volume_flux < 0 || cfg->volume_flux > 2 || cfg->surface_flux < 0 ||
+      cfg->surface_flux > 3 || cfg->surface_noncons < 0 || cfg->surface_noncons > 1 ||
+      cfg->source_terms < 0 || cfg->source_terms > 2 ||
+      standard != (cfg->surface_flux == ATMO_SWE_SURF_LAX_FRIEDRICHS_STANDARD) ||
+      (cfg->source_terms != 0 && standard != (cfg->source_terms == 2)))
     return fail(nullptr, ATMO_ERR_INVALID, "unsupported flux or source choice");
   if (cfg->precision != ATMO_PREC_FP64)
     return fail(nullptr, ATMO_ERR_INVALID, "the covariant form runs in FP64");
@@ -409,6 +414,9 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
   L.gravity = cfg->gravity;
   L.rotation_rate = cfg->rotation_rate;
   for (int q = 0; q < n * n; ++q) L.dsplit[q] = b.Dsplit[q];
+  for (int i = 0; i < n; ++i)
+    for (int m = 0; m < n; ++m)  // derivative_hat[i, m] = -D[m, i] w_m / w_i
+      L.dhat[i * n + m] = -b.D[m * n + i] * b.w[m] / b.w[i];
   L.inv_w = 1.0 / b.w[0];
   L.aux = h->d_aux;
   L.xyz = h->d_xyz;

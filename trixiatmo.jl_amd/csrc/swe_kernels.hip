@@ -100,10 +100,65 @@ __device__ __forceinline__ double max_abs_speed(double g, const double* ul, cons
               fabs(ur[1 + o] / ur[0]) + sqrt(xr.Gcon(o, o) * ur[0] * g));
 }
 
+// CovariantShallowWaterEquations2D, the standard form (covariant_shallow_water.jl:166-187):
+// contravariant flux in orientation o with the pressure term G^{io} g h^2 / 2
+__device__ __forceinline__ void flux_standard(double g, double h, double hv1, double hv2,
+                                              double J, double Gi_0o, double Gi_1o, double hvo,
+                                              double* f) {
+  const double vcon = hvo / h;
+  const double grav = 0.5 * g * h * h;
+  f[0] = J * hvo;
+  f[1] = J * (hv1 * vcon + Gi_0o * grav);
+  f[2] = J * (hv2 * vcon + Gi_1o * grav);
+}
+
+__device__ __forceinline__ void flux_standard(double g, const double* u, const AuxV& x, int o,
+                                              double* f) {
+  flux_standard(g, u[0], u[1], u[2], x.J(), x.Gcon(0, o), x.Gcon(1, o), u[1 + o], f);
+}
+
+// source_terms_geometric_coriolis of the standard form (:215-238)
+__device__ __forceinline__ void source_standard(double g, double omega, const double* u,
+                                                double sin_lat, const AuxV& a, double* s) {
+  const double h = u[0], hv[2] = {u[1], u[2]};
+  const double v[2] = {u[1] / h, u[2] / h};
+  const double grav = 0.5 * g * h * h;
+  double tau[2][2];  // h v^i v^j + G^{ij} g h^2 / 2
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) tau[i][j] = hv[i] * v[j] + grav * a.Gcon(i, j);
+  const double f = 2 * omega * sin_lat;
+  double src[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    double sg = 0.0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) sg += a.Gam(k, i, j) * tau[i][j];
+    src[k] = sg + f * a.J() * (a.Gcon(k, 1) * hv[0] - a.Gcon(k, 0) * hv[1]);
+  }
+  s[0] = 0.0;
+  s[1] = -src[0];
+  s[2] = -src[1];
+}
+
 // surface flux (u_ll, u_rr, aux, aux) + 1/2 the nonconservative surface term
 __device__ __forceinline__ void surface_terms(const SweLaunch& L, const double* ul,
                                               const double* ur, const AuxV& x, int o,
                                               double* f) {
+  if (L.surface_flux == 3) {
+    // flux_lax_friedrichs of the standard form: flux_central of the standard
+    // flux plus the local Lax-Friedrichs dissipation; no nonconservative term
+    double fl[NV], fr[NV];
+    flux_standard(L.gravity, ul, x, o, fl);
+    flux_standard(L.gravity, ur, x, o, fr);
+    const double lam = max_abs_speed(L.gravity, ul, ur, x, x, o);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) f[v] = 0.5 * (fl[v] + fr[v]) + -0.5 * x.J() * lam * (ur[v] - ul[v]);
+    return;
+  }
   if (L.surface_flux == 1) flux_central(ul, ur, x, x, o, f);
   else flux_ec(ul, ur, x, x, o, f);
   if (L.surface_flux != 2) {  // DissipationLocalLaxFriedrichs, equations.jl:283-290
@@ -211,7 +266,23 @@ __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
     __syncthreads();
 
     double du[NV] = {0, 0, 0};
-    if (active) {
+    if (active && L.volume_flux == 2) {
+      // weak form (dg_2d_manifold_in_3d_covariant.jl:100-130): node (i, j)
+      // gathers derivative_hat[i, m] F^1(m, j) + derivative_hat[j, m] F^2(i, m),
+      // the standard flux of every node on its two lines evaluated in place
+#pragma unroll
+      for (int m = 0; m < N; ++m) {
+        const int p1 = m + N * tj, p2 = ti + N * m;
+        double f1[NV], f2[NV];
+        flux_standard(L.gravity, s_n[es][0][p1], s_n[es][1][p1], s_n[es][2][p1], s_n[es][7][p1],
+                      s_n[es][11][p1], s_n[es][12][p1], s_n[es][1][p1], f1);
+        flux_standard(L.gravity, s_n[es][0][p2], s_n[es][1][p2], s_n[es][2][p2], s_n[es][7][p2],
+                      s_n[es][12][p2], s_n[es][13][p2], s_n[es][2][p2], f2);
+        const double d1 = L.dhat[ti * N + m], d2 = L.dhat[tj * N + m];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) du[v] += d1 * f1[v] + d2 * f2[v];
+      }
+    } else if (active) {
       const double hl = s_n[es][0][q], hvl[2] = {s_n[es][1][q], s_n[es][2][q]};
       const double vl[2] = {s_n[es][3][q], s_n[es][4][q]};
       const double Jl = s_n[es][7][q], hsl = s_n[es][14][q];
@@ -315,7 +386,10 @@ __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) xa[20 + c] = gg[(8 + c) * NN];
         double sv[NV];
-        source_split(L.rotation_rate, uo, gg[14 * NN], AuxV{xa, 1}, sv);
+        if (L.source_terms == 2)
+          source_standard(L.gravity, L.rotation_rate, uo, gg[14 * NN], AuxV{xa, 1}, sv);
+        else
+          source_split(L.rotation_rate, uo, gg[14 * NN], AuxV{xa, 1}, sv);
 #pragma unroll
         for (int v = 0; v < NV; ++v) du[v] += sv[v];
       }

@@ -8,12 +8,14 @@ namespace atmo {
 
 struct SweLaunch {
   int n, n_elements;
-  int volume_flux;      // 0 flux_ec, 1 flux_central
-  int surface_flux;     // 0 FluxPlusDissipation(flux_ec, LLF), 1 flux_lax_friedrichs, 2 flux_ec
-  int surface_noncons;  // 0 surface_simplified, 1 flux_nonconservative_ec
-  int source_terms;     // 1 source_terms_geometric_coriolis
+  int volume_flux;      // 0 flux_ec, 1 flux_central, 2 weak form of the standard form
+  int surface_flux;     // 0 FluxPlusDissipation(flux_ec, LLF), 1 flux_lax_friedrichs, 2 flux_ec,
+                        // 3 flux_lax_friedrichs of the standard form
+  int surface_noncons;  // 0 surface_simplified, 1 flux_nonconservative_ec (split form only)
+  int source_terms;     // 1 source_terms_geometric_coriolis, 2 the same of the standard form
   double gravity, rotation_rate;
   double dsplit[64];    // Dsplit[i * n + m]
+  double dhat[64];      // Trixi's derivative_hat[i * n + m] (weak form)
   double inv_w;         // 1 / w_end
   const double* u;
   double* out;

@@ -6,7 +6,8 @@ hand-written gfx950 HIP kernels of libatmo_hip.so (C ABI: include/atmo_hip.h).
 from ._lib import AtmoError, lib
 from .equations import (EARTH_GRAVITATIONAL_ACCELERATION, EARTH_RADIUS, EARTH_ROTATION_RATE,
                         CompressibleEulerEnergyEquationsWithGravity3D, FluxLMARS,
-                        FluxPlusDissipation, SplitCovariantShallowWaterEquations2D)
+                        CovariantShallowWaterEquations2D, FluxPlusDissipation,
+                        SplitCovariantShallowWaterEquations2D)
 from .initial_conditions import (initial_condition_baroclinic_instability,
                                  initial_condition_isothermal_perturbed,
                                  initial_condition_periodic_modes)
@@ -23,7 +24,7 @@ __all__ = [
     "initial_condition_baroclinic_instability", "initial_condition_isothermal_perturbed", "initial_condition_periodic_modes",
     "P4estMesh", "P4estMeshCubedSphereTopography", "lgl_basis", "SemidiscretizationHyperbolic",
     "DGSEM", "VolumeIntegralFluxDifferencing", "analysis", "workloads",
-    "SplitCovariantShallowWaterEquations2D", "P4estMeshCubedSphere2D", "MetricTermsCovariantSphere",
+    "SplitCovariantShallowWaterEquations2D", "CovariantShallowWaterEquations2D", "P4estMeshCubedSphere2D", "MetricTermsCovariantSphere",
     "SemidiscretizationEnsemble", "solve", "NonlinearSolveDG", "SaturationMLP",
     "SemidiscretizationRainy",
 ]

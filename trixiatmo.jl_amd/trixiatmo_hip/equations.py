@@ -21,6 +21,10 @@ SURFACE_FLUXES = {"flux_lax_friedrichs": 0, "flux_ranocha": 1, "flux_kennedy_gru
 # covariant shallow water: volume fluxes -> ATMO_SWE_VOL_*, surface fluxes ->
 # ATMO_SWE_SURF_*, surface nonconservative terms -> ATMO_SWE_NC_*
 SWE_VOLUME_FLUXES = {"flux_ec": 0, "flux_central": 1}
+# the standard (non-split) covariant form: VolumeIntegralWeakForm() and its
+# flux_lax_friedrichs (ATMO_SWE_VOL_WEAK_FORM, ATMO_SWE_SURF_LAX_FRIEDRICHS_STANDARD)
+SWE_VOLUME_WEAK_FORM = 2
+SWE_SURFACE_LLF_STANDARD = 3
 SWE_VOLUME_NONCONS = {"flux_nonconservative_ec"}
 SWE_SURFACE_FLUXES = {"FluxPlusDissipation(flux_ec, DissipationLocalLaxFriedrichs())": 0,
                       "flux_lax_friedrichs": 1, "flux_ec": 2}
@@ -76,6 +80,27 @@ class SplitCovariantShallowWaterEquations2D:
     2D manifold in 3D, variables (h, h v^1, h v^2) with contravariant
     momentum, split form with entropy-conservative two-point fluxes.  The
     global coordinate system is GlobalCartesianCoordinates()."""
+    gravity: float
+    rotation_rate: float
+    global_coordinate_system: str = "GlobalCartesianCoordinates()"
+
+    nvariables = 3
+    varnames = ("h", "h_vcon1", "h_vcon2")
+
+    def __post_init__(self):
+        if self.global_coordinate_system != "GlobalCartesianCoordinates()":
+            raise ValueError("the covariant path uses GlobalCartesianCoordinates()")
+
+
+@dataclass(frozen=True)
+class CovariantShallowWaterEquations2D:
+    """src/equations/covariant_shallow_water.jl:1-100: the standard covariant
+    shallow-water form, variables (h, h v^1, h v^2), conservative flux with the
+    pressure term G^{ij} g h^2 / 2 (no nonconservative terms, no bottom
+    topography), for VolumeIntegralWeakForm().  Global coordinates are
+    Cartesian: the reference's GlobalSphericalCoordinates() give the same
+    metric and the same map between neighbouring contravariant bases up to
+    rounding (the state is contravariant in both)."""
     gravity: float
     rotation_rate: float
     global_coordinate_system: str = "GlobalCartesianCoordinates()"

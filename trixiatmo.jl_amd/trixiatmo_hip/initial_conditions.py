@@ -218,6 +218,19 @@ def initial_condition_unsteady_solid_body_rotation(x, t):
     return H, v[0], v[1], v[2], bottom_topography_unsteady_solid_body_rotation(x)
 
 
+def initial_condition_geostrophic_balance(x, t):
+    """reference_data.jl:145-162 (Williamson et al. 1992, case 2): zonal flow
+    v_0 cos(lat), v_0 = 2 pi a / 12 days, in geostrophic balance with
+    h = (2.94e4 - (a Omega v_0 + v_0^2 / 2) sin^2(lat)) / g; the 2.94f4 and
+    0.5f0 literals are exact in Float32."""
+    a, _, lat = _lon_lat(x)
+    v_0 = 2.0 * np.pi * a / (12 * SECONDS_PER_DAY)
+    vlon = v_0 * np.cos(lat)
+    h = 1 / EARTH_GRAVITATIONAL_ACCELERATION * \
+        (2.94e4 - (a * EARTH_ROTATION_RATE * v_0 + 0.5 * v_0 ** 2) * np.sin(lat) ** 2)
+    return _spherical_state(h, vlon, np.zeros_like(h), np.zeros_like(h), x)
+
+
 def bottom_topography_isolated_mountain(x):
     """reference_data.jl:286-294 (Williamson et al. 1992, case 5)."""
     a, lon, lat = _lon_lat(x)

@@ -10,9 +10,11 @@ import numpy as np
 
 from ._lib import (AtmoConfig, AtmoGeometry2D, AtmoGeometry3D, AtmoSweConfig, check, dptr, iptr,
                    lib)
-from .equations import (EARTH_ROTATION_RATE, SWE_SURFACE_FLUXES, SWE_SURFACE_NONCONS,
-                        SWE_VOLUME_FLUXES, SWE_VOLUME_NONCONS,
+from .equations import (EARTH_ROTATION_RATE, SWE_SURFACE_FLUXES, SWE_SURFACE_LLF_STANDARD,
+                        SWE_SURFACE_NONCONS, SWE_VOLUME_FLUXES, SWE_VOLUME_NONCONS,
+                        SWE_VOLUME_WEAK_FORM, CovariantShallowWaterEquations2D,
                         SplitCovariantShallowWaterEquations2D)
+from .solvers import VolumeIntegralWeakForm
 
 ATMO_EQ_EULER_ENERGY_GRAVITY_3D = 1
 ATMO_BC_SLIP_WALL = 0
@@ -23,6 +25,28 @@ SOURCE_TERMS = {None: (0, 0.0), "source_terms_coriolis": (1, EARTH_ROTATION_RATE
 
 # covariant source terms -> ATMO_SRC_* of atmo_swe_config
 COVARIANT_SOURCE_TERMS = {None: 0, "source_terms_geometric_coriolis": 1}
+# source_terms_geometric_coriolis of the standard form (covariant_shallow_water.jl:215-238)
+COVARIANT_STANDARD_SOURCE_TERMS = {None: 0, "source_terms_geometric_coriolis": 2}
+COVARIANT_EQUATIONS = (SplitCovariantShallowWaterEquations2D, CovariantShallowWaterEquations2D)
+
+
+def swe_codes(equations, solver, source_terms):
+    """(volume_flux, surface_flux, surface_noncons, source_terms) codes of a
+    covariant semidiscretization (atmo_swe_config)"""
+    if isinstance(equations, CovariantShallowWaterEquations2D):
+        sf = solver.surface_flux if isinstance(solver.surface_flux, str) else solver.surface_flux[0]
+        if not isinstance(solver.volume_integral, VolumeIntegralWeakForm) or \
+                sf != "flux_lax_friedrichs":
+            raise ValueError("CovariantShallowWaterEquations2D runs with VolumeIntegralWeakForm() "
+                             "and flux_lax_friedrichs")
+        return (SWE_VOLUME_WEAK_FORM, SWE_SURFACE_LLF_STANDARD, 0,
+                COVARIANT_STANDARD_SOURCE_TERMS[source_terms])
+    vol, vnc = solver.volume_integral.volume_flux
+    sf, snc = solver.surface_flux
+    if vnc not in SWE_VOLUME_NONCONS:
+        raise ValueError(f"unsupported volume nonconservative flux {vnc}")
+    return (SWE_VOLUME_FLUXES[vol], SWE_SURFACE_FLUXES[sf], SWE_SURFACE_NONCONS[snc],
+            COVARIANT_SOURCE_TERMS[source_terms])
 
 
 class MetricTermsCovariantSphere:
@@ -43,16 +67,11 @@ class MetricTermsCovariantSphere:
 
 
 def _swe_config(equations, solver, n_elements, source_terms):
-    vol, vnc = solver.volume_integral.volume_flux
-    sf, snc = solver.surface_flux
-    if vnc not in SWE_VOLUME_NONCONS:
-        raise ValueError(f"unsupported volume nonconservative flux {vnc}")
     if solver.precision != "fp64":
         raise ValueError("the covariant form runs in FP64")
-    return AtmoSweConfig(polydeg=solver.polydeg, n_elements=n_elements,
-                         volume_flux=SWE_VOLUME_FLUXES[vol], surface_flux=SWE_SURFACE_FLUXES[sf],
-                         surface_noncons=SWE_SURFACE_NONCONS[snc],
-                         source_terms=COVARIANT_SOURCE_TERMS[source_terms], precision=0,
+    vol, sf, snc, src = swe_codes(equations, solver, source_terms)
+    return AtmoSweConfig(polydeg=solver.polydeg, n_elements=n_elements, volume_flux=vol,
+                         surface_flux=sf, surface_noncons=snc, source_terms=src, precision=0,
                          gravity=equations.gravity, rotation_rate=equations.rotation_rate)
 
 
@@ -80,7 +99,7 @@ class SemidiscretizationHyperbolic:
         self.mesh, self.equations, self.solver = mesh, equations, solver
         self.initial_condition = initial_condition
         self.source_terms = source_terms
-        if isinstance(equations, SplitCovariantShallowWaterEquations2D):
+        if isinstance(equations, COVARIANT_EQUATIONS):
             self._init_covariant(source_terms, metric_terms, auxiliary_field)
             return
         if solver.polydeg != mesh.polydeg:
@@ -146,7 +165,7 @@ class SemidiscretizationHyperbolic:
 
     @property
     def is_covariant(self):
-        return isinstance(self.equations, SplitCovariantShallowWaterEquations2D)
+        return isinstance(self.equations, COVARIANT_EQUATIONS)
 
     def __del__(self):
         h = getattr(self, "_h", None)

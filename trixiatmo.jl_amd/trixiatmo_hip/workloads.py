@@ -3,7 +3,7 @@ from .equations import CompressibleEulerEnergyEquationsWithGravity3D, EARTH_RADI
 from .initial_conditions import (initial_condition_isothermal_perturbed,
                                  initial_condition_periodic_modes)
 from .meshes import P4estMesh, P4estMeshCubedSphereTopography
-from .solvers import DGSEM, VolumeIntegralFluxDifferencing
+from .solvers import DGSEM, VolumeIntegralFluxDifferencing, VolumeIntegralWeakForm
 
 SHELL_RADIUS = 6.371e6
 
@@ -110,20 +110,26 @@ class CovariantCase:
 
 
 def covariant_elixir(name, cells=5, polydeg=3, **override):
-    """The split covariant elixirs of examples/shallow_water/covariant with the
+    """The covariant elixirs of examples/shallow_water/covariant with the
     Cartesian global coordinate system, at `cells` trees per face dimension:
+    "geostrophic_balance" (the standard form, weak form), and the split-form
     "unsteady_solid_body_rotation_EC", "barotropic_instability",
     "isolated_mountain", "rossby_haurwitz", "well_balanced".  `override`
     replaces volume_flux / surface_flux / christoffel as the reference's tests
     do through @test_trixi_include."""
     from . import initial_conditions as ic
     from .equations import (EARTH_GRAVITATIONAL_ACCELERATION, EARTH_ROTATION_RATE,
+                            CovariantShallowWaterEquations2D,
                             SplitCovariantShallowWaterEquations2D)
     from .meshes import P4estMeshCubedSphere2D
-    eq = SplitCovariantShallowWaterEquations2D(EARTH_GRAVITATIONAL_ACCELERATION,
-                                               EARTH_ROTATION_RATE)
     mesh = P4estMeshCubedSphere2D(cells, EARTH_RADIUS, polydeg=polydeg)
     spec = {
+        # elixir_geostrophic_balance.jl:9-37: the standard form with the weak
+        # form and flux_lax_friedrichs (the elixir's GlobalSphericalCoordinates()
+        # gives the same metric and basis maps up to rounding)
+        "geostrophic_balance": dict(
+            ic=ic.initial_condition_geostrophic_balance, field=None, standard=True,
+            surface_flux="flux_lax_friedrichs", cfl=0.4),
         # examples/shallow_water/covariant/elixir_unsteady_solid_body_rotation_EC.jl:20-46
         "unsteady_solid_body_rotation_EC": dict(
             ic=ic.initial_condition_unsteady_solid_body_rotation,
@@ -154,8 +160,16 @@ def covariant_elixir(name, cells=5, polydeg=3, **override):
             surface_flux=(SWE_EC_LLF, "flux_nonconservative_ec"), cfl=0.1),
     }[name]
     spec.update(override)
-    solver = DGSEM(polydeg=polydeg, surface_flux=tuple(spec["surface_flux"]),
-                   volume_integral=VolumeIntegralFluxDifferencing(tuple(spec["volume_flux"])))
+    if spec.get("standard"):
+        eq = CovariantShallowWaterEquations2D(EARTH_GRAVITATIONAL_ACCELERATION,
+                                              EARTH_ROTATION_RATE)
+        solver = DGSEM(polydeg=polydeg, surface_flux=spec["surface_flux"],
+                       volume_integral=VolumeIntegralWeakForm())
+    else:
+        eq = SplitCovariantShallowWaterEquations2D(EARTH_GRAVITATIONAL_ACCELERATION,
+                                                   EARTH_ROTATION_RATE)
+        solver = DGSEM(polydeg=polydeg, surface_flux=tuple(spec["surface_flux"]),
+                       volume_integral=VolumeIntegralFluxDifferencing(tuple(spec["volume_flux"])))
     return CovariantCase(mesh, eq, solver, ic.transform_initial_condition(spec["ic"]),
                          "source_terms_geometric_coriolis", spec["field"],
                          spec.get("christoffel", "ChristoffelSymbolsAutodiff()"), spec["cfl"])
