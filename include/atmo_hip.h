@@ -268,8 +268,8 @@ typedef struct {
  * integral of each of the nine variables (out[0..8]). */
 int atmo_rainy_create(const atmo_rainy_config* config, atmo_handle** handle);
 
-/* time of the resident state (used by time-dependent sources); atmo_step
- * advances it */
+/* time of the resident state (used by time-dependent sources and as the
+ * start of atmo_solve); atmo_step advances it */
 int atmo_set_time(atmo_handle* handle, double t);
 
 typedef struct {
@@ -364,6 +364,23 @@ int atmo_step(atmo_handle* handle, double dt, int32_t n_steps);
 /* StepsizeCallback max_dt on the resident state: dt = cfl * 2 / (n * max
  * scaled speed) (src/callbacks_step/stepsize_dg2d.jl:5-43 for the 2D form). */
 int atmo_max_dt(atmo_handle* handle, double cfl, double* dt);
+
+/* solve(ode, CarpenterKennedy2N54(williamson_condition = false); callback =
+ * StepsizeCallback(cfl)) from the handle's time (atmo_set_time; atmo_step
+ * advances it) to t_end, on the device: before every step
+ * dt = min(atmo_max_dt, t_end - t), computed on the device from the resident
+ * state, and the five stages read it there
+ * (examples/shallow_water/covariant/elixir_barotropic_instability.jl:79-80
+ * with the StepsizeCallback of :67).  The same steps and state as a host loop
+ * of atmo_max_dt + atmo_step; the host waits for the device only every
+ * `check_every` (>= 1) steps.  Ensemble members each take their own step
+ * sizes.  steps (may be NULL) receives the steps taken per member
+ * (the config's n_members entries, 1 for other handles), t_reached (may be NULL) the
+ * time reached (the smallest over members), which becomes the handle's time.
+ * ATMO_ERR_NONFINITE when a wave speed is not finite (the state is left at
+ * the last finite step). */
+int atmo_solve(atmo_handle* handle, double t_end, double cfl, int32_t check_every,
+               int64_t* steps, double* t_reached);
 
 /* Integrals on the resident state (analysis_covariant.jl:53-105 analogues):
  * out[0] mass, out[1] entropy, out[2] dS/dt = int w(u).du of the last

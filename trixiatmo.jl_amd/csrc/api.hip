@@ -166,7 +166,51 @@ int run_rhs(atmo_handle* h, double* du, const double* u) {
   return launch_stage(h, L);
 }
 
+// the five stages of one CarpenterKennedy2N54 step on the resident state
+// (Carpenter & Kennedy (1994) five-stage fourth-order 2N coefficients)
+int launch_rk_step(atmo_handle* h, E3Launch& L) {
+  static const double A[5] = {0.0, -567301805773.0 / 1357537059087.0,
+                              -2404267990393.0 / 2016746695238.0,
+                              -3550918686646.0 / 2091501179385.0,
+                              -1275806237668.0 / 842570457699.0};
+  static const double B[5] = {1432997174477.0 / 9575080441755.0,
+                              5161836677717.0 / 13612068292357.0,
+                              1720146321549.0 / 2090206949498.0,
+                              3134564353537.0 / 4481467310338.0,
+                              2277821191437.0 / 14882151754819.0};
+  for (int st = 0; st < 5; ++st) {
+    L.u = h->d_u;
+    L.out = h->d_u2;
+    L.rk_a = A[st];
+    L.rk_b = B[st];
+    const int rc = launch_stage(h, L);
+    if (rc != ATMO_OK) return rc;
+    std::swap(h->d_u, h->d_u2);
+  }
+  return ATMO_OK;
+}
+
 }  // namespace
+
+namespace atmo {
+
+int e3_enqueue_resident_step(atmo_handle* h) {
+  E3Launch L = base_launch(h);
+  L.u = h->d_u;
+  TimerPair tp;
+  if (h->timing) timer_begin(h, 1, &tp);
+  e3_launch_reduction(L, 0, nullptr, h->d_partial, h->d_result, h->stream);
+  if (h->timing) timer_end(h, &tp);
+  launch_step_control(h->d_result, 1, h->n, h->d_clock, h->d_clock + 1, h->stream);
+  HIP_TRY(h, hipGetLastError());
+  L.mode = 1;
+  L.k = h->d_k;
+  L.rk_dt = 0.0;
+  L.dt_dev = h->d_clock + 1;
+  return launch_rk_step(h, L);
+}
+
+}  // namespace atmo
 
 extern "C" {
 
@@ -419,6 +463,7 @@ void atmo_destroy(atmo_handle* h) {
   double* swe_bufs[] = {h->d_aux, h->d_xyz, h->d_dt_member, h->d_geo, h->d_fxf};
   for (double* p : swe_bufs)
     if (p) hipFree(p);
+  if (h->d_clock) hipFree(h->d_clock);
   if (h->d_member) hipFree(h->d_member);
   if (h->d_member_first) hipFree(h->d_member_first);
   if (h->d_face_info) hipFree(h->d_face_info);
@@ -480,32 +525,22 @@ int atmo_get_state(atmo_handle* h, double* u) {
 int atmo_step(atmo_handle* h, double dt, int32_t n_steps) {
   if (!h || n_steps < 0 || !(dt > 0) || !std::isfinite(dt))
     return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "bad step arguments");
-  if (h->eq == ATMO_EQ_COVARIANT_SWE_2D) return swe_step(h, nullptr, dt, n_steps);
   if (h->eq == ATMO_EQ_RAINY_EULER_2D) return rainy_step(h, dt, n_steps);
-  // Carpenter & Kennedy (1994) five-stage fourth-order 2N coefficients
-  static const double A[5] = {0.0, -567301805773.0 / 1357537059087.0,
-                              -2404267990393.0 / 2016746695238.0,
-                              -3550918686646.0 / 2091501179385.0,
-                              -1275806237668.0 / 842570457699.0};
-  static const double B[5] = {1432997174477.0 / 9575080441755.0,
-                              5161836677717.0 / 13612068292357.0,
-                              1720146321549.0 / 2090206949498.0,
-                              3134564353537.0 / 4481467310338.0,
-                              2277821191437.0 / 14882151754819.0};
+  if (h->eq == ATMO_EQ_COVARIANT_SWE_2D) {
+    const int rc = swe_step(h, nullptr, dt, n_steps);
+    if (rc != ATMO_OK) return rc;
+    for (int s = 0; s < n_steps; ++s) h->time += dt;
+    return ATMO_OK;
+  }
   E3Launch L = base_launch(h);
   L.mode = 1;
   L.k = h->d_k;
   L.rk_dt = dt;
-  for (int s = 0; s < n_steps; ++s)
-    for (int st = 0; st < 5; ++st) {
-      L.u = h->d_u;
-      L.out = h->d_u2;
-      L.rk_a = A[st];
-      L.rk_b = B[st];
-      int rc = launch_stage(h, L);
-      if (rc != ATMO_OK) return rc;
-      std::swap(h->d_u, h->d_u2);
-    }
+  for (int s = 0; s < n_steps; ++s) {
+    const int rc = launch_rk_step(h, L);
+    if (rc != ATMO_OK) return rc;
+    h->time += dt;
+  }
   h->du_valid = false;
   return ATMO_OK;
 }

@@ -77,20 +77,50 @@ int rainy_rhs_device(atmo_handle* h, double* du, const double* u, double t) {
   return launch_stage(h, L);
 }
 
+namespace {
+
+// Carpenter & Kennedy (1994) five-stage fourth-order 2N coefficients and the
+// stage times c
+const double kA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                      -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+const double kB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                      1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                      2277821191437.0 / 14882151754819.0};
+const double kC[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+                      2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
+
+}  // namespace
+
+int rainy_enqueue_resident_step(atmo_handle* h) {
+  RainyLaunch L = h->rainy_base;
+  L.u = h->d_u;
+  TimerPair tp;
+  if (h->timing) timer_begin(h, 1, &tp);
+  const bool ok = rainy_launch_max_speed(L, h->d_partial, h->d_result, h->stream);
+  if (h->timing) timer_end(h, &tp);
+  if (!ok) return fail(&h->err, ATMO_ERR_INVALID, "no rainy kernel for this polydeg");
+  launch_step_control(h->d_result, 1, h->n, h->d_clock, h->d_clock + 1, h->stream);
+  HIP_TRY(h, hipGetLastError());
+  L.mode = 1;
+  L.k = h->d_k;
+  L.clock = h->d_clock;
+  for (int st = 0; st < 5; ++st) {
+    L.u = h->d_u;
+    L.out = h->d_u2;
+    L.rk_a = kA[st];
+    L.rk_b = kB[st];
+    L.c_stage = kC[st];
+    const int rc = launch_stage(h, L);
+    if (rc != ATMO_OK) return rc;
+    std::swap(h->d_u, h->d_u2);
+  }
+  return ATMO_OK;
+}
+
 int rainy_step(atmo_handle* h, double dt, int n_steps) {
-  static const double A[5] = {0.0, -567301805773.0 / 1357537059087.0,
-                              -2404267990393.0 / 2016746695238.0,
-                              -3550918686646.0 / 2091501179385.0,
-                              -1275806237668.0 / 842570457699.0};
-  static const double Bc[5] = {1432997174477.0 / 9575080441755.0,
-                               5161836677717.0 / 13612068292357.0,
-                               1720146321549.0 / 2090206949498.0,
-                               3134564353537.0 / 4481467310338.0,
-                               2277821191437.0 / 14882151754819.0};
-  static const double c[5] = {0.0, 1432997174477.0 / 9575080441755.0,
-                              2526269341429.0 / 6820363962896.0,
-                              2006345519317.0 / 3224310063776.0,
-                              2802321613138.0 / 2924317926251.0};
+  const double* A = kA;
+  const double* Bc = kB;
+  const double* c = kC;
   RainyLaunch L = h->rainy_base;
   L.mode = 1;
   L.k = h->d_k;

@@ -94,7 +94,10 @@ int enqueue_steps(atmo_handle* h, int n_steps) {
 // capture two steps (ten launches; the ping-pong buffers come back to their
 // places) into a graph for replay
 int ensure_graph(atmo_handle* h) {
-  if (h->swe_graph && h->swe_graph_variant == h->swe_base.variant) return ATMO_OK;
+  // the graph holds the state buffers' addresses: an odd number of steps
+  // taken outside it swaps them, and it is captured again
+  if (h->swe_graph && h->swe_graph_variant == h->swe_base.variant && h->swe_graph_u == h->d_u)
+    return ATMO_OK;
   if (h->swe_graph) {
     HIP_TRY(h, hipGraphExecDestroy(h->swe_graph));
     h->swe_graph = nullptr;
@@ -112,10 +115,25 @@ int ensure_graph(atmo_handle* h) {
   hipGraphDestroy(g);
   HIP_TRY(h, ei);
   h->swe_graph_variant = h->swe_base.variant;
+  h->swe_graph_u = h->d_u;
   return ATMO_OK;
 }
 
 }  // namespace
+
+int swe_enqueue_resident_step(atmo_handle* h) {
+  SweLaunch L = h->swe_base;
+  L.u = h->d_u;
+  TimerPair tp;
+  if (h->timing) timer_begin(h, 1, &tp);
+  const bool ok = swe_launch_member_max_speed(L, h->d_partial, h->d_member_first, h->n_members,
+                                              h->d_result, h->stream);
+  if (h->timing) timer_end(h, &tp);
+  if (!ok) return fail(&h->err, ATMO_ERR_INVALID, "no covariant kernel for this polydeg");
+  launch_step_control(h->d_result, h->n_members, h->n, h->d_clock, h->d_dt_member, h->stream);
+  HIP_TRY(h, hipGetLastError());
+  return enqueue_steps(h, 1);
+}
 
 int swe_autotune(atmo_handle* h, double* ms_per_variant) {
   hipEvent_t a, b;

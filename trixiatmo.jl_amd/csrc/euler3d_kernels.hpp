@@ -32,6 +32,7 @@ struct E3Launch {
   const int* nbr;  // [E, 6, n*n] neighbour global node id, < 0 boundary
   const int* face_info;  // [E, 6] 0 primary/boundary, 8 | dp secondary of primary direction dp
   double rk_a, rk_dt, rk_b;
+  const double* dt_dev;  // if set, the stage's time step is read here (resident time loop)
   double coriolis;  // 2 Omega, 0 = no source
   Background bg;
 };

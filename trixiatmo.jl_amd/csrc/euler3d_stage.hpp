@@ -625,7 +625,7 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
     double* __restrict__ out, double* __restrict__ kreg, const T* __restrict__ cv,
     const double* __restrict__ invJ, const int* __restrict__ nbr,
     const int* __restrict__ face_info, int surf, int snc, int mode, double rk_a, double rk_dt,
-    double rk_b, double coriolis, Background bg) {
+    const double* __restrict__ dt_dev, double rk_b, double coriolis, Background bg) {
   using S = Shape<N>;
   using C = Chunking<N, T, NC>;
   constexpr int N2 = S::N2, N3 = S::N3, P = S::P, B = S::BLOCK;
@@ -735,7 +735,7 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
   __syncthreads();
   if (!active) return;
   stage_epilogue<N, T, KAHAN>(acc, acc_nc, s_fl, ti, tj, tk, gnode, u, out, kreg, invJ, mode,
-                              rk_a, rk_dt, rk_b, coriolis, bg);
+                              rk_a, dt_dev ? dt_dev[0] : rk_dt, rk_b, coriolis, bg);
 }
 
 // Direct form: every node evaluates the two-point flux with each of its n-1
@@ -752,7 +752,7 @@ void e3_stage_direct(
     double* __restrict__ out, double* __restrict__ kreg, const T* __restrict__ cv,
     const double* __restrict__ invJ, const int* __restrict__ nbr,
     const int* __restrict__ face_info, int surf, int snc, int mode, double rk_a, double rk_dt,
-    double rk_b, double coriolis, Background bg) {
+    const double* __restrict__ dt_dev, double rk_b, double coriolis, Background bg) {
   using S = Shape<N>;
   constexpr int N2 = S::N2, N3 = S::N3;
   constexpr bool MIXED = sizeof(T) == 4;
@@ -877,7 +877,7 @@ void e3_stage_direct(
   __syncthreads();
   if (!active) return;
   stage_epilogue<N, T, KAHAN>(acc, acc_nc, s_buf, ti, tj, tk, gnode, u, out, kreg, invJ, mode,
-                              rk_a, rk_dt, rk_b, coriolis, bg);
+                              rk_a, dt_dev ? dt_dev[0] : rk_dt, rk_b, coriolis, bg);
 }
 
 // which volume form a shape runs, from timings on MI355X: the direct form for
@@ -899,13 +899,13 @@ void launch_stage_inst(const E3Launch& L, hipStream_t s) {
     hipLaunchKernelGGL((e3_stage_direct<N, VOL, NC, T, KAHAN>), dim3(grid),
                        dim3(Shape<N>::BLOCK), 0, s, L.q, dst, L.inv_w, L.n_elements, L.u, L.out,
                        L.k, reinterpret_cast<const T*>(L.cv_t), L.invJ, L.nbr, L.face_info,
-                       L.surface_flux, L.surface_noncons, L.mode, L.rk_a, L.rk_dt, L.rk_b,
+                       L.surface_flux, L.surface_noncons, L.mode, L.rk_a, L.rk_dt, L.dt_dev, L.rk_b,
                        L.coriolis, L.bg);
   else
     hipLaunchKernelGGL((e3_stage<N, VOL, NC, T, KAHAN>), dim3(grid), dim3(Shape<N>::BLOCK), 0, s,
                        L.q, dst, L.inv_w, L.n_elements, L.u, L.out, L.k,
                        reinterpret_cast<const T*>(L.cv_t), L.invJ, L.nbr, L.face_info,
-                       L.surface_flux, L.surface_noncons, L.mode, L.rk_a, L.rk_dt, L.rk_b,
+                       L.surface_flux, L.surface_noncons, L.mode, L.rk_a, L.rk_dt, L.dt_dev, L.rk_b,
                        L.coriolis, L.bg);
 }
 

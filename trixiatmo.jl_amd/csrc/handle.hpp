@@ -56,10 +56,12 @@ struct atmo_handle {
   bool swe_tuned = false;                  // workgroup shape chosen by the autotuner
   hipGraphExec_t swe_graph = nullptr;      // two RK steps captured for replay
   int swe_graph_variant = -1;
+  double* swe_graph_u = nullptr;           // state buffer the graph was captured with
   // rainy Euler
   atmo_rainy_config rainy{};
   atmo::RainyLaunch rainy_base{};
   double time = 0.0;                       // time of the resident state
+  double* d_clock = nullptr;               // device clock of the resident time loop
   unsigned long long* d_stats = nullptr;   // Newton statistics
   atmo_mlp* mlp = nullptr;                 // seed of the saturation solve
   bool du_valid = false;
@@ -93,6 +95,14 @@ int rainy_step(atmo_handle* h, double dt, int n_steps);
 int rainy_max_dt(atmo_handle* h, double cfl, double* dt);
 int rainy_integrals(atmo_handle* h, double* out);
 void rainy_free(atmo_handle* h);
+// the resident time loop: one step (maximum wave speed, step-size control,
+// five stages reading the step size from d_clock) enqueued on the stream
+int e3_enqueue_resident_step(atmo_handle* h);
+int swe_enqueue_resident_step(atmo_handle* h);
+int rainy_enqueue_resident_step(atmo_handle* h);
+// dt[m] and the clock of n_members members from their maximum wave speeds
+void launch_step_control(const double* max_speed, int n_members, int n, double* clock, double* dt,
+                         hipStream_t s);
 }  // namespace atmo
 
 #endif

@@ -499,6 +499,13 @@ __device__ __forceinline__ bool mlp_seed(const double* u, const float* y, double
   return x[2] > 0.0;
 }
 
+// t0 + c dt rounded as two separate operations (the host's order, no fused
+// multiply-add), so a stage time read from the device clock equals the host's
+__device__ __forceinline__ double stage_time(double t0, double c, double dt) {
+#pragma clang fp contract(off)
+  return t0 + c * dt;
+}
+
 template <int N, bool SEEDED>
 __global__ __launch_bounds__(RShape<N>::BLOCK)
 void rainy_stage(RainyLaunch L) {
@@ -516,6 +523,18 @@ void rainy_stage(RainyLaunch L) {
 
   const int t = threadIdx.x;
   const int e0 = blockIdx.x * EPB;
+  double rk_dt = L.rk_dt, t_src = L.t;
+  if (L.clock) {
+    rk_dt = L.clock[1];
+    t_src = stage_time(L.clock[0], L.c_stage, rk_dt);
+    if (L.mode == 1 && rk_dt == 0.0) {  // finished: the state is carried over as it is
+      for (int idx = t; idx < EPB * N2 * NV; idx += B) {
+        const size_t g = (size_t)e0 * N2 * NV + idx;
+        if (e0 + idx / (N2 * NV) < L.n_elements) L.out[g] = L.u[g];
+      }
+      return;
+    }
+  }
   for (int idx = t; idx < EPB * N2 * NV; idx += B) {
     const int es = idx / (N2 * NV), r = idx - es * N2 * NV;
     if (e0 + es < L.n_elements) s_u[es][r % NV][r / NV] = L.u[(size_t)(e0 + es) * N2 * NV + r];
@@ -682,7 +701,7 @@ void rainy_stage(RainyLaunch L) {
         const int ex = e % L.nx, ey = e / L.nx;
         const double x = L.x0 + L.dx * (ex + 0.5 * (L.xi[ti] + 1.0));
         const double y = L.y0 + L.dy * (ey + 0.5 * (L.xi[tj] + 1.0));
-        source_convergence(x, y, L.t, sv);
+        source_convergence(x, y, t_src, sv);
       } else {  // source_terms_no_phase_change (:579-593)
 #pragma unroll
         for (int v = 0; v < NV; ++v) sv[v] = 0.0;
@@ -700,7 +719,7 @@ void rainy_stage(RainyLaunch L) {
       const size_t g = ((size_t)e * N2 + q) * NV;
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
-        const double kn = L.rk_a * L.k[g + v] + L.rk_dt * du[v];
+        const double kn = L.rk_a * L.k[g + v] + rk_dt * du[v];
         L.k[g + v] = kn;
         un[v] = uo[v] + L.rk_b * kn;
       }

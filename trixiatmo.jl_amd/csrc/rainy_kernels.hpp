@@ -101,6 +101,11 @@ struct RainyLaunch {
   int mode;                   // 0 rhs, 1 RK stage (+ Newton on the new state)
   double t;                   // time of the rhs evaluation (time-dependent sources)
   double rk_a, rk_dt, rk_b;
+  // resident time loop: if set, the step's start time and size are read from
+  // clock[0] and clock[1] (the stage's time is clock[0] + c_stage clock[1]),
+  // and a step of size 0 copies the state unchanged
+  const double* clock;
+  double c_stage;
   double newton_tol;
   int newton_max_iter;
   unsigned long long* stats;  // optional: [sum of updates, failures, max updates, fallbacks]

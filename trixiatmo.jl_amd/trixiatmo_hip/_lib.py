@@ -168,6 +168,9 @@ SIGNATURES = [
     ("atmo_get_state", ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     ("atmo_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]),
     ("atmo_max_dt", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, c_double_p]),
+    ("atmo_solve", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+      ctypes.POINTER(ctypes.c_int64), c_double_p]),
     ("atmo_integrals", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_double_p]),
     ("atmo_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("atmo_enable_timing", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),

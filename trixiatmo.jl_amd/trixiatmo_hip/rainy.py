@@ -114,6 +114,16 @@ class SemidiscretizationRainy:
     def step(self, dt, n_steps=1):
         check(lib().atmo_step(self._h, float(dt), int(n_steps)), self._h)
 
+    def set_time(self, t):
+        check(lib().atmo_set_time(self._h, float(t)), self._h)
+
+    def solve_resident(self, t_end, cfl, check_every=64):
+        """StepsizeCallback(cfl) time loop from the resident time to t_end on
+        the device, the stage limiter in every stage; returns (steps, time
+        reached)."""
+        from .semidiscretization import solve_resident
+        return solve_resident(self._h, 1, t_end, cfl, check_every)
+
     def max_dt(self, cfl):
         dt = ctypes.c_double()
         check(lib().atmo_max_dt(self._h, float(cfl), ctypes.byref(dt)), self._h)

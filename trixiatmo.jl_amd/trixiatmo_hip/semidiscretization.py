@@ -220,6 +220,15 @@ class SemidiscretizationHyperbolic:
         check(lib().atmo_max_dt(self._h, float(cfl), ctypes.byref(dt)), self._h)
         return dt.value
 
+    def set_time(self, t):
+        """time of the resident state (the start of solve_resident)"""
+        check(lib().atmo_set_time(self._h, float(t)), self._h)
+
+    def solve_resident(self, t_end, cfl, check_every=64):
+        """StepsizeCallback(cfl) time loop from the resident time to t_end on
+        the device (see atmo_solve); returns (steps per member, time reached)."""
+        return solve_resident(self._h, getattr(self, "n_members", 1), t_end, cfl, check_every)
+
     def integrals(self, fresh_rhs=True):
         """(mass, entropy, dS/dt, integral of each conserved variable...)"""
         out = np.zeros(3 if self.is_covariant else 9)
@@ -321,17 +330,38 @@ class SemidiscretizationEnsemble(SemidiscretizationHyperbolic):
         check(lib().atmo_step_members(self._h, dptr(d), int(n_steps)), self._h)
 
 
-def solve(semi, u0, tspan, cfl):
+def solve_resident(handle, n_members, t_end, cfl, check_every=64):
+    """atmo_solve on a handle: (steps taken per member, time reached)"""
+    steps = np.zeros(n_members, dtype=np.int64)
+    t = ctypes.c_double()
+    check(lib().atmo_solve(handle, float(t_end), float(cfl), int(check_every),
+                           steps.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                           ctypes.byref(t)), handle)
+    return steps, t.value
+
+
+def solve(semi, u0, tspan, cfl, resident=False, check_every=64):
     """`solve(ode, CarpenterKennedy2N54(williamson_condition = false);
     callback = StepsizeCallback(cfl))` on the resident state: before every
     step dt = cfl * max_dt(u), and the last step is cut to end at tspan[1]
-    as OrdinaryDiffEq's tstop handling does.  Returns (final state, steps)."""
+    as OrdinaryDiffEq's tstop handling does
+    (examples/shallow_water/covariant/elixir_barotropic_instability.jl:67,79-80).
+    Host loop by default; `resident=True` keeps the loop on the device
+    (same steps and state, the host waiting only every `check_every` steps;
+    ensemble members then each take their own step sizes).  Returns
+    (final state, steps), steps per member for a resident ensemble."""
     t, t_end = float(tspan[0]), float(tspan[1])
     semi.set_state(u0)
-    steps = 0
-    while t < t_end:
-        dt = min(semi.max_dt(cfl), t_end - t)
-        semi.step(dt, 1)
-        t += dt
-        steps += 1
+    semi.set_time(t)
+    if resident:
+        steps, _ = semi.solve_resident(t_end, cfl, check_every)
+        if getattr(semi, "n_members", 1) == 1:
+            steps = int(steps[0])
+    else:
+        steps = 0
+        while t < t_end:
+            dt = min(semi.max_dt(cfl), t_end - t)
+            semi.step(dt, 1)
+            t += dt
+            steps += 1
     return semi.get_state().reshape(semi.shape, order="F"), steps
