@@ -127,6 +127,95 @@ def bench_covariant(steps, warmup, trees=32, polydeg=3, n_members=32):
     return out
 
 
+def _time_loops(semi, u0, cfl, steps, check_every=256):
+    """the StepsizeCallback loop over the span that `steps` steps of the
+    initial step size cover, run as a host loop (atmo_max_dt + atmo_step per
+    step) and resident on the device (atmo_solve); returns ms per step of each"""
+    import numpy as np
+    semi.set_state(u0)
+    t_end = steps * semi.max_dt(cfl)
+    out = {}
+    for mode in ("host", "resident"):
+        for rep in range(2):  # the first pass warms up (graph capture, autotuning)
+            semi.set_state(u0)
+            semi.set_time(0.0)
+            semi.synchronize()
+            t0 = time.perf_counter()
+            if mode == "host":
+                t, n = 0.0, 0
+                while t < t_end:
+                    dt = min(semi.max_dt(cfl), t_end - t)
+                    semi.step(dt, 1)
+                    t += dt
+                    n += 1
+                n = np.array([n])
+            else:
+                n, _ = semi.solve_resident(t_end, cfl, check_every)
+            semi.synchronize()
+            el = time.perf_counter() - t0
+        out[mode] = {"steps": int(n.max()), "ms_per_step": el / max(1, int(n.max())) * 1e3,
+                     "seconds": el}
+    out["speedup"] = out["host"]["ms_per_step"] / out["resident"]["ms_per_step"]
+    return out
+
+
+def bench_time_loop(steps, trees, layers, polydeg, cov_trees=32, n_members=32):
+    """time to solution of the StepsizeCallback loop, host-driven against
+    resident, on the 3D shell, the covariant single member and (resident only,
+    members on their own step sizes) the covariant ensemble"""
+    import trixiatmo_hip as T
+    W = T.workloads
+    out = {}
+    mesh, eq, solver, bcs, u0 = W.shell(trees, layers, polydeg=polydeg, seed=0)
+    semi = T.SemidiscretizationHyperbolic(mesh, eq, solver, bcs)
+    out["euler3d_shell"] = _time_loops(semi, u0, 0.5, steps)
+    del semi
+    case = W.zonal_jet(cov_trees, 3, seed=0)
+    out["covariant_single"] = _time_loops(case.semi(), case.u0(), 0.4, steps)
+    members, eq, solver, src = W.zonal_jet_ensemble(n_members=n_members, polydeg=3, seed=0)
+    ens = T.SemidiscretizationEnsemble(members, eq, solver, source_terms=src)
+    u0 = ens.compute_coefficients()
+    ens.set_state(u0)
+    t_end = steps * ens.max_dt_members(0.4).min()
+    for rep in range(2):
+        ens.set_state(u0)
+        ens.set_time(0.0)
+        ens.synchronize()
+        t0 = time.perf_counter()
+        n, _ = ens.solve_resident(t_end, 0.4, 256)
+        ens.synchronize()
+        el = time.perf_counter() - t0
+    out["covariant_ensemble_resident"] = {
+        "steps_per_member_min_max": [int(n.min()), int(n.max())], "seconds": el,
+        "ms_per_step": el / int(n.max()) * 1e3}
+    for k, v in out.items():
+        log(f"[time loop {k}] {v}")
+    return out
+
+
+def bench_setup(trees, layers, polydeg):
+    """set-up time of the 3D shell: mesh (node coordinates and connectivity on
+    the host), then the handle with the metric terms computed on the device
+    from the node coordinates, or on the host and uploaded"""
+    import trixiatmo_hip as T
+    W = T.workloads
+    t0 = time.perf_counter()
+    mesh, eq, solver, bcs, _ = W.shell(trees, layers, polydeg=polydeg, seed=0)
+    t_mesh = time.perf_counter() - t0
+    out = {"workload": f"shell 6x{trees}^2x{layers}, polydeg {polydeg}", "mesh_seconds": t_mesh}
+    for on_device in (True, False):
+        mesh._metric = None
+        t0 = time.perf_counter()
+        semi = T.SemidiscretizationHyperbolic(mesh, eq, solver, bcs,
+                                              metric_terms_on_device=on_device)
+        semi.synchronize()
+        out["handle_seconds_" + ("device_metric" if on_device else "host_metric")] = \
+            time.perf_counter() - t0
+        del semi
+    log(f"[setup] {out}")
+    return out
+
+
 def bench_precision_stress(steps=1000):
     """the entropy-conservative shell (6 x 8^2 x 4, polydeg 7) in three forms:
     FP64, FP32 fluxes with Kahan sums, FP32 fluxes with plain sums; each runs
@@ -297,6 +386,8 @@ def main():
     ap.add_argument("--stress-steps", type=int, default=1000,
                     help="steps of the precision-stress forms (0 skips them)")
     ap.add_argument("--rainy-steps", type=int, default=20)
+    ap.add_argument("--loop-steps", type=int, default=200,
+                    help="steps of the host-driven and resident time loops (0 skips them)")
     ap.add_argument("--traffic-kib", type=float, nargs=3, default=None,
                     metavar=("FETCH", "WRITE", "FACTOR"),
                     help="rocprofv3 FETCH_SIZE / WRITE_SIZE of the stage kernel and the FETCH scale")
@@ -393,6 +484,13 @@ def main():
         log("rainy Euler")
         rainy = bench_rainy(a.rainy_steps, min(a.warmup, 3))
 
+    setup = bench_setup(a.trees, a.layers, a.polydeg)
+
+    loops = None
+    if a.loop_steps > 0:
+        log("time loops")
+        loops = bench_time_loop(a.loop_steps, a.trees, a.layers, a.polydeg)
+
     stress = None
     if a.stress_steps > 0:
         log("precision stress")
@@ -444,6 +542,8 @@ def main():
         "covariant_swe": cov,
         "rainy_euler": rainy,
         "precision_stress": stress,
+        "time_loop": loops,
+        "setup": setup,
         "state_finite": ok,
     }
     print(json.dumps(out))

@@ -162,6 +162,8 @@ typedef struct {
 } atmo_config;
 
 typedef struct {
+  /* metric terms, or NULL (both) to compute them on the device from
+   * node_coordinates */
   const double* contravariant_vectors;   /* [3,3,n,n,n,E] */
   const double* inverse_jacobian;        /* [n,n,n,E] */
   int32_t n_interfaces;
@@ -171,7 +173,21 @@ typedef struct {
   const int32_t* boundary_element;       /* [B] */
   const int32_t* boundary_node_indices;  /* [3,B] */
   const int32_t* boundary_condition;     /* [B] ATMO_BC_* */
+  const double* node_coordinates;        /* [3,n,n,n,E]; read only when the metric terms are NULL */
 } atmo_geometry3d;
+
+/* Metric terms of P4estMesh{3} elements from their node coordinates
+ * [3,n,n,n,E] (Trixi init_elements! with MetricTermsInvariantCurl,
+ * src/semidiscretization/metric_terms.jl:25 [ext]): the Jacobian matrix by
+ * the collocation derivative, the contravariant vectors [3,3,n,n,n,E] in
+ * Kopriva's invariant curl form and the inverse Jacobian [n,n,n,E].
+ * atmo_metric_terms3d takes host arrays; atmo_metric_terms3d_device device
+ * pointers (one workgroup per element, complete on return); both give the
+ * same bits. */
+int atmo_metric_terms3d(int polydeg, int32_t n_elements, const double* node_coordinates,
+                        double* contravariant_vectors, double* inverse_jacobian);
+int atmo_metric_terms3d_device(int polydeg, int32_t n_elements, const double* node_coordinates,
+                               double* contravariant_vectors, double* inverse_jacobian);
 
 /* covariant shallow water: fluxes.  SplitCovariantShallowWaterEquations2D
  * takes ATMO_SWE_VOL_EC / _CENTRAL with ATMO_SWE_SURF_EC_PLUS_LLF /

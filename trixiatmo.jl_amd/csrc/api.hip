@@ -32,6 +32,7 @@ std::string& global_error() {
 
 struct atmo_mesh3d {
   Mesh3D m;
+  bool metric_done = false;  // metric terms are computed on the first request
 };
 
 namespace {
@@ -239,7 +240,6 @@ int atmo_mesh3d_cartesian(int polydeg, const int32_t trees[3], const double xmin
   const int t[3] = {trees[0], trees[1], trees[2]};
   const int p[3] = {periodic[0], periodic[1], periodic[2]};
   build_cartesian(mh->m, b, t, xmin, xmax, p);
-  compute_metric_terms(mh->m, b);
   std::string err;
   if (!build_connectivity(mh->m, &err)) {
     delete mh;
@@ -260,7 +260,6 @@ int atmo_mesh3d_cubed_sphere_shell(int polydeg, int trees_per_face_dimension, in
   mh->m.n = polydeg + 1;
   Basis b(polydeg + 1);
   build_cubed_sphere_shell(mh->m, b, trees_per_face_dimension, layers, inner_radius, thickness);
-  compute_metric_terms(mh->m, b);
   std::string err;
   if (!build_connectivity(mh->m, &err)) {
     delete mh;
@@ -282,6 +281,11 @@ int atmo_mesh3d_arrays(const atmo_mesh3d* mh, const double** xyz, const double**
                        const double** invJ, const int32_t** iids, const int32_t** iidx,
                        const int32_t** be, const int32_t** bidx, const int32_t** bk) {
   if (!mh) return fail(nullptr, ATMO_ERR_INVALID, "null mesh");
+  if ((cv || invJ) && !mh->metric_done) {
+    auto* mm = const_cast<atmo_mesh3d*>(mh);  // a cache filled on first use
+    compute_metric_terms(mm->m, Basis(mm->m.n));
+    mm->metric_done = true;
+  }
   if (xyz) *xyz = mh->m.node_coordinates.data();
   if (cv) *cv = mh->m.contravariant.data();
   if (invJ) *invJ = mh->m.inverse_jacobian.data();
@@ -294,6 +298,30 @@ int atmo_mesh3d_arrays(const atmo_mesh3d* mh, const double** xyz, const double**
 }
 
 void atmo_mesh3d_destroy(atmo_mesh3d* mh) { delete mh; }
+
+int atmo_metric_terms3d(int polydeg, int32_t ne, const double* xyz, double* cv, double* invJ) {
+  if (polydeg < 1 || polydeg > 15 || ne < 0 || (ne > 0 && (!xyz || !cv || !invJ)))
+    return fail(nullptr, ATMO_ERR_INVALID, "bad metric-term arguments");
+  const int n = polydeg + 1;
+  Basis b(n);
+  std::vector<double> jac((size_t)9 * n * n * n * ne);
+  compute_metric_terms(n, ne, xyz, b.D.data(), jac.data(), cv, invJ);
+  return ATMO_OK;
+}
+
+int atmo_metric_terms3d_device(int polydeg, int32_t ne, const double* xyz, double* cv,
+                               double* invJ) {
+  if (polydeg < 1 || polydeg > 7 || ne < 0 || (ne > 0 && (!xyz || !cv || !invJ)))
+    return fail(nullptr, ATMO_ERR_INVALID, "bad metric-term arguments (polydeg 1..7)");
+  if (ne == 0) return ATMO_OK;
+  Basis b(polydeg + 1);
+  if (!e3_launch_metric_terms(polydeg + 1, ne, b.D.data(), xyz, cv, invJ, nullptr))
+    return fail(nullptr, ATMO_ERR_INVALID, "polydeg out of range");
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  if (e != hipSuccess) return fail(nullptr, ATMO_ERR_DEVICE, hipGetErrorString(e));
+  return ATMO_OK;
+}
 
 const char* atmo_last_error(const atmo_handle* h) {
   return h ? h->err.c_str() : global_error().c_str();
@@ -321,7 +349,9 @@ int atmo_create(const atmo_config* cfg, const atmo_geometry3d* g, atmo_handle** 
     return fail(nullptr, ATMO_ERR_INVALID, "FluxLMARS needs a positive speed of sound");
   if (cfg->source_terms != ATMO_SRC_NONE && cfg->source_terms != ATMO_SRC_CORIOLIS)
     return fail(nullptr, ATMO_ERR_INVALID, "unsupported source terms");
-  if (!g->contravariant_vectors || !g->inverse_jacobian)
+  const bool metric_on_device = !g->contravariant_vectors && !g->inverse_jacobian;
+  if ((!metric_on_device && (!g->contravariant_vectors || !g->inverse_jacobian)) ||
+      (metric_on_device && !g->node_coordinates))
     return fail(nullptr, ATMO_ERR_INVALID, "missing geometry arrays");
   if (g->n_interfaces < 0 || g->n_boundaries < 0 ||
       (g->n_interfaces > 0 && (!g->interface_neighbor_ids || !g->interface_node_indices)) ||
@@ -419,11 +449,30 @@ int atmo_create(const atmo_config* cfg, const atmo_geometry3d* g, atmo_handle** 
     fail(nullptr, ATMO_ERR_DEVICE, "device allocation failed");
     return cleanup_fail(ATMO_ERR_DEVICE);
   }
-  if (hipMemcpy(h->d_cv, g->contravariant_vectors, sizeof(double) * 9 * n3 * (size_t)E,
-                hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->d_invJ, g->inverse_jacobian, sizeof(double) * n3 * (size_t)E,
-                hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->d_nbr, nbr.data(), sizeof(int32_t) * nbr.size(), hipMemcpyHostToDevice) !=
+  if (metric_on_device) {
+    // upload the node coordinates only and compute the metric terms here
+    double* d_xyz = nullptr;
+    Basis b(n);
+    const bool ok = hipMalloc(&d_xyz, sizeof(double) * 3 * n3 * (size_t)E) == hipSuccess &&
+                    hipMemcpy(d_xyz, g->node_coordinates, sizeof(double) * 3 * n3 * (size_t)E,
+                              hipMemcpyHostToDevice) == hipSuccess &&
+                    e3_launch_metric_terms(n, E, b.D.data(), d_xyz, h->d_cv, h->d_invJ, h->stream) &&
+                    hipGetLastError() == hipSuccess;
+    if (ok && h->d_cv32) launch_narrow_to_float(h->d_cv, h->d_cv32, (size_t)9 * n3 * E, h->stream);
+    const bool done = ok && hipStreamSynchronize(h->stream) == hipSuccess;
+    if (d_xyz) (void)hipFree(d_xyz);
+    if (!done) {
+      fail(nullptr, ATMO_ERR_DEVICE, "metric terms on the device failed");
+      return cleanup_fail(ATMO_ERR_DEVICE);
+    }
+  } else if (hipMemcpy(h->d_cv, g->contravariant_vectors, sizeof(double) * 9 * n3 * (size_t)E,
+                       hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(h->d_invJ, g->inverse_jacobian, sizeof(double) * n3 * (size_t)E,
+                       hipMemcpyHostToDevice) != hipSuccess) {
+    fail(nullptr, ATMO_ERR_DEVICE, "device upload failed");
+    return cleanup_fail(ATMO_ERR_DEVICE);
+  }
+  if (hipMemcpy(h->d_nbr, nbr.data(), sizeof(int32_t) * nbr.size(), hipMemcpyHostToDevice) !=
           hipSuccess ||
       hipMemcpy(h->d_face_info, face_info.data(), sizeof(int32_t) * face_info.size(),
                 hipMemcpyHostToDevice) != hipSuccess ||
@@ -431,7 +480,7 @@ int atmo_create(const atmo_config* cfg, const atmo_geometry3d* g, atmo_handle** 
     fail(nullptr, ATMO_ERR_DEVICE, "device upload failed");
     return cleanup_fail(ATMO_ERR_DEVICE);
   }
-  if (h->d_cv32) {
+  if (h->d_cv32 && !metric_on_device) {
     std::vector<float> cv32((size_t)9 * n3 * E);
     for (size_t q = 0; q < cv32.size(); ++q) cv32[q] = (float)g->contravariant_vectors[q];
     if (hipMemcpy(h->d_cv32, cv32.data(), sizeof(float) * cv32.size(), hipMemcpyHostToDevice) !=

@@ -54,6 +54,11 @@ bool build_cubed_sphere_2d(Mesh2D& m, const Basis& b, int trees_per_face, double
 
 // node coordinates -> Jacobian matrix, invariant-curl metric terms, 1/J
 void compute_metric_terms(Mesh3D& m, const Basis& b);
+// the same on raw arrays: X [3,n,n,n,E], D [n,n] -> jacobian_matrix
+// [3,3,n,n,n,E] (scratch), contravariant [3,3,n,n,n,E], inverse_jacobian [n,n,n,E]
+void compute_metric_terms(int n, int n_elements, const double* X, const double* D,
+                          double* jacobian_matrix, double* contravariant,
+                          double* inverse_jacobian);
 // face matching by node coordinates -> interfaces + boundaries
 bool build_connectivity(Mesh3D& m, std::string* err);
 

@@ -49,6 +49,12 @@ bool e3_launch_stage_fp32(const E3Launch& L, hipStream_t s);
 // which 1: integrals (9 columns) -> result[0..8]
 bool e3_launch_reduction(const E3Launch& L, int which, const double* du, double* partial,
                          double* result, hipStream_t s);
+// contravariant vectors [3,3,n,n,n,E] and 1/J [n,n,n,E] from the node
+// coordinates [3,n,n,n,E] (D: the n x n derivative matrix); false for n > 8
+bool e3_launch_metric_terms(int n, int n_elements, const double* D, const double* xyz,
+                            double* cv, double* invJ, hipStream_t s);
+// b[q] = (float) a[q]
+void launch_narrow_to_float(const double* a, float* b, size_t n, hipStream_t s);
 
 }  // namespace atmo
 

@@ -39,13 +39,16 @@ static inline size_t nidx(int n, int i, int j, int k, int e) {
 }
 
 void compute_metric_terms(Mesh3D& m, const Basis& b) {
-  const int n = m.n, E = m.n_elements;
-  const size_t n3 = (size_t)n * n * n;
-  m.jacobian_matrix.assign(9 * n3 * E, 0.0);
-  m.contravariant.assign(9 * n3 * E, 0.0);
-  m.inverse_jacobian.assign(n3 * E, 0.0);
-  const double* X = m.node_coordinates.data();
-  const double* D = b.D.data();
+  const size_t n3 = (size_t)m.n * m.n * m.n;
+  m.jacobian_matrix.assign(9 * n3 * m.n_elements, 0.0);
+  m.contravariant.assign(9 * n3 * m.n_elements, 0.0);
+  m.inverse_jacobian.assign(n3 * m.n_elements, 0.0);
+  compute_metric_terms(m.n, m.n_elements, m.node_coordinates.data(), b.D.data(),
+                       m.jacobian_matrix.data(), m.contravariant.data(), m.inverse_jacobian.data());
+}
+
+void compute_metric_terms(int n, int E, const double* X, const double* D, double* jacobian_matrix,
+                          double* contravariant, double* inverse_jacobian) {
   auto at = [&](int c, int i, int j, int k, int e) { return X[3 * nidx(n, i, j, k, e) + c]; };
   auto step = [](int d, int& i, int& j, int& k) -> int& { return d == 0 ? i : (d == 1 ? j : k); };
   for (int e = 0; e < E; ++e) {
@@ -53,7 +56,7 @@ void compute_metric_terms(Mesh3D& m, const Basis& b) {
     for (int k = 0; k < n; ++k)
       for (int j = 0; j < n; ++j)
         for (int i = 0; i < n; ++i) {
-          double* J = &m.jacobian_matrix[9 * nidx(n, i, j, k, e)];
+          double* J = &jacobian_matrix[9 * nidx(n, i, j, k, e)];
           for (int d = 0; d < 3; ++d) {
             int ii = i, jj = j, kk = k;
             int& ix = step(d, ii, jj, kk);
@@ -88,7 +91,7 @@ void compute_metric_terms(Mesh3D& m, const Basis& b) {
                 for (int q = 0; q < n; ++q) {
                   ix = q;
                   const size_t nd = nidx(n, ii, jj, kk, e);
-                  const double* Jq = &m.jacobian_matrix[9 * nd];
+                  const double* Jq = &jacobian_matrix[9 * nd];
                   const double g = X[3 * nd + mm] * Jq[ll + 3 * d2] - X[3 * nd + ll] * Jq[mm + 3 * d2];
                   term1 += D[own * n + q] * g;
                 }
@@ -100,23 +103,23 @@ void compute_metric_terms(Mesh3D& m, const Basis& b) {
                 for (int q = 0; q < n; ++q) {
                   ix = q;
                   const size_t nd = nidx(n, ii, jj, kk, e);
-                  const double* Jq = &m.jacobian_matrix[9 * nd];
+                  const double* Jq = &jacobian_matrix[9 * nd];
                   const double g = X[3 * nd + mm] * Jq[ll + 3 * d1] - X[3 * nd + ll] * Jq[mm + 3 * d1];
                   term2 += D[own * n + q] * g;
                 }
               }
-              m.contravariant[9 * nidx(n, i, j, k, e) + c + 3 * d] = 0.5 * (term1 - term2);
+              contravariant[9 * nidx(n, i, j, k, e) + c + 3 * d] = 0.5 * (term1 - term2);
             }
       }
     }
     for (int k = 0; k < n; ++k)
       for (int j = 0; j < n; ++j)
         for (int i = 0; i < n; ++i) {
-          const double* J = &m.jacobian_matrix[9 * nidx(n, i, j, k, e)];
+          const double* J = &jacobian_matrix[9 * nidx(n, i, j, k, e)];
           // J[c + 3 d]; determinant by the rule of Sarrus
           const double det = J[0] * J[4] * J[8] + J[3] * J[7] * J[2] + J[6] * J[1] * J[5] -
                              J[6] * J[4] * J[2] - J[3] * J[1] * J[8] - J[0] * J[7] * J[5];
-          m.inverse_jacobian[nidx(n, i, j, k, e)] = 1.0 / det;
+          inverse_jacobian[nidx(n, i, j, k, e)] = 1.0 / det;
         }
   }
 }

@@ -56,6 +56,7 @@ class AtmoGeometry3D(ctypes.Structure):
         ("boundary_element", c_int32_p),
         ("boundary_node_indices", c_int32_p),
         ("boundary_condition", c_int32_p),
+        ("node_coordinates", c_double_p),
     ]
 
 
@@ -168,6 +169,10 @@ SIGNATURES = [
     ("atmo_get_state", ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     ("atmo_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]),
     ("atmo_max_dt", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, c_double_p]),
+    ("atmo_metric_terms3d", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_int32, c_double_p, c_double_p, c_double_p]),
+    ("atmo_metric_terms3d_device", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("atmo_solve", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
       ctypes.POINTER(ctypes.c_int64), c_double_p]),

@@ -32,9 +32,11 @@ class P4estMesh3D:
         ne, ni, nb = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         check(L.atmo_mesh3d_sizes(handle, ctypes.byref(ne), ctypes.byref(ni), ctypes.byref(nb)))
         self.n_elements, self.n_interfaces, self.n_boundaries = ne.value, ni.value, nb.value
-        ptrs_d = [c_double_p() for _ in range(3)]
+        xyz = c_double_p()
         ptrs_i = [c_int32_p() for _ in range(5)]
-        check(L.atmo_mesh3d_arrays(handle, *[ctypes.byref(p) for p in ptrs_d],
+        # the metric terms are not requested here: they are computed on first
+        # use (host) or by the semidiscretization on the device
+        check(L.atmo_mesh3d_arrays(handle, ctypes.byref(xyz), None, None,
                                    *[ctypes.byref(p) for p in ptrs_i]))
         E, I, B = self.n_elements, self.n_interfaces, self.n_boundaries
 
@@ -50,9 +52,8 @@ class P4estMesh3D:
                 return np.zeros(shape, dtype=np.int32, order="F")
             return np.ctypeslib.as_array(p, shape=(cnt,)).copy().reshape(shape, order="F")
 
-        self.node_coordinates = dcopy(ptrs_d[0], (3, n, n, n, E))
-        self.contravariant_vectors = dcopy(ptrs_d[1], (3, 3, n, n, n, E))
-        self.inverse_jacobian = dcopy(ptrs_d[2], (n, n, n, E))
+        self.node_coordinates = dcopy(xyz, (3, n, n, n, E))
+        self._metric = None
         self.interface_neighbor_ids = icopy(ptrs_i[0], (2, I))
         self.interface_node_indices = icopy(ptrs_i[1], (3, 2, I))
         self.boundary_element = icopy(ptrs_i[2], (B,))
@@ -63,6 +64,29 @@ class P4estMesh3D:
     @property
     def nnodes(self):
         return self.polydeg + 1
+
+    def _metric_terms(self):
+        if self._metric is None:
+            n, E = self.nnodes, self.n_elements
+            xyz = np.ascontiguousarray(self.node_coordinates.ravel(order="F"))
+            cv = np.empty(9 * n ** 3 * E)
+            ij = np.empty(n ** 3 * E)
+            check(lib().atmo_metric_terms3d(self.polydeg, E, xyz.ctypes.data_as(c_double_p),
+                                            cv.ctypes.data_as(c_double_p),
+                                            ij.ctypes.data_as(c_double_p)))
+            self._metric = (cv.reshape((3, 3, n, n, n, E), order="F"),
+                            ij.reshape((n, n, n, E), order="F"))
+        return self._metric
+
+    @property
+    def contravariant_vectors(self):
+        """[3,3,n,n,n,E] in Kopriva's invariant curl form (host, computed on first use)"""
+        return self._metric_terms()[0]
+
+    @property
+    def inverse_jacobian(self):
+        """[n,n,n,E] (host, computed on first use)"""
+        return self._metric_terms()[1]
 
 
 def P4estMesh(trees_per_dimension, *, polydeg, coordinates_min, coordinates_max,

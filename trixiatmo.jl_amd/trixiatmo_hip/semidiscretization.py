@@ -95,7 +95,8 @@ class SemidiscretizationHyperbolic:
     and `metric_terms` / `auxiliary_field` are the reference's keywords."""
 
     def __init__(self, mesh, equations, solver, boundary_conditions=None, source_terms=None,
-                 initial_condition=None, metric_terms=None, auxiliary_field=None):
+                 initial_condition=None, metric_terms=None, auxiliary_field=None,
+                 metric_terms_on_device=True):
         self.mesh, self.equations, self.solver = mesh, equations, solver
         self.initial_condition = initial_condition
         self.source_terms = source_terms
@@ -116,21 +117,32 @@ class SemidiscretizationHyperbolic:
         for b in range(mesh.n_boundaries):
             name = bc.get(kinds[int(mesh.boundary_kind[b])], "boundary_condition_slip_wall")
             bcodes[b] = BOUNDARY_CONDITIONS[name]
-        self._keep = [np.ascontiguousarray(mesh.contravariant_vectors.ravel(order="F")),
-                      np.ascontiguousarray(mesh.inverse_jacobian.ravel(order="F")),
+        # P4estMesh{3} metric terms: computed on the device from the node
+        # coordinates (the same bits as the host's), or uploaded from the host
+        if metric_terms_on_device:
+            geo_arrays = [None, None]
+            xyz = np.ascontiguousarray(mesh.node_coordinates.ravel(order="F"))
+        else:
+            geo_arrays = [np.ascontiguousarray(mesh.contravariant_vectors.ravel(order="F")),
+                          np.ascontiguousarray(mesh.inverse_jacobian.ravel(order="F"))]
+            xyz = None
+        self._keep = [*geo_arrays,
                       np.ascontiguousarray(mesh.interface_neighbor_ids.ravel(order="F")),
                       np.ascontiguousarray(mesh.interface_node_indices.ravel(order="F")),
                       np.ascontiguousarray(mesh.boundary_element),
                       np.ascontiguousarray(mesh.boundary_node_indices.ravel(order="F")),
                       bcodes]
         k = self._keep
-        geo = AtmoGeometry3D(contravariant_vectors=dptr(k[0]), inverse_jacobian=dptr(k[1]),
+        geo = AtmoGeometry3D(contravariant_vectors=dptr(k[0]) if k[0] is not None else None,
+                             inverse_jacobian=dptr(k[1]) if k[1] is not None else None,
                              n_interfaces=mesh.n_interfaces, interface_neighbor_ids=iptr(k[2]),
                              interface_node_indices=iptr(k[3]), n_boundaries=mesh.n_boundaries,
                              boundary_element=iptr(k[4]), boundary_node_indices=iptr(k[5]),
-                             boundary_condition=iptr(k[6]))
+                             boundary_condition=iptr(k[6]),
+                             node_coordinates=dptr(xyz) if xyz is not None else None)
         h = ctypes.c_void_p()
         check(lib().atmo_create(ctypes.byref(cfg), ctypes.byref(geo), ctypes.byref(h)))
+        del xyz
         self._h = h
         n = mesh.nnodes
         self.shape = (equations.nvariables, n, n, n, mesh.n_elements)
