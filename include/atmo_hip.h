@@ -404,6 +404,17 @@ int atmo_solve(atmo_handle* handle, double t_end, double cfl, int32_t check_ever
  * conserved variable (3D Euler: out[0..8]; covariant shallow water: out[0..2]). */
 int atmo_integrals(atmo_handle* handle, int32_t fresh_rhs, double* out);
 
+/* SaveSolutionCallback's solution_variables of the resident state, computed
+ * on the device: kind 0 cons2cons, kind 1 cons2prim (3D Euler,
+ * compressible_euler_energy_with_gravity_3d.jl:568-578; rainy Euler,
+ * compressible_rainy_euler_2d.jl:114-129) or cons2prim_and_vorticity
+ * (covariant shallow water, save_solution_covariant.jl:65-117: H, the global
+ * velocity, h_s, the relative vorticity).  out (host) receives
+ * [n_vars][nodes], each variable contiguous in Trixi's node order (the
+ * variables_<v> datasets of the solution file); n_vars (may be NULL) the
+ * number of variables: 6 (3D), 3 or 6 (covariant), 9 (rainy). */
+int atmo_solution_variables(atmo_handle* handle, int32_t kind, double* out, int32_t* n_vars);
+
 /* wait for all work of the handle's stream */
 int atmo_synchronize(atmo_handle* handle);
 

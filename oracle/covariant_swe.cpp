@@ -455,3 +455,43 @@ extern "C" void ora_swe_integrals(const ora_swe_params* p, const double* aux, co
         }
       }
 }
+
+// SaveSolutionCallback's solution_variables = cons2prim_and_vorticity
+// (src/callbacks_step/save_solution_covariant.jl:65-76): per node the total
+// height H = h + h_s (cons2prim, covariant_shallow_water.jl:113-118), the
+// contravariant velocity mapped to global components by the covariant basis
+// (contravariant2global, :140-145), h_s, and the relative vorticity of
+// calc_vorticity_node (save_solution_covariant.jl:92-117): the collocation
+// derivatives d1 of v_2 along i and d2 of v_1 along j, v_i = G_ij vcon^j,
+// divided by J.  out[6][nodes], variable-major.
+extern "C" void ora_swe_prim_vorticity(const ora_swe_params* p, const double* aux, const double* u,
+                                       double* out) {
+  const int n = p->n;
+  const size_t n_nodes = (size_t)n * n * p->n_elements;
+  std::vector<double> x(n), w(n), D(n * n);
+  ora_lgl_nodes_weights(n, x.data(), w.data());
+  ora_derivative_matrix(n, x.data(), D.data());
+  auto vcov = [&](size_t q, int c) {
+    const Aux a{aux + NA * q};
+    const double* un = u + NV * q;
+    const double vcon[2] = {un[1] / un[0], un[2] / un[0]};
+    return a.Gcov(c, 0) * vcon[0] + a.Gcov(c, 1) * vcon[1];
+  };
+  for (int e = 0; e < p->n_elements; ++e)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const size_t base = (size_t)n * n * e, q = base + i + n * j;
+        const Aux a{aux + NA * q};
+        const double* un = u + NV * q;
+        const double h = un[0], v1 = un[1] / h, v2 = un[2] / h;
+        double dv2dxi1 = 0.0, dv1dxi2 = 0.0;
+        for (int m = 0; m < n; ++m) {
+          dv2dxi1 += D[i * n + m] * vcov(base + m + n * j, 1);
+          dv1dxi2 += D[j * n + m] * vcov(base + i + n * m, 0);
+        }
+        const double w[6] = {h + a.hs(), a.A(0, 0) * v1 + a.A(0, 1) * v2,
+                             a.A(1, 0) * v1 + a.A(1, 1) * v2, a.A(2, 0) * v1 + a.A(2, 1) * v2,
+                             a.hs(), (dv2dxi1 - dv1dxi2) / a.J()};
+        for (int v = 0; v < 6; ++v) out[v * n_nodes + q] = w[v];
+      }
+}

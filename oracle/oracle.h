@@ -151,6 +151,10 @@ int ora_swe_ck2n54(const ora_swe_params* p, const ora_connectivity2d* c, const d
                    const double* xyz, double* u, double dt, int n_steps);
 void ora_swe_integrals(const ora_swe_params* p, const double* aux, const double* u,
                        const double* du, double* out);
+/* cons2prim_and_vorticity of SaveSolutionCallback: out[6][nodes] = (H, global
+ * velocity v1..v3, h_s, relative vorticity), variable-major */
+void ora_swe_prim_vorticity(const ora_swe_params* p, const double* aux, const double* u,
+                            double* out);
 
 /* ---- compressible rainy Euler (CompressibleRainyEulerEquations2D) on a
  *      uniform Cartesian StructuredMesh{2}; state [9, n, n, E] = (rho_dry,

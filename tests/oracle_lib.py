@@ -250,6 +250,13 @@ class SweOracle:
                                  _d(out))
         return out
 
+    def prim_vorticity(self, u):
+        """cons2prim_and_vorticity: [6, nodes] (H, v1, v2, v3, b, vorticity)"""
+        uf = self._u(u)
+        out = np.zeros(2 * uf.size)
+        self.L.ora_swe_prim_vorticity(ctypes.byref(self.p), _d(self._keep[0]), _d(uf), _d(out))
+        return out.reshape(6, -1)
+
     def solve(self, u0, t_end, cfl):
         """StepsizeCallback(cfl) + CarpenterKennedy2N54 to t_end (the last
         step cut to end at t_end)."""

@@ -1,6 +1,7 @@
-"""The C-ABI library loads on a machine without a GPU, exports every function
-declared in include/atmo_hip.h, and rejects bad configurations before it
-touches a device."""
+"""The C-ABI libraries load on a machine without a GPU and export every
+function their headers declare (include/atmo_hip.h: libatmo_hip.so;
+include/atmo_h5.h: libatmo_h5.so), and libatmo_hip rejects bad
+configurations before it touches a device."""
 import ctypes
 import os
 import re
@@ -14,23 +15,31 @@ from trixiatmo_hip import _lib
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
-    names = set()
-    for fn in os.listdir(os.path.join(ROOT, "include")):
-        if fn.endswith(".h"):
-            text = open(os.path.join(ROOT, "include", fn)).read()
-            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-            names |= set(re.findall(r"\b(atmo_[a-z0-9_]+)\s*\(", text))
-    return names
+def declared_functions(header="atmo_hip.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(atmo_[a-z0-9_]+)\s*\(", text))
 
 
 def test_every_declared_symbol_is_exported():
+    assert sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")) == \
+        ["atmo_h5.h", "atmo_hip.h"]
     L = _lib.lib()
     names = declared_functions()
     assert len(names) >= 20
     for name in sorted(names):
         assert hasattr(L, name), name
     assert names == {s[0] for s in _lib.SIGNATURES}
+
+
+def test_h5_library_exports_its_header():
+    from trixiatmo_hip import solution_files
+    L = solution_files.h5lib()
+    names = declared_functions("atmo_h5.h")
+    assert names == {"atmo_h5_write_solution", "atmo_h5_read_attributes",
+                     "atmo_h5_read_variable", "atmo_h5_last_error"}
+    for name in sorted(names):
+        assert hasattr(L, name), name
 
 
 def test_invalid_configurations_are_rejected():

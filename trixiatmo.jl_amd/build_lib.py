@@ -13,6 +13,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libatmo_hip.so")
+# the HDF5 solution/restart file library (host only); it links the HDF5 C
+# library of the image's conda tree and is skipped where that is absent
+H5_LIB = os.path.join(HERE, "libatmo_h5.so")
+H5_SRC = os.path.join(CSRC, "h5", "atmo_h5.cpp")
+H5_PREFIX = os.environ.get("ATMO_HDF5_PREFIX", "/opt/conda")
 ARCH = "gfx950"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -55,7 +60,26 @@ def build(jobs=8, verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, LIB)
+    build_h5()
     return LIB
+
+
+def build_h5():
+    """libatmo_h5.so; returns its path, or None when no HDF5 library is found"""
+    inc = os.path.join(H5_PREFIX, "include")
+    libdir = os.path.join(H5_PREFIX, "lib")
+    if not (os.path.exists(os.path.join(inc, "hdf5.h")) and
+            os.path.exists(os.path.join(libdir, "libhdf5.so"))):
+        sys.stderr.write(f"HDF5 not found under {H5_PREFIX}: libatmo_h5.so is not built\n")
+        return None
+    tmp = H5_LIB + ".tmp"
+    # the C++ runtime is linked in statically: the only library taken from the
+    # conda tree is libhdf5 (and what it needs), never a second libstdc++
+    _run(["g++", *CXX_FLAGS, "-shared", f"-I{inc}", H5_SRC, "-o", tmp,
+          "-static-libstdc++", "-static-libgcc",
+          os.path.join(libdir, "libhdf5.so"), f"-Wl,-rpath,{libdir}", "-Wl,--enable-new-dtags"])
+    os.replace(tmp, H5_LIB)
+    return H5_LIB
 
 
 if __name__ == "__main__":

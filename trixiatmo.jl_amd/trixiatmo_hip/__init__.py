@@ -16,7 +16,7 @@ from .semidiscretization import (MetricTermsCovariantSphere, SemidiscretizationE
                                  SemidiscretizationHyperbolic, solve)
 from .solvers import DGSEM, VolumeIntegralFluxDifferencing
 from .rainy import NonlinearSolveDG, SaturationMLP, SemidiscretizationRainy
-from . import analysis, workloads
+from . import analysis, solution_files, workloads
 
 __all__ = [
     "AtmoError", "lib", "EARTH_GRAVITATIONAL_ACCELERATION", "EARTH_RADIUS", "EARTH_ROTATION_RATE",
@@ -26,5 +26,5 @@ __all__ = [
     "DGSEM", "VolumeIntegralFluxDifferencing", "analysis", "workloads",
     "SplitCovariantShallowWaterEquations2D", "CovariantShallowWaterEquations2D", "P4estMeshCubedSphere2D", "MetricTermsCovariantSphere",
     "SemidiscretizationEnsemble", "solve", "NonlinearSolveDG", "SaturationMLP",
-    "SemidiscretizationRainy",
+    "SemidiscretizationRainy", "solution_files",
 ]
