@@ -169,7 +169,7 @@ int run_rhs(atmo_handle* h, double* du, const double* u) {
 
 // the five stages of one CarpenterKennedy2N54 step on the resident state
 // (Carpenter & Kennedy (1994) five-stage fourth-order 2N coefficients)
-int launch_rk_step(atmo_handle* h, E3Launch& L) {
+int launch_rk_step(atmo_handle* h, E3Launch& L, double* speed_partial = nullptr) {
   static const double A[5] = {0.0, -567301805773.0 / 1357537059087.0,
                               -2404267990393.0 / 2016746695238.0,
                               -3550918686646.0 / 2091501179385.0,
@@ -184,6 +184,7 @@ int launch_rk_step(atmo_handle* h, E3Launch& L) {
     L.out = h->d_u2;
     L.rk_a = A[st];
     L.rk_b = B[st];
+    L.res.speed_partial = st == 4 ? speed_partial : nullptr;
     const int rc = launch_stage(h, L);
     if (rc != ATMO_OK) return rc;
     std::swap(h->d_u, h->d_u2);
@@ -195,20 +196,29 @@ int launch_rk_step(atmo_handle* h, E3Launch& L) {
 
 namespace atmo {
 
-int e3_enqueue_resident_step(atmo_handle* h) {
+// per-element speeds of the state a resident loop starts from; every later
+// step's come from its own last stage
+int e3_resident_prologue(atmo_handle* h) {
   E3Launch L = base_launch(h);
   L.u = h->d_u;
-  TimerPair tp;
-  if (h->timing) timer_begin(h, 1, &tp);
   e3_launch_reduction(L, 0, nullptr, h->d_partial, h->d_result, h->stream);
-  if (h->timing) timer_end(h, &tp);
-  launch_step_control(h->d_result, 1, h->n, h->d_clock, h->d_clock + 1, h->stream);
   HIP_TRY(h, hipGetLastError());
+  return ATMO_OK;
+}
+
+// one step: the step size from the per-element speeds in d_partial, then the
+// five stages, the last of which writes the new state's speeds there
+int e3_enqueue_resident_step(atmo_handle* h) {
+  launch_step_control_reduce(h->d_partial, h->cfg.n_elements, h->n, h->d_clock, h->d_clock + 1,
+                             h->stream);
+  HIP_TRY(h, hipGetLastError());
+  E3Launch L = base_launch(h);
   L.mode = 1;
   L.k = h->d_k;
   L.rk_dt = 0.0;
-  L.dt_dev = h->d_clock + 1;
-  return launch_rk_step(h, L);
+  L.res.dt = h->d_clock + 1;
+  L.res.cv64 = h->d_cv;
+  return launch_rk_step(h, L, h->d_partial);
 }
 
 }  // namespace atmo

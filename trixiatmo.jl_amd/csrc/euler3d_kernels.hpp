@@ -15,6 +15,15 @@ struct Background {
   const float* B;  // [n nodes, 3] FP32 copy of the background's momentum tendency
 };
 
+// links of a stage to the resident time loop
+struct ResidentLink {
+  const double* dt;       // if set, the stage's time step is read here
+  double* speed_partial;  // if set (RK stage): per element, StepsizeCallback's
+                          // sum over directions of the maximum scaled speed of
+                          // the new state, as the max-speed reduction computes it
+  const double* cv64;     // FP64 contravariant vectors for that speed
+};
+
 struct E3Launch {
   E3Params q;
   int n, n_elements;
@@ -32,7 +41,7 @@ struct E3Launch {
   const int* nbr;  // [E, 6, n*n] neighbour global node id, < 0 boundary
   const int* face_info;  // [E, 6] 0 primary/boundary, 8 | dp secondary of primary direction dp
   double rk_a, rk_dt, rk_b;
-  const double* dt_dev;  // if set, the stage's time step is read here (resident time loop)
+  ResidentLink res;      // resident time loop: device step size, speeds of the new state
   double coriolis;  // 2 Omega, 0 = no source
   Background bg;
 };

@@ -111,6 +111,20 @@ __device__ __forceinline__ Prim<T> cons2prim(const E3Params& q, const T* u) {
   return w;
 }
 
+// StepsizeCallback's scaled wave speeds at one node: l[d] = |1/J| |Ja^d . a|
+// with a_c = |v_c| + c (src/callbacks_step/stepsize_dg2d.jl:5-43 in 3D);
+// the max-speed reduction and the last stage of a resident step both call
+// this one function, so their speeds agree bit for bit
+__device__ __forceinline__ void node_scaled_speeds(const E3Params& q, const double* uu,
+                                                   const double* ja, double ij_abs, double l[3]) {
+  const Prim<double> w = cons2prim(q, uu);
+  const double c = sqrt(q.gamma * w.p / w.rho);
+  const double a1 = fabs(w.v1) + c, a2 = fabs(w.v2) + c, a3 = fabs(w.v3) + c;
+  l[0] = ij_abs * fabs(ja[0] * a1 + ja[1] * a2 + ja[2] * a3);
+  l[1] = ij_abs * fabs(ja[3] * a1 + ja[4] * a2 + ja[5] * a3);
+  l[2] = ij_abs * fabs(ja[6] * a1 + ja[7] * a2 + ja[8] * a3);
+}
+
 // flux_ranocha, :382-415
 template <typename T>
 __device__ __forceinline__ void flux_ranocha(const E3Params& q, const Prim<T>& l, const Prim<T>& r,

@@ -32,16 +32,12 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_max_speed(E3Params q, cons
   double l1 = 0, l2 = 0, l3 = 0;
   if (t < N3) {
     const size_t g = (size_t)e * N3 + t;
-    double uu[6];
+    double uu[6], l[3];
     for (int v = 0; v < 6; ++v) uu[v] = u[g * 6 + v];
-    const Prim<double> w = cons2prim(q, uu);
-    const double c = sqrt(q.gamma * w.p / w.rho);
-    const double a1 = fabs(w.v1) + c, a2 = fabs(w.v2) + c, a3 = fabs(w.v3) + c;
-    const double* ja = cv + g * 9;
-    const double ij = fabs(invJ[g]);
-    l1 = ij * fabs(ja[0] * a1 + ja[1] * a2 + ja[2] * a3);
-    l2 = ij * fabs(ja[3] * a1 + ja[4] * a2 + ja[5] * a3);
-    l3 = ij * fabs(ja[6] * a1 + ja[7] * a2 + ja[8] * a3);
+    node_scaled_speeds(q, uu, cv + g * 9, fabs(invJ[g]), l);
+    l1 = l[0];
+    l2 = l[1];
+    l3 = l[2];
   }
   s[0][t] = l1; s[1][t] = l2; s[2][t] = l3;
   __syncthreads();

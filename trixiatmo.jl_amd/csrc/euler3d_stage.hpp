@@ -569,7 +569,7 @@ __device__ __forceinline__ void stage_epilogue(
     const Acc<T, KAHAN>* acc, const Acc<T, KAHAN>* acc_nc, const T* s_sf, int ti, int tj, int tk,
     size_t gnode, const double* __restrict__ u, double* __restrict__ out,
     double* __restrict__ kreg, const double* __restrict__ invJ, int mode, double rk_a,
-    double rk_dt, double rk_b, double coriolis, const Background& bg) {
+    double rk_dt, double rk_b, double coriolis, const Background& bg, double* un) {
   constexpr int NV = 6, N2 = Shape<N>::N2;
   double du[NV];
   du[0] = double(acc[0].value());
@@ -612,8 +612,37 @@ __device__ __forceinline__ void stage_epilogue(
     for (int v = 0; v < NV; ++v) {
       const double kn = rk_a * kreg[gnode * NV + v] + rk_dt * du[v];
       kreg[gnode * NV + v] = kn;
-      out[gnode * NV + v] = uo[v] + rk_b * kn;
+      un[v] = uo[v] + rk_b * kn;
+      out[gnode * NV + v] = un[v];
     }
+  }
+}
+
+// per element: the sum over directions of the maximum scaled speed of the
+// new states un (the resident loop's next step size), reduced over the
+// workgroup with fmax (exact, so any order gives the max-speed reduction's
+// value); every thread of the workgroup takes part
+template <int B>
+__device__ __forceinline__ void element_speed(const E3Params& q, bool active, const double* un,
+                                              const double* __restrict__ cv64,
+                                              const double* __restrict__ invJ, size_t gnode,
+                                              int t, double (*red)[3], double* partial) {
+  double l[3] = {0.0, 0.0, 0.0};
+  if (active) node_scaled_speeds(q, un, cv64 + gnode * 9, fabs(invJ[gnode]), l);
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) l[d] = fmax(l[d], __shfl_xor(l[d], o, 64));
+  if ((t & 63) == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) red[t >> 6][d] = l[d];
+  }
+  __syncthreads();
+  if (t == 0) {
+    double m[3] = {red[0][0], red[0][1], red[0][2]};
+    for (int w = 1; w < B / 64; ++w)
+      for (int d = 0; d < 3; ++d) m[d] = fmax(m[d], red[w][d]);
+    *partial = m[0] + m[1] + m[2];
   }
 }
 
@@ -625,7 +654,7 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
     double* __restrict__ out, double* __restrict__ kreg, const T* __restrict__ cv,
     const double* __restrict__ invJ, const int* __restrict__ nbr,
     const int* __restrict__ face_info, int surf, int snc, int mode, double rk_a, double rk_dt,
-    const double* __restrict__ dt_dev, double rk_b, double coriolis, Background bg) {
+    ResidentLink res, double rk_b, double coriolis, Background bg) {
   using S = Shape<N>;
   using C = Chunking<N, T, NC>;
   constexpr int N2 = S::N2, N3 = S::N3, P = S::P, B = S::BLOCK;
@@ -733,9 +762,14 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
   face_fluxes_to_lds<N, T>(q, e, t, inv_w, u, pf, fn, nbr, face_info, surf, snc, bg, s_w, s_pb,
                            s_fl);
   __syncthreads();
-  if (!active) return;
-  stage_epilogue<N, T, KAHAN>(acc, acc_nc, s_fl, ti, tj, tk, gnode, u, out, kreg, invJ, mode,
-                              rk_a, dt_dev ? dt_dev[0] : rk_dt, rk_b, coriolis, bg);
+  double un[6];
+  if (active)
+    stage_epilogue<N, T, KAHAN>(acc, acc_nc, s_fl, ti, tj, tk, gnode, u, out, kreg, invJ, mode,
+                                rk_a, res.dt ? res.dt[0] : rk_dt, rk_b, coriolis, bg, un);
+  if (res.speed_partial) {
+    __shared__ double s_red[B / 64][3];
+    element_speed<B>(q, active, un, res.cv64, invJ, gnode, t, s_red, res.speed_partial + e);
+  }
 }
 
 // Direct form: every node evaluates the two-point flux with each of its n-1
@@ -752,7 +786,7 @@ void e3_stage_direct(
     double* __restrict__ out, double* __restrict__ kreg, const T* __restrict__ cv,
     const double* __restrict__ invJ, const int* __restrict__ nbr,
     const int* __restrict__ face_info, int surf, int snc, int mode, double rk_a, double rk_dt,
-    const double* __restrict__ dt_dev, double rk_b, double coriolis, Background bg) {
+    ResidentLink res, double rk_b, double coriolis, Background bg) {
   using S = Shape<N>;
   constexpr int N2 = S::N2, N3 = S::N3;
   constexpr bool MIXED = sizeof(T) == 4;
@@ -875,9 +909,15 @@ void e3_stage_direct(
   face_fluxes_to_lds<N, T>(q, e, t, inv_w, u, pf, fn, nbr, face_info, surf, snc, bg, s_w, s_pb,
                            s_buf);
   __syncthreads();
-  if (!active) return;
-  stage_epilogue<N, T, KAHAN>(acc, acc_nc, s_buf, ti, tj, tk, gnode, u, out, kreg, invJ, mode,
-                              rk_a, dt_dev ? dt_dev[0] : rk_dt, rk_b, coriolis, bg);
+  double un[6];
+  if (active)
+    stage_epilogue<N, T, KAHAN>(acc, acc_nc, s_buf, ti, tj, tk, gnode, u, out, kreg, invJ, mode,
+                                rk_a, res.dt ? res.dt[0] : rk_dt, rk_b, coriolis, bg, un);
+  if (res.speed_partial) {
+    __shared__ double s_red[S::BLOCK / 64][3];
+    element_speed<S::BLOCK>(q, active, un, res.cv64, invJ, gnode, t, s_red,
+                            res.speed_partial + e);
+  }
 }
 
 // which volume form a shape runs, from timings on MI355X: the direct form for
@@ -899,13 +939,13 @@ void launch_stage_inst(const E3Launch& L, hipStream_t s) {
     hipLaunchKernelGGL((e3_stage_direct<N, VOL, NC, T, KAHAN>), dim3(grid),
                        dim3(Shape<N>::BLOCK), 0, s, L.q, dst, L.inv_w, L.n_elements, L.u, L.out,
                        L.k, reinterpret_cast<const T*>(L.cv_t), L.invJ, L.nbr, L.face_info,
-                       L.surface_flux, L.surface_noncons, L.mode, L.rk_a, L.rk_dt, L.dt_dev, L.rk_b,
+                       L.surface_flux, L.surface_noncons, L.mode, L.rk_a, L.rk_dt, L.res, L.rk_b,
                        L.coriolis, L.bg);
   else
     hipLaunchKernelGGL((e3_stage<N, VOL, NC, T, KAHAN>), dim3(grid), dim3(Shape<N>::BLOCK), 0, s,
                        L.q, dst, L.inv_w, L.n_elements, L.u, L.out, L.k,
                        reinterpret_cast<const T*>(L.cv_t), L.invJ, L.nbr, L.face_info,
-                       L.surface_flux, L.surface_noncons, L.mode, L.rk_a, L.rk_dt, L.dt_dev, L.rk_b,
+                       L.surface_flux, L.surface_noncons, L.mode, L.rk_a, L.rk_dt, L.res, L.rk_b,
                        L.coriolis, L.bg);
 }
 

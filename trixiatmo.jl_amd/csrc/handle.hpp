@@ -103,6 +103,12 @@ int rainy_enqueue_resident_step(atmo_handle* h);
 // dt[m] and the clock of n_members members from their maximum wave speeds
 void launch_step_control(const double* max_speed, int n_members, int n, double* clock, double* dt,
                          hipStream_t s);
+// the same for one member from per-element partial maxima
+void launch_step_control_reduce(const double* partial, int n_items, int n, double* clock,
+                                double* dt, hipStream_t s);
+// set-up of a resident loop before its steps are enqueued (the first step's
+// speeds where the stages provide the later ones)
+int e3_resident_prologue(atmo_handle* h);
 }  // namespace atmo
 
 #endif

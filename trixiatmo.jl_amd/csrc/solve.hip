@@ -29,10 +29,8 @@ namespace {
 // the step just taken is added to t, then the next step size is
 // dt = min(cfl * 2 / (n * max speed), t_end - t) as atmo_max_dt and the host
 // loop compute it (same operations in the same order), or 0 once t >= t_end
-__global__ void step_control_kernel(const double* __restrict__ max_speed, int M, int n,
-                                    double* clock, double* dt) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= M) return;
+__device__ __forceinline__ void step_control_kernel_body(const double* max_speed, int m, int M,
+                                                         int n, double* clock, double* dt) {
   const double t_end = clock[3 * M + 1], cfl = clock[3 * M + 2];
   const double t = clock[m] + dt[m];
   double d = 0.0;
@@ -50,12 +48,45 @@ __global__ void step_control_kernel(const double* __restrict__ max_speed, int M,
   dt[m] = d;
 }
 
+__global__ void step_control_kernel(const double* __restrict__ max_speed, int M, int n,
+                                    double* clock, double* dt) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < M) step_control_kernel_body(max_speed, m, M, n, clock, dt);
+}
+
+// the same for one member whose maximum speed is the fmax over per-element
+// partials (written by the last stage of the step just taken)
+__global__ __launch_bounds__(1024) void step_control_reduce_kernel(const double* __restrict__ partial,
+                                                                   int n_items, int n,
+                                                                   double* clock, double* dt) {
+  __shared__ double s[1024 / 64];
+  const int t = threadIdx.x;
+  double m = 0.0;
+  for (int i = t; i < n_items; i += 1024) m = fmax(m, partial[i]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((t & 63) == 0) s[t >> 6] = m;
+  __syncthreads();
+  if (t == 0) {
+    for (int w = 1; w < 1024 / 64; ++w) m = fmax(m, s[w]);
+    s[0] = m;
+  }
+  __syncthreads();
+  if (t == 0) step_control_kernel_body(&s[0], 0, 1, n, clock, dt);
+}
+
 }  // namespace
 
 void launch_step_control(const double* max_speed, int n_members, int n, double* clock, double* dt,
                          hipStream_t s) {
   hipLaunchKernelGGL(step_control_kernel, dim3((n_members + 63) / 64), dim3(64), 0, s, max_speed,
                      n_members, n, clock, dt);
+}
+
+void launch_step_control_reduce(const double* partial, int n_items, int n, double* clock,
+                                double* dt, hipStream_t s) {
+  hipLaunchKernelGGL(step_control_reduce_kernel, dim3(1), dim3(1024), 0, s, partial, n_items, n,
+                     clock, dt);
 }
 
 }  // namespace atmo
@@ -124,6 +155,10 @@ int atmo_solve(atmo_handle* h, double t_end, double cfl, int32_t check_every, in
   }
   h->du_valid = false;
   bool done = !(h->time < t_end);
+  if (!done && h->eq == ATMO_EQ_EULER_ENERGY_GRAVITY_3D) {
+    const int rc = atmo::e3_resident_prologue(h);
+    if (rc != ATMO_OK) return rc;
+  }
   hipGraphExec_t graph = nullptr;
   if (!h->timing && !done) {
     const int rc = capture_two_steps(h, &graph);
