@@ -157,3 +157,25 @@ def test_resident_loop_rainy_convergence_regression():
     l2, linf = calc_error_norms_2d(ur, mesh, exact, 2.0)
     np.testing.assert_allclose(l2, CONV_L2, rtol=1e-7)
     np.testing.assert_allclose(linf, CONV_LINF, rtol=1e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_kahan"])
+def test_euler3d_step_graph_matches_launches_one_by_one(precision):
+    """atmo_step replays pairs of 3D steps from a captured graph whose stages
+    read dt from device memory: the same bits as steps launched one by one,
+    across odd step counts (which swap the state buffers) and changes of dt"""
+    print(f"case: 3D shell 2x2 trees x 2 layers polydeg 3 {precision}", flush=True)
+    make, u0 = euler3d(W.shell, trees_per_face=2, layers=2, polydeg=3, seed=4,
+                       precision=precision)
+    semi = make()
+    semi.set_state(u0)
+    dt = semi.max_dt(0.5)
+    plan = [(dt, 2), (dt, 3), (0.7 * dt, 2), (dt, 4)]
+    for d, k in plan:
+        semi.step(d, k)
+    a = semi.get_state()
+    semi.set_state(u0)
+    for d, k in plan:
+        for _ in range(k):
+            semi.step(d, 1)
+    np.testing.assert_array_equal(a, semi.get_state())

@@ -192,6 +192,37 @@ int launch_rk_step(atmo_handle* h, E3Launch& L, double* speed_partial = nullptr)
   return ATMO_OK;
 }
 
+// two RK steps (ten stage launches, after which the ping-pong buffers are
+// back in place) captured once per state buffer, dt read from d_step_dt
+int ensure_e3_graph(atmo_handle* h) {
+  if (h->e3_graph && h->e3_graph_u == h->d_u) return ATMO_OK;
+  if (h->e3_graph) {
+    HIP_TRY(h, hipGraphExecDestroy(h->e3_graph));
+    h->e3_graph = nullptr;
+  }
+  if (!h->d_step_dt) HIP_TRY(h, hipMalloc(&h->d_step_dt, sizeof(double)));
+  E3Launch L = base_launch(h);
+  L.mode = 1;
+  L.k = h->d_k;
+  L.rk_dt = 0.0;
+  L.res.dt = h->d_step_dt;
+  hipGraph_t g = nullptr;
+  HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = launch_rk_step(h, L);
+  if (rc == ATMO_OK) rc = launch_rk_step(h, L);
+  const hipError_t ec = hipStreamEndCapture(h->stream, &g);
+  if (rc != ATMO_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  HIP_TRY(h, ec);
+  const hipError_t ei = hipGraphInstantiate(&h->e3_graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIP_TRY(h, ei);
+  h->e3_graph_u = h->d_u;
+  return ATMO_OK;
+}
+
 }  // namespace
 
 namespace atmo {
@@ -523,6 +554,8 @@ void atmo_destroy(atmo_handle* h) {
   for (double* p : swe_bufs)
     if (p) hipFree(p);
   if (h->d_clock) hipFree(h->d_clock);
+  if (h->d_step_dt) hipFree(h->d_step_dt);
+  if (h->e3_graph) hipGraphExecDestroy(h->e3_graph);
   if (h->d_member) hipFree(h->d_member);
   if (h->d_member_first) hipFree(h->d_member_first);
   if (h->d_face_info) hipFree(h->d_face_info);
@@ -595,7 +628,20 @@ int atmo_step(atmo_handle* h, double dt, int32_t n_steps) {
   L.mode = 1;
   L.k = h->d_k;
   L.rk_dt = dt;
-  for (int s = 0; s < n_steps; ++s) {
+  int done = 0;
+  if (!h->timing && n_steps >= 2) {
+    // pairs of steps from a captured graph whose stages read dt from memory
+    const int rc = ensure_e3_graph(h);
+    if (rc != ATMO_OK) return rc;
+    launch_set_scalar(h->d_step_dt, dt, h->stream);
+    HIP_TRY(h, hipGetLastError());
+    for (; done + 2 <= n_steps; done += 2) {
+      HIP_TRY(h, hipGraphLaunch(h->e3_graph, h->stream));
+      h->time += dt;
+      h->time += dt;
+    }
+  }
+  for (; done < n_steps; ++done) {
     const int rc = launch_rk_step(h, L);
     if (rc != ATMO_OK) return rc;
     h->time += dt;

@@ -62,6 +62,10 @@ struct atmo_handle {
   atmo::RainyLaunch rainy_base{};
   double time = 0.0;                       // time of the resident state
   double* d_clock = nullptr;               // device clock of the resident time loop
+  // 3D atmo_step: two RK steps captured for replay, their step size read from d_step_dt
+  hipGraphExec_t e3_graph = nullptr;
+  double* e3_graph_u = nullptr;            // state buffer the graph was captured with
+  double* d_step_dt = nullptr;
   unsigned long long* d_stats = nullptr;   // Newton statistics
   atmo_mlp* mlp = nullptr;                 // seed of the saturation solve
   bool du_valid = false;
@@ -100,6 +104,8 @@ void rainy_free(atmo_handle* h);
 int e3_enqueue_resident_step(atmo_handle* h);
 int swe_enqueue_resident_step(atmo_handle* h);
 int rainy_enqueue_resident_step(atmo_handle* h);
+// *ptr = value, stream-ordered (a value for kernels that read it from memory)
+void launch_set_scalar(double* ptr, double value, hipStream_t s);
 // dt[m] and the clock of n_members members from their maximum wave speeds
 void launch_step_control(const double* max_speed, int n_members, int n, double* clock, double* dt,
                          hipStream_t s);

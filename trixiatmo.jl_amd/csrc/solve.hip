@@ -83,6 +83,16 @@ void launch_step_control(const double* max_speed, int n_members, int n, double* 
                      n_members, n, clock, dt);
 }
 
+namespace {
+__global__ void set_scalar_kernel(double* ptr, double value) {
+  if (threadIdx.x == 0) *ptr = value;
+}
+}  // namespace
+
+void launch_set_scalar(double* ptr, double value, hipStream_t s) {
+  hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(64), 0, s, ptr, value);
+}
+
 void launch_step_control_reduce(const double* partial, int n_items, int n, double* clock,
                                 double* dt, hipStream_t s) {
   hipLaunchKernelGGL(step_control_reduce_kernel, dim3(1), dim3(1024), 0, s, partial, n_items, n,
