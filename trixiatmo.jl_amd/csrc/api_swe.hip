@@ -73,7 +73,7 @@ const double kB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612
 
 // the 5 stage launches of n_steps CarpenterKennedy2N54 steps, time steps read
 // from d_dt_member
-int enqueue_steps(atmo_handle* h, int n_steps) {
+int enqueue_steps(atmo_handle* h, int n_steps, double* speed_partial = nullptr) {
   SweLaunch L = h->swe_base;
   L.mode = 1;
   L.k = h->d_k;
@@ -84,6 +84,7 @@ int enqueue_steps(atmo_handle* h, int n_steps) {
       L.out = h->d_u2;
       L.rk_a = kA[st];
       L.rk_b = kB[st];
+      L.speed_partial = st == 4 ? speed_partial : nullptr;
       const int rc = launch(h, L);
       if (rc != ATMO_OK) return rc;
       std::swap(h->d_u, h->d_u2);
@@ -121,18 +122,25 @@ int ensure_graph(atmo_handle* h) {
 
 }  // namespace
 
-int swe_enqueue_resident_step(atmo_handle* h) {
+// per-element speeds of the state a resident loop starts from; every later
+// step's come from its own last stage
+int swe_resident_prologue(atmo_handle* h) {
   SweLaunch L = h->swe_base;
   L.u = h->d_u;
-  TimerPair tp;
-  if (h->timing) timer_begin(h, 1, &tp);
-  const bool ok = swe_launch_member_max_speed(L, h->d_partial, h->d_member_first, h->n_members,
-                                              h->d_result, h->stream);
-  if (h->timing) timer_end(h, &tp);
-  if (!ok) return fail(&h->err, ATMO_ERR_INVALID, "no covariant kernel for this polydeg");
-  launch_step_control(h->d_result, h->n_members, h->n, h->d_clock, h->d_dt_member, h->stream);
+  if (!swe_launch_member_max_speed(L, h->d_partial, h->d_member_first, h->n_members, h->d_result,
+                                   h->stream))
+    return fail(&h->err, ATMO_ERR_INVALID, "no covariant kernel for this polydeg");
   HIP_TRY(h, hipGetLastError());
-  return enqueue_steps(h, 1);
+  return ATMO_OK;
+}
+
+// one step: every member's step size from its elements' speeds in d_partial,
+// then the five stages, the last of which writes the new speeds there
+int swe_enqueue_resident_step(atmo_handle* h) {
+  launch_step_control_members_reduce(h->d_partial, h->d_member_first, h->n_members, h->n,
+                                     h->d_clock, h->d_dt_member, h->stream);
+  HIP_TRY(h, hipGetLastError());
+  return enqueue_steps(h, 1, h->d_partial);
 }
 
 int swe_autotune(atmo_handle* h, double* ms_per_variant) {

@@ -115,6 +115,11 @@ void launch_step_control_reduce(const double* partial, int n_items, int n, doubl
 // set-up of a resident loop before its steps are enqueued (the first step's
 // speeds where the stages provide the later ones)
 int e3_resident_prologue(atmo_handle* h);
+int swe_resident_prologue(atmo_handle* h);
+// the same per ensemble member over its element range
+void launch_step_control_members_reduce(const double* partial, const int* member_first,
+                                        int n_members, int n, double* clock, double* dt,
+                                        hipStream_t s);
 }  // namespace atmo
 
 #endif

@@ -29,13 +29,13 @@ namespace {
 // the step just taken is added to t, then the next step size is
 // dt = min(cfl * 2 / (n * max speed), t_end - t) as atmo_max_dt and the host
 // loop compute it (same operations in the same order), or 0 once t >= t_end
-__device__ __forceinline__ void step_control_kernel_body(const double* max_speed, int m, int M,
-                                                         int n, double* clock, double* dt) {
+__device__ __forceinline__ void step_control_kernel_body(double max_speed, int m, int M, int n,
+                                                         double* clock, double* dt) {
   const double t_end = clock[3 * M + 1], cfl = clock[3 * M + 2];
   const double t = clock[m] + dt[m];
   double d = 0.0;
   if (t < t_end) {
-    double mx = max_speed[m];
+    double mx = max_speed;
     if (!isfinite(mx)) {
       clock[3 * M] = 1.0;  // every member that finds one stores the same value
     } else {
@@ -51,7 +51,7 @@ __device__ __forceinline__ void step_control_kernel_body(const double* max_speed
 __global__ void step_control_kernel(const double* __restrict__ max_speed, int M, int n,
                                     double* clock, double* dt) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m < M) step_control_kernel_body(max_speed, m, M, n, clock, dt);
+  if (m < M) step_control_kernel_body(max_speed[m], m, M, n, clock, dt);
 }
 
 // the same for one member whose maximum speed is the fmax over per-element
@@ -72,10 +72,36 @@ __global__ __launch_bounds__(1024) void step_control_reduce_kernel(const double*
     s[0] = m;
   }
   __syncthreads();
-  if (t == 0) step_control_kernel_body(&s[0], 0, 1, n, clock, dt);
+  if (t == 0) step_control_kernel_body(s[0], 0, 1, n, clock, dt);
+}
+
+// one workgroup per ensemble member: its maximum speed is the fmax over the
+// per-element partials of its elements [member_first[m], member_first[m + 1])
+__global__ __launch_bounds__(256) void step_control_members_reduce_kernel(
+    const double* __restrict__ partial, const int* __restrict__ member_first, int M, int n,
+    double* clock, double* dt) {
+  __shared__ double s[256 / 64];
+  const int m = blockIdx.x, t = threadIdx.x;
+  double mx = 0.0;
+  for (int e = member_first[m] + t; e < member_first[m + 1]; e += 256) mx = fmax(mx, partial[e]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  if ((t & 63) == 0) s[t >> 6] = mx;
+  __syncthreads();
+  if (t == 0) {
+    for (int w = 1; w < 256 / 64; ++w) mx = fmax(mx, s[w]);
+    step_control_kernel_body(mx, m, M, n, clock, dt);
+  }
 }
 
 }  // namespace
+
+void launch_step_control_members_reduce(const double* partial, const int* member_first,
+                                        int n_members, int n, double* clock, double* dt,
+                                        hipStream_t s) {
+  hipLaunchKernelGGL(step_control_members_reduce_kernel, dim3(n_members), dim3(256), 0, s, partial,
+                     member_first, n_members, n, clock, dt);
+}
 
 void launch_step_control(const double* max_speed, int n_members, int n, double* clock, double* dt,
                          hipStream_t s) {
@@ -165,8 +191,10 @@ int atmo_solve(atmo_handle* h, double t_end, double cfl, int32_t check_every, in
   }
   h->du_valid = false;
   bool done = !(h->time < t_end);
-  if (!done && h->eq == ATMO_EQ_EULER_ENERGY_GRAVITY_3D) {
-    const int rc = atmo::e3_resident_prologue(h);
+  if (!done) {
+    const int rc = h->eq == ATMO_EQ_EULER_ENERGY_GRAVITY_3D ? atmo::e3_resident_prologue(h)
+                   : swe                                  ? atmo::swe_resident_prologue(h)
+                                                          : ATMO_OK;
     if (rc != ATMO_OK) return rc;
   }
   hipGraphExec_t graph = nullptr;

@@ -223,7 +223,19 @@ constexpr int kSweFields = 15;
 
 }  // namespace
 
-template <int N, int WAVES, bool PERSIST>
+// StepsizeCallback's scaled wave speeds at one node (stepsize_dg2d.jl:47-71):
+// |v^i| + sqrt(G^ii h g); the max-speed reduction and the last stage of a
+// resident step both call this one function, so their speeds agree bit for bit
+__device__ __forceinline__ void swe_node_speeds(const double* u, double gcon11, double gcon22,
+                                                double gravity, double l[2]) {
+  l[0] = fabs(u[1] / u[0]) + sqrt(gcon11 * u[0] * gravity);
+  l[1] = fabs(u[2] / u[0]) + sqrt(gcon22 * u[0] * gravity);
+}
+
+// SPEED: the RK stage also writes L.speed_partial (the last stage of a
+// resident step); a variant of its own, so the other stages keep their
+// register budget
+template <int N, int WAVES, bool PERSIST, bool SPEED>
 __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
   using S = SweShape<N, WAVES>;
   constexpr int N2 = S::N2, EPB = S::EPB, B = S::BLOCK;
@@ -398,12 +410,35 @@ __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
         for (int v = 0; v < NV; ++v) L.out[g * NV + v] = du[v];
       } else {
         const double dt = L.dt_member ? L.dt_member[L.member[e]] : L.rk_dt;
+        double un[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           const double kn = L.rk_a * L.k[g * NV + v] + dt * du[v];
           L.k[g * NV + v] = kn;
-          L.out[g * NV + v] = uo[v] + L.rk_b * kn;
+          un[v] = uo[v] + L.rk_b * kn;
+          L.out[g * NV + v] = un[v];
         }
+        if constexpr (SPEED) {
+          // the node's speeds go to its first two LDS fields, which only this
+          // lane reads after the face phase and which it has read already
+          double l[2];
+          swe_node_speeds(un, s_n[es][11][q], s_n[es][13][q], L.gravity, l);
+          s_n[es][0][q] = l[0];
+          s_n[es][1][q] = l[1];
+        }
+      }
+    }
+    if constexpr (SPEED) {
+      // per element: fmax over its nodes of each speed (exact in any order),
+      // then the sum over the directions, by the element's first lane
+      __syncthreads();
+      if (active && q == 0) {
+        double m1 = s_n[es][0][0], m2 = s_n[es][1][0];
+        for (int r = 1; r < N2; ++r) {
+          m1 = fmax(m1, s_n[es][0][r]);
+          m2 = fmax(m2, s_n[es][1][r]);
+        }
+        L.speed_partial[e] = m1 + m2;
       }
     }
   }
@@ -420,8 +455,10 @@ __global__ __launch_bounds__(64) void swe_max_speed(SweLaunch L, double* partial
     const size_t g = (size_t)e * N2 + q;
     const double* u = L.u + g * NV;
     const AuxV a{L.aux + g * NA, 1};
-    l1 = fmax(l1, fabs(u[1] / u[0]) + sqrt(a.Gcon(0, 0) * u[0] * L.gravity));
-    l2 = fmax(l2, fabs(u[2] / u[0]) + sqrt(a.Gcon(1, 1) * u[0] * L.gravity));
+    double l[2];
+    swe_node_speeds(u, a.Gcon(0, 0), a.Gcon(1, 1), L.gravity, l);
+    l1 = fmax(l1, l[0]);
+    l2 = fmax(l2, l[1]);
   }
   s1[t] = l1;
   s2[t] = l2;
@@ -460,7 +497,10 @@ void launch_variant(const SweLaunch& L, hipStream_t s) {
   const int groups = (L.n_elements + S::EPB - 1) / S::EPB;
   // persistent: a few workgroups per CU (256 CUs), each looping over groups
   const int grid = PERSIST ? (groups < 256 * 8 ? groups : 256 * 8) : groups;
-  hipLaunchKernelGGL((swe_stage<N, WAVES, PERSIST>), dim3(grid), dim3(S::BLOCK), 0, s, L);
+  if (L.speed_partial && L.mode == 1)
+    hipLaunchKernelGGL((swe_stage<N, WAVES, PERSIST, true>), dim3(grid), dim3(S::BLOCK), 0, s, L);
+  else
+    hipLaunchKernelGGL((swe_stage<N, WAVES, PERSIST, false>), dim3(grid), dim3(S::BLOCK), 0, s, L);
 }
 
 template <int N>

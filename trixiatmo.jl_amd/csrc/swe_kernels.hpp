@@ -34,6 +34,8 @@ struct SweLaunch {
   int mode;             // 0 rhs, 1 RK stage
   double rk_a, rk_dt, rk_b;
   const double* dt_member;  // per-member time steps (NULL: rk_dt for all)
+  double* speed_partial;    // RK stage, if set: per element the StepsizeCallback sum over
+                            // directions of the maximum scaled speed of the new state
   const int* member;        // [E] member of each element
   int variant;              // workgroup shape: 0..2 = 1, 2, 4 waves; +3 persistent grid
 };
