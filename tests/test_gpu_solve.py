@@ -179,3 +179,29 @@ def test_euler3d_step_graph_matches_launches_one_by_one(precision):
         for _ in range(k):
             semi.step(d, 1)
     np.testing.assert_array_equal(a, semi.get_state())
+
+
+@pytest.mark.parametrize("kind", ["euler3d", "covariant", "rainy"])
+def test_a_nan_state_is_reported_not_stepped(kind):
+    """a wave speed that is not a number makes max_dt and the resident loop
+    fail with ATMO_ERR_NONFINITE (the reference's max_dt propagates NaN
+    through Julia's max and the integrator stops)"""
+    print(f"case: NaN in the state, {kind}", flush=True)
+    if kind == "euler3d":
+        make, u0 = euler3d(W.periodic_cube, trees=2, polydeg=3, seed=1)
+    elif kind == "covariant":
+        case = W.zonal_jet(3, 3, seed=1)
+        make, u0 = case.semi, case.u0()
+    else:
+        make, u0 = rainy_make()
+    flat = np.array(u0, copy=True).ravel(order="F")
+    flat[u0.shape[0] * 7 + 1] = np.nan  # the second variable of node 7
+    u = flat.reshape(u0.shape, order="F")
+    semi = make()
+    semi.set_state(u)
+    with pytest.raises(T.AtmoError, match="NONFINITE"):
+        semi.max_dt(0.5)
+    semi.set_state(u)
+    semi.set_time(0.0)
+    with pytest.raises(T.AtmoError, match="NONFINITE"):
+        semi.solve_resident(1.0, 0.5)

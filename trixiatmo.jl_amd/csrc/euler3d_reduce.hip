@@ -5,6 +5,7 @@
 
 #include "euler3d_kernels.hpp"
 #include "euler3d_physics.hpp"
+#include "speed_max.hpp"
 
 namespace atmo {
 
@@ -43,9 +44,9 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_max_speed(E3Params q, cons
   __syncthreads();
   for (int w = B / 2; w > 0; w >>= 1) {
     if (t < w) {
-      s[0][t] = fmax(s[0][t], s[0][t + w]);
-      s[1][t] = fmax(s[1][t], s[1][t + w]);
-      s[2][t] = fmax(s[2][t], s[2][t + w]);
+      s[0][t] = speed_max(s[0][t], s[0][t + w]);
+      s[1][t] = speed_max(s[1][t], s[1][t + w]);
+      s[2][t] = speed_max(s[2][t], s[2][t + w]);
     }
     __syncthreads();
   }
@@ -106,12 +107,12 @@ __global__ __launch_bounds__(256) void reduce_columns(const double* __restrict__
     double acc = op == 0 ? 0.0 : -1e300;
     for (int i = t; i < n_items; i += 256) {
       const double x = partial[(size_t)i * width + c];
-      acc = op == 0 ? acc + x : fmax(acc, x);
+      acc = op == 0 ? acc + x : speed_max(acc, x);
     }
     s[t] = acc;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-      if (t < w) s[t] = op == 0 ? s[t] + s[t + w] : fmax(s[t], s[t + w]);
+      if (t < w) s[t] = op == 0 ? s[t] + s[t + w] : speed_max(s[t], s[t + w]);
       __syncthreads();
     }
     if (t == 0) result[c] = s[0];

@@ -30,6 +30,7 @@
 
 #include "euler3d_kernels.hpp"
 #include "euler3d_physics.hpp"
+#include "speed_max.hpp"
 
 namespace atmo {
 
@@ -620,8 +621,8 @@ __device__ __forceinline__ void stage_epilogue(
 
 // per element: the sum over directions of the maximum scaled speed of the
 // new states un (the resident loop's next step size), reduced over the
-// workgroup with fmax (exact, so any order gives the max-speed reduction's
-// value); every thread of the workgroup takes part
+// workgroup with speed_max (exact, so any order gives the max-speed
+// reduction's value); every thread of the workgroup takes part
 template <int B>
 __device__ __forceinline__ void element_speed(const E3Params& q, bool active, const double* un,
                                               const double* __restrict__ cv64,
@@ -632,7 +633,7 @@ __device__ __forceinline__ void element_speed(const E3Params& q, bool active, co
 #pragma unroll
   for (int d = 0; d < 3; ++d)
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) l[d] = fmax(l[d], __shfl_xor(l[d], o, 64));
+    for (int o = 32; o > 0; o >>= 1) l[d] = speed_max(l[d], __shfl_xor(l[d], o, 64));
   if ((t & 63) == 0) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) red[t >> 6][d] = l[d];
@@ -641,7 +642,7 @@ __device__ __forceinline__ void element_speed(const E3Params& q, bool active, co
   if (t == 0) {
     double m[3] = {red[0][0], red[0][1], red[0][2]};
     for (int w = 1; w < B / 64; ++w)
-      for (int d = 0; d < 3; ++d) m[d] = fmax(m[d], red[w][d]);
+      for (int d = 0; d < 3; ++d) m[d] = speed_max(m[d], red[w][d]);
     *partial = m[0] + m[1] + m[2];
   }
 }

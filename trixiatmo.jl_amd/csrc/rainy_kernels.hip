@@ -14,6 +14,7 @@
 
 #include "euler3d_physics.hpp"
 #include "rainy_kernels.hpp"
+#include "speed_max.hpp"
 
 namespace atmo {
 
@@ -780,16 +781,16 @@ __global__ __launch_bounds__(64) void rainy_max_speed(RainyLaunch L, double* par
     const RNode a = rnode(L.u + ((size_t)e * N2 + q) * NV);
     const double lam1 = fabs(a.v1) + a.c;
     const double lam2 = fabs(a.v2) + a.c + fabs(a.vr_m);
-    l1 = fmax(l1, fabs(L.Ja11 * lam1 + 0.0 * lam2) * L.inv_J);
-    l2 = fmax(l2, fabs(0.0 * lam1 + L.Ja22 * lam2) * L.inv_J);
+    l1 = speed_max(l1, fabs(L.Ja11 * lam1 + 0.0 * lam2) * L.inv_J);
+    l2 = speed_max(l2, fabs(0.0 * lam1 + L.Ja22 * lam2) * L.inv_J);
   }
   s1[t] = l1;
   s2[t] = l2;
   __syncthreads();
   for (int w = 32; w > 0; w >>= 1) {
     if (t < w) {
-      s1[t] = fmax(s1[t], s1[t + w]);
-      s2[t] = fmax(s2[t], s2[t + w]);
+      s1[t] = speed_max(s1[t], s1[t + w]);
+      s2[t] = speed_max(s2[t], s2[t + w]);
     }
     __syncthreads();
   }
@@ -800,11 +801,11 @@ __global__ __launch_bounds__(256) void rainy_max_reduce(const double* partial, i
   __shared__ double s[256];
   const int t = threadIdx.x;
   double acc = 5e-324;
-  for (int i = t; i < n; i += 256) acc = fmax(acc, partial[i]);
+  for (int i = t; i < n; i += 256) acc = speed_max(acc, partial[i]);
   s[t] = acc;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if (t < w) s[t] = fmax(s[t], s[t + w]);
+    if (t < w) s[t] = speed_max(s[t], s[t + w]);
     __syncthreads();
   }
   if (t == 0) out[0] = s[0];

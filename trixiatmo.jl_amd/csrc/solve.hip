@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "handle.hpp"
+#include "speed_max.hpp"
 
 // clock layout (M members): t[M] (start of the step in flight), dt[M], steps
 // taken [M], status (1: non-finite wave speed), t_end, cfl
@@ -54,7 +55,7 @@ __global__ void step_control_kernel(const double* __restrict__ max_speed, int M,
   if (m < M) step_control_kernel_body(max_speed[m], m, M, n, clock, dt);
 }
 
-// the same for one member whose maximum speed is the fmax over per-element
+// the same for one member whose maximum speed is the maximum over per-element
 // partials (written by the last stage of the step just taken)
 __global__ __launch_bounds__(1024) void step_control_reduce_kernel(const double* __restrict__ partial,
                                                                    int n_items, int n,
@@ -62,20 +63,20 @@ __global__ __launch_bounds__(1024) void step_control_reduce_kernel(const double*
   __shared__ double s[1024 / 64];
   const int t = threadIdx.x;
   double m = 0.0;
-  for (int i = t; i < n_items; i += 1024) m = fmax(m, partial[i]);
+  for (int i = t; i < n_items; i += 1024) m = speed_max(m, partial[i]);
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  for (int o = 32; o > 0; o >>= 1) m = speed_max(m, __shfl_xor(m, o, 64));
   if ((t & 63) == 0) s[t >> 6] = m;
   __syncthreads();
   if (t == 0) {
-    for (int w = 1; w < 1024 / 64; ++w) m = fmax(m, s[w]);
+    for (int w = 1; w < 1024 / 64; ++w) m = speed_max(m, s[w]);
     s[0] = m;
   }
   __syncthreads();
   if (t == 0) step_control_kernel_body(s[0], 0, 1, n, clock, dt);
 }
 
-// one workgroup per ensemble member: its maximum speed is the fmax over the
+// one workgroup per ensemble member: its maximum speed is the maximum over the
 // per-element partials of its elements [member_first[m], member_first[m + 1])
 __global__ __launch_bounds__(256) void step_control_members_reduce_kernel(
     const double* __restrict__ partial, const int* __restrict__ member_first, int M, int n,
@@ -83,13 +84,13 @@ __global__ __launch_bounds__(256) void step_control_members_reduce_kernel(
   __shared__ double s[256 / 64];
   const int m = blockIdx.x, t = threadIdx.x;
   double mx = 0.0;
-  for (int e = member_first[m] + t; e < member_first[m + 1]; e += 256) mx = fmax(mx, partial[e]);
+  for (int e = member_first[m] + t; e < member_first[m + 1]; e += 256) mx = speed_max(mx, partial[e]);
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  for (int o = 32; o > 0; o >>= 1) mx = speed_max(mx, __shfl_xor(mx, o, 64));
   if ((t & 63) == 0) s[t >> 6] = mx;
   __syncthreads();
   if (t == 0) {
-    for (int w = 1; w < 256 / 64; ++w) mx = fmax(mx, s[w]);
+    for (int w = 1; w < 256 / 64; ++w) mx = speed_max(mx, s[w]);
     step_control_kernel_body(mx, m, M, n, clock, dt);
   }
 }

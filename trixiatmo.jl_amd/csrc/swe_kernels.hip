@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include "swe_kernels.hpp"
+#include "speed_max.hpp"
 
 namespace atmo {
 
@@ -429,14 +430,14 @@ __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
       }
     }
     if constexpr (SPEED) {
-      // per element: fmax over its nodes of each speed (exact in any order),
+      // per element: the maximum over its nodes of each speed (exact in any order),
       // then the sum over the directions, by the element's first lane
       __syncthreads();
       if (active && q == 0) {
         double m1 = s_n[es][0][0], m2 = s_n[es][1][0];
         for (int r = 1; r < N2; ++r) {
-          m1 = fmax(m1, s_n[es][0][r]);
-          m2 = fmax(m2, s_n[es][1][r]);
+          m1 = speed_max(m1, s_n[es][0][r]);
+          m2 = speed_max(m2, s_n[es][1][r]);
         }
         L.speed_partial[e] = m1 + m2;
       }
@@ -457,16 +458,16 @@ __global__ __launch_bounds__(64) void swe_max_speed(SweLaunch L, double* partial
     const AuxV a{L.aux + g * NA, 1};
     double l[2];
     swe_node_speeds(u, a.Gcon(0, 0), a.Gcon(1, 1), L.gravity, l);
-    l1 = fmax(l1, l[0]);
-    l2 = fmax(l2, l[1]);
+    l1 = speed_max(l1, l[0]);
+    l2 = speed_max(l2, l[1]);
   }
   s1[t] = l1;
   s2[t] = l2;
   __syncthreads();
   for (int w = 32; w > 0; w >>= 1) {
     if (t < w) {
-      s1[t] = fmax(s1[t], s1[t + w]);
-      s2[t] = fmax(s2[t], s2[t + w]);
+      s1[t] = speed_max(s1[t], s1[t + w]);
+      s2[t] = speed_max(s2[t], s2[t + w]);
     }
     __syncthreads();
   }
@@ -479,11 +480,11 @@ __global__ void swe_member_max(const double* partial, const int* member_first, i
   const int m = blockIdx.x, t = threadIdx.x;
   __shared__ double s[256];
   double acc = 5e-324;
-  for (int e = member_first[m] + t; e < member_first[m + 1]; e += 256) acc = fmax(acc, partial[e]);
+  for (int e = member_first[m] + t; e < member_first[m + 1]; e += 256) acc = speed_max(acc, partial[e]);
   s[t] = acc;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if (t < w) s[t] = fmax(s[t], s[t + w]);
+    if (t < w) s[t] = speed_max(s[t], s[t + w]);
     __syncthreads();
   }
   if (t == 0) out[m] = s[0];
