@@ -63,7 +63,14 @@ __global__ __launch_bounds__(1024) void step_control_reduce_kernel(const double*
   __shared__ double s[1024 / 64];
   const int t = threadIdx.x;
   double m = 0.0;
-  for (int i = t; i < n_items; i += 1024) m = speed_max(m, partial[i]);
+  // eight independent loads in flight per lane, then their maximum
+  for (int i0 = t; i0 < n_items; i0 += 8 * 1024) {
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = i0 + k * 1024 < n_items ? partial[i0 + k * 1024] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) m = speed_max(m, v[k]);
+  }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = speed_max(m, __shfl_xor(m, o, 64));
   if ((t & 63) == 0) s[t >> 6] = m;
@@ -84,7 +91,14 @@ __global__ __launch_bounds__(256) void step_control_members_reduce_kernel(
   __shared__ double s[256 / 64];
   const int m = blockIdx.x, t = threadIdx.x;
   double mx = 0.0;
-  for (int e = member_first[m] + t; e < member_first[m + 1]; e += 256) mx = speed_max(mx, partial[e]);
+  const int e1 = member_first[m + 1];
+  for (int e0 = member_first[m] + t; e0 < e1; e0 += 8 * 256) {
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = e0 + k * 256 < e1 ? partial[e0 + k * 256] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) mx = speed_max(mx, v[k]);
+  }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mx = speed_max(mx, __shfl_xor(mx, o, 64));
   if ((t & 63) == 0) s[t >> 6] = mx;
