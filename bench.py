@@ -161,8 +161,9 @@ def _time_loops(semi, u0, cfl, steps, check_every=256):
 
 def bench_time_loop(steps, trees, layers, polydeg, cov_trees=32, n_members=32):
     """time to solution of the StepsizeCallback loop, host-driven against
-    resident, on the 3D shell, the covariant single member and (resident only,
-    members on their own step sizes) the covariant ensemble"""
+    resident, on the 3D shell, the covariant single member, (resident only,
+    members on their own step sizes) the covariant ensemble and the rainy
+    bubble (a quarter of the steps)"""
     import trixiatmo_hip as T
     W = T.workloads
     out = {}
@@ -188,6 +189,10 @@ def bench_time_loop(steps, trees, layers, polydeg, cov_trees=32, n_members=32):
     out["covariant_ensemble_resident"] = {
         "steps_per_member_min_max": [int(n.min()), int(n.max())], "seconds": el,
         "ms_per_step": el / int(n.max()) * 1e3}
+    del ens
+    mesh, eq, solver, src, lim, u0, _ = W.rainy_bubble(seed=0)
+    rainy = T.SemidiscretizationRainy(mesh, eq, solver, source_terms=src, stage_limiter=lim)
+    out["rainy_bubble"] = _time_loops(rainy, u0, 0.5, max(2, steps // 4))
     for k, v in out.items():
         log(f"[time loop {k}] {v}")
     return out

@@ -91,16 +91,25 @@ const double kC[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 
 
 }  // namespace
 
-int rainy_enqueue_resident_step(atmo_handle* h) {
+// per-element speeds of the state a resident loop starts from; every later
+// step's come from its own last stage
+int rainy_resident_prologue(atmo_handle* h) {
   RainyLaunch L = h->rainy_base;
   L.u = h->d_u;
-  TimerPair tp;
-  if (h->timing) timer_begin(h, 1, &tp);
-  const bool ok = rainy_launch_max_speed(L, h->d_partial, h->d_result, h->stream);
-  if (h->timing) timer_end(h, &tp);
-  if (!ok) return fail(&h->err, ATMO_ERR_INVALID, "no rainy kernel for this polydeg");
-  launch_step_control(h->d_result, 1, h->n, h->d_clock, h->d_clock + 1, h->stream);
+  if (!rainy_launch_max_speed(L, h->d_partial, h->d_result, h->stream))
+    return fail(&h->err, ATMO_ERR_INVALID, "no rainy kernel for this polydeg");
   HIP_TRY(h, hipGetLastError());
+  return ATMO_OK;
+}
+
+// one step: the step size from the per-element speeds in d_partial, then the
+// five stages (each with its stage limiter), the last of which writes the
+// limited new state's speeds there
+int rainy_enqueue_resident_step(atmo_handle* h) {
+  launch_step_control_reduce(h->d_partial, h->rainy_base.n_elements, h->n, h->d_clock,
+                             h->d_clock + 1, h->stream);
+  HIP_TRY(h, hipGetLastError());
+  RainyLaunch L = h->rainy_base;
   L.mode = 1;
   L.k = h->d_k;
   L.clock = h->d_clock;
@@ -110,6 +119,7 @@ int rainy_enqueue_resident_step(atmo_handle* h) {
     L.rk_a = kA[st];
     L.rk_b = kB[st];
     L.c_stage = kC[st];
+    L.speed_partial = st == 4 ? h->d_partial : nullptr;
     const int rc = launch_stage(h, L);
     if (rc != ATMO_OK) return rc;
     std::swap(h->d_u, h->d_u2);

@@ -116,6 +116,7 @@ void launch_step_control_reduce(const double* partial, int n_items, int n, doubl
 // speeds where the stages provide the later ones)
 int e3_resident_prologue(atmo_handle* h);
 int swe_resident_prologue(atmo_handle* h);
+int rainy_resident_prologue(atmo_handle* h);
 // the same per ensemble member over its element range
 void launch_step_control_members_reduce(const double* partial, const int* member_first,
                                         int n_members, int n, double* clock, double* dt,

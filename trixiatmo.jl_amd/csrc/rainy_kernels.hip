@@ -507,7 +507,23 @@ __device__ __forceinline__ double stage_time(double t0, double c, double dt) {
   return t0 + c * dt;
 }
 
-template <int N, bool SEEDED>
+// StepsizeCallback's scaled wave speeds at one node of the uniform mesh:
+// |Ja^i . lambda| / J with lambda_1 = |v_1| + c, lambda_2 = |v_2| + c + |v_r|
+// (the rain's terminal velocity); the max-speed reduction and the last stage
+// of a resident step both call this one function, so they agree bit for bit
+__device__ __forceinline__ void rainy_node_speeds(const RainyLaunch& L, const double* u,
+                                                  double l[2]) {
+  const RNode a = rnode(u);
+  const double lam1 = fabs(a.v1) + a.c;
+  const double lam2 = fabs(a.v2) + a.c + fabs(a.vr_m);
+  l[0] = fabs(L.Ja11 * lam1 + 0.0 * lam2) * L.inv_J;
+  l[1] = fabs(0.0 * lam1 + L.Ja22 * lam2) * L.inv_J;
+}
+
+// SPEED: the RK stage also writes L.speed_partial (the last stage of a
+// resident step); a variant of its own, so the other stages keep their
+// register budget
+template <int N, bool SEEDED, bool SPEED>
 __global__ __launch_bounds__(RShape<N>::BLOCK)
 void rainy_stage(RainyLaunch L) {
   using S = RShape<N>;
@@ -764,6 +780,27 @@ void rainy_stage(RainyLaunch L) {
       }
 #pragma unroll
       for (int v = 0; v < NV; ++v) L.out[g + v] = un[v];
+      if constexpr (SPEED) {
+        // the node's speeds go to its first two state fields in LDS, which
+        // only this lane reads after the face phase and which it has read
+        double l[2];
+        rainy_node_speeds(L, un, l);
+        s_u[es][0][q] = l[0];
+        s_u[es][1][q] = l[1];
+      }
+    }
+  }
+  if constexpr (SPEED) {
+    // per element: the maximum over its nodes of each speed (exact in any
+    // order, NaN-propagating), then the sum over the directions
+    __syncthreads();
+    if (active && q == 0) {
+      double m1 = s_u[es][0][0], m2 = s_u[es][1][0];
+      for (int r = 1; r < N2; ++r) {
+        m1 = speed_max(m1, s_u[es][0][r]);
+        m2 = speed_max(m2, s_u[es][1][r]);
+      }
+      L.speed_partial[e] = m1 + m2;
     }
   }
   // every lane of the (single-wave) workgroup takes part in the reduction
@@ -778,11 +815,10 @@ __global__ __launch_bounds__(64) void rainy_max_speed(RainyLaunch L, double* par
   __shared__ double s1[64], s2[64];
   double l1 = 0.0, l2 = 0.0;
   for (int q = t; q < N2; q += 64) {
-    const RNode a = rnode(L.u + ((size_t)e * N2 + q) * NV);
-    const double lam1 = fabs(a.v1) + a.c;
-    const double lam2 = fabs(a.v2) + a.c + fabs(a.vr_m);
-    l1 = speed_max(l1, fabs(L.Ja11 * lam1 + 0.0 * lam2) * L.inv_J);
-    l2 = speed_max(l2, fabs(0.0 * lam1 + L.Ja22 * lam2) * L.inv_J);
+    double l[2];
+    rainy_node_speeds(L, L.u + ((size_t)e * N2 + q) * NV, l);
+    l1 = speed_max(l1, l[0]);
+    l2 = speed_max(l2, l[1]);
   }
   s1[t] = l1;
   s2[t] = l2;
@@ -937,10 +973,16 @@ bool rainy_launch_stage(const RainyLaunch& L, hipStream_t s) {
 #define ATMO_CASE(NN)                                                                         \
   case NN: {                                                                                  \
     const int grid = (L.n_elements + RShape<NN>::EPB - 1) / RShape<NN>::EPB;                  \
-    if (L.mlp.params && L.mode == 1 && L.newton_max_iter > 0)                                 \
-      hipLaunchKernelGGL((rainy_stage<NN, true>), dim3(grid), dim3(RShape<NN>::BLOCK), 0, s, L); \
+    const bool seeded = L.mlp.params && L.mode == 1 && L.newton_max_iter > 0;                 \
+    const bool speed = L.speed_partial && L.mode == 1;                                          \
+    if (seeded && speed)                                                                        \
+      hipLaunchKernelGGL((rainy_stage<NN, true, true>), dim3(grid), dim3(RShape<NN>::BLOCK), 0, s, L); \
+    else if (seeded)                                                                            \
+      hipLaunchKernelGGL((rainy_stage<NN, true, false>), dim3(grid), dim3(RShape<NN>::BLOCK), 0, s, L); \
+    else if (speed)                                                                             \
+      hipLaunchKernelGGL((rainy_stage<NN, false, true>), dim3(grid), dim3(RShape<NN>::BLOCK), 0, s, L); \
     else                                                                                        \
-      hipLaunchKernelGGL((rainy_stage<NN, false>), dim3(grid), dim3(RShape<NN>::BLOCK), 0, s, L); \
+      hipLaunchKernelGGL((rainy_stage<NN, false, false>), dim3(grid), dim3(RShape<NN>::BLOCK), 0, s, L); \
     return true;                                                                              \
   }
   ATMO_RAINY_SWITCH(ATMO_CASE)

@@ -106,6 +106,9 @@ struct RainyLaunch {
   // and a step of size 0 copies the state unchanged
   const double* clock;
   double c_stage;
+  // RK stage, if set: per element the StepsizeCallback sum over directions of
+  // the maximum scaled speed of the new (limited) state
+  double* speed_partial;
   double newton_tol;
   int newton_max_iter;
   unsigned long long* stats;  // optional: [sum of updates, failures, max updates, fallbacks]

@@ -209,7 +209,7 @@ int atmo_solve(atmo_handle* h, double t_end, double cfl, int32_t check_every, in
   if (!done) {
     const int rc = h->eq == ATMO_EQ_EULER_ENERGY_GRAVITY_3D ? atmo::e3_resident_prologue(h)
                    : swe                                  ? atmo::swe_resident_prologue(h)
-                                                          : ATMO_OK;
+                                                          : atmo::rainy_resident_prologue(h);
     if (rc != ATMO_OK) return rc;
   }
   hipGraphExec_t graph = nullptr;
