@@ -30,11 +30,17 @@ METRIC = "GDOF\u00b7RHS/s, mixed vs FP64 on 1 GPU at stated rel. err; % of VALU 
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch); FETCH_SIZE
 # reads half the bytes of a plain device copy of known size on this GPU, hence
 # the factor 2 (profiles/e3_stage_shell16x8_p4_pmc.txt).
-STAGE_FETCH_KIB, STAGE_WRITE_KIB, FETCH_FACTOR = 2.102e5, 1.321e5, 2.0
+STAGE_FETCH_KIB, STAGE_WRITE_KIB, FETCH_FACTOR = 2.103e5, 1.321e5, 2.0
 # Executed VALU flops per node and stage (SQ_INSTS_VALU_{ADD,MUL,TRANS}_F64 x 64
 # + SQ_INSTS_VALU_FMA_F64 x 128 per dispatch / nodes, same profile file); the
 # mixed form executes mostly FP32.
 EXEC_FP64_FLOPS_PER_NODE = 1790.0
+# Algorithmic FP64 work of one rhs! per node of the shell 6x16^2x8, polydeg 4
+# (KG + Waruszewski volume, KG + LLF + Waruszewski surface, slip walls): the
+# oracle's own code run with an operation-counting scalar (oracle/count_flops.py;
+# + - * / sqrt log exp pow one each), every node pair of a line evaluated once
+# as the reference does
+ALG_FP64_FLOPS_PER_NODE = 1680.4
 EXEC_MIXED_FLOPS_PER_NODE = {"fp32": 1870.0, "fp64": 315.0}
 
 
@@ -529,6 +535,10 @@ def main():
                      "bytes_per_node": bytes_per_node, "traffic_bytes_per_node": traffic / nodes},
         "valu": {"fp64_executed_tflops": valu64,
                  "frac_of_fp64_vector_peak": valu64 / FP64_VECTOR_PEAK_TFLOPS,
+                 "algorithmic_flops_per_node": ALG_FP64_FLOPS_PER_NODE,
+                 "fp64_algorithmic_tflops": ALG_FP64_FLOPS_PER_NODE * nodes / avg_s / 1e12,
+                 "algorithmic_frac_of_fp64_vector_peak":
+                     ALG_FP64_FLOPS_PER_NODE * nodes / avg_s / 1e12 / FP64_VECTOR_PEAK_TFLOPS,
                  "mixed_frac_of_vector_peak": valu_mix,
                  "note": "executed VALU flops from rocprofv3 SQ_INSTS_VALU_* counters"},
         "cpu_baseline": cpu,

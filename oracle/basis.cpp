@@ -9,6 +9,7 @@
 // negative-sum diagonal).  Pinned in tests by closed-form nodes/weights for
 // n = 4, 5, quadrature exactness, polynomial exactness of D and the SBP
 // property M D + D^T M = diag(-1, 0, ..., 0, 1).
+#include "counted.h"
 #include "oracle.h"
 
 #include <cmath>
@@ -103,22 +104,8 @@ extern "C" void ora_derivative_split(int n, const double* x, const double* w,
   Ds[n * n - 1] -= 1.0 / w[n - 1];
 }
 
-// Logarithmic mean with the f^2 < 1e-4 series branch, Ranocha et al. (2023),
-// "Efficient implementation of modern entropy stable and kinetic energy
-// preserving discontinuous Galerkin methods for conservation laws", ACM TOMS,
-// Section 4 (the form Trixi's ln_mean implements; imported src/TrixiAtmo.jl:37).
-extern "C" double ora_ln_mean(double x, double y) {
-  const double f2 = (x * (x - 2 * y) + y * y) / (x * (x + 2 * y) + y * y);
-  if (f2 < 1.0e-4) {
-    return (x + y) / (2.0 + f2 * (2.0 / 3.0 + f2 * (2.0 / 5.0 + f2 * (2.0 / 7.0))));
-  }
-  return (y - x) / std::log(y / x);
-}
+// Logarithmic mean and its inverse: the templates of counted.h (Ranocha et
+// al. 2023, the form Trixi's ln_mean implements) in FP64
+extern "C" double ora_ln_mean(double x, double y) { return ora_ln_mean_t(x, y); }
 
-extern "C" double ora_inv_ln_mean(double x, double y) {
-  const double f2 = (x * (x - 2 * y) + y * y) / (x * (x + 2 * y) + y * y);
-  if (f2 < 1.0e-4) {
-    return (2.0 + f2 * (2.0 / 3.0 + f2 * (2.0 / 5.0 + f2 * (2.0 / 7.0)))) / (x + y);
-  }
-  return std::log(y / x) / (y - x);
-}
+extern "C" double ora_inv_ln_mean(double x, double y) { return ora_inv_ln_mean_t(x, y); }

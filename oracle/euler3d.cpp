@@ -14,9 +14,23 @@
 #include <cstring>
 #include <vector>
 
+#include <algorithm>
+
+#include "counted.h"
 #include "oracle.h"
 
+OraCounts& ora_counts() {
+  static OraCounts c;
+  return c;
+}
+
 namespace {
+
+// OpenMP for the FP64 oracle; counting runs are serial (plain counters)
+template <typename R>
+constexpr bool kParallel = true;
+template <>
+constexpr bool kParallel<Counted> = false;
 
 constexpr int NV = 6;
 
@@ -30,8 +44,9 @@ struct Eq {
 };
 
 // compressible_euler_energy_with_gravity_3d.jl:568-578
-inline void cons2prim(const Eq& eq, const double* u, double& rho, double& v1,
-                      double& v2, double& v3, double& p) {
+template <typename R>
+inline void cons2prim(const Eq& eq, const R* u, R& rho, R& v1,
+                      R& v2, R& v3, R& p) {
   rho = u[0];
   v1 = u[1] / rho;
   v2 = u[2] / rho;
@@ -42,12 +57,13 @@ inline void cons2prim(const Eq& eq, const double* u, double& rho, double& v1,
 // compressible_euler_energy_with_gravity_3d.jl:77-91.  The reference reads
 // `rho_e = last(u)`, i.e. the sixth state entry (the geopotential phi), for the
 // energy flux; that behaviour is kept as is.
-void flux(const Eq& eq, const double* u, const double* n, double* f) {
-  double rho, v1, v2, v3, p;
+template <typename R>
+void flux(const Eq& eq, const R* u, const R* n, R* f) {
+  R rho, v1, v2, v3, p;
   cons2prim(eq, u, rho, v1, v2, v3, p);
-  const double rho_e = u[5];
-  const double vn = v1 * n[0] + v2 * n[1] + v3 * n[2];
-  const double rvn = rho * vn;
+  const R rho_e = u[5];
+  const R vn = v1 * n[0] + v2 * n[1] + v3 * n[2];
+  const R rvn = rho * vn;
   f[0] = rvn;
   f[1] = rvn * v1 + p * n[0];
   f[2] = rvn * v2 + p * n[1];
@@ -57,20 +73,21 @@ void flux(const Eq& eq, const double* u, const double* n, double* f) {
 }
 
 // :382-415
-void flux_ranocha(const Eq& eq, const double* ul, const double* ur,
-                  const double* n, double* f) {
-  double rl, v1l, v2l, v3l, pl, rr, v1r, v2r, v3r, pr;
+template <typename R>
+void flux_ranocha(const Eq& eq, const R* ul, const R* ur,
+                  const R* n, R* f) {
+  R rl, v1l, v2l, v3l, pl, rr, v1r, v2r, v3r, pr;
   cons2prim(eq, ul, rl, v1l, v2l, v3l, pl);
   cons2prim(eq, ur, rr, v1r, v2r, v3r, pr);
-  const double vnl = v1l * n[0] + v2l * n[1] + v3l * n[2];
-  const double vnr = v1r * n[0] + v2r * n[1] + v3r * n[2];
-  const double rho_mean = ora_ln_mean(rl, rr);
-  const double inv_rho_p_mean = pl * pr * ora_inv_ln_mean(rl * pr, rr * pl);
-  const double v1a = 0.5 * (v1l + v1r), v2a = 0.5 * (v2l + v2r),
+  const R vnl = v1l * n[0] + v2l * n[1] + v3l * n[2];
+  const R vnr = v1r * n[0] + v2r * n[1] + v3r * n[2];
+  const R rho_mean = ora_ln_mean_t(rl, rr);
+  const R inv_rho_p_mean = pl * pr * ora_inv_ln_mean_t(rl * pr, rr * pl);
+  const R v1a = 0.5 * (v1l + v1r), v2a = 0.5 * (v2l + v2r),
                v3a = 0.5 * (v3l + v3r);
-  const double pa = 0.5 * (pl + pr);
-  const double vsq = 0.5 * (v1l * v1r + v2l * v2r + v3l * v3r);
-  const double f1 = rho_mean * 0.5 * (vnl + vnr);
+  const R pa = 0.5 * (pl + pr);
+  const R vsq = 0.5 * (v1l * v1r + v2l * v2r + v3l * v3r);
+  const R f1 = rho_mean * 0.5 * (vnl + vnr);
   f[0] = f1;
   f[1] = f1 * v1a + pa * n[0];
   f[2] = f1 * v2a + pa * n[1];
@@ -80,18 +97,19 @@ void flux_ranocha(const Eq& eq, const double* ul, const double* ur,
 }
 
 // :289-318
-void flux_kennedy_gruber(const Eq& eq, const double* ul, const double* ur,
-                         const double* n, double* f) {
-  double rl, v1l, v2l, v3l, pl, rr, v1r, v2r, v3r, pr;
+template <typename R>
+void flux_kennedy_gruber(const Eq& eq, const R* ul, const R* ur,
+                         const R* n, R* f) {
+  R rl, v1l, v2l, v3l, pl, rr, v1r, v2r, v3r, pr;
   cons2prim(eq, ul, rl, v1l, v2l, v3l, pl);
   cons2prim(eq, ur, rr, v1r, v2r, v3r, pr);
-  const double ra = 0.5 * (rl + rr);
-  const double v1a = 0.5 * (v1l + v1r), v2a = 0.5 * (v2l + v2r),
+  const R ra = 0.5 * (rl + rr);
+  const R v1a = 0.5 * (v1l + v1r), v2a = 0.5 * (v2l + v2r),
                v3a = 0.5 * (v3l + v3r);
-  const double pa = 0.5 * (pl + pr);
-  const double ea = 0.5 * (ul[4] / rl + ur[4] / rr);
-  const double vna = v1a * n[0] + v2a * n[1] + v3a * n[2];
-  const double f1 = ra * vna;
+  const R pa = 0.5 * (pl + pr);
+  const R ea = 0.5 * (ul[4] / rl + ur[4] / rr);
+  const R vna = v1a * n[0] + v2a * n[1] + v3a * n[2];
+  const R f1 = ra * vna;
   f[0] = f1;
   f[1] = f1 * v1a + pa * n[0];
   f[2] = f1 * v2a + pa * n[1];
@@ -101,20 +119,21 @@ void flux_kennedy_gruber(const Eq& eq, const double* ul, const double* ur,
 }
 
 // :248-276
-void flux_shima(const Eq& eq, const double* ul, const double* ur, const double* n,
-                double* f) {
-  double rl, v1l, v2l, v3l, pl, rr, v1r, v2r, v3r, pr;
+template <typename R>
+void flux_shima(const Eq& eq, const R* ul, const R* ur, const R* n,
+                R* f) {
+  R rl, v1l, v2l, v3l, pl, rr, v1r, v2r, v3r, pr;
   cons2prim(eq, ul, rl, v1l, v2l, v3l, pl);
   cons2prim(eq, ur, rr, v1r, v2r, v3r, pr);
-  const double vnl = v1l * n[0] + v2l * n[1] + v3l * n[2];
-  const double vnr = v1r * n[0] + v2r * n[1] + v3r * n[2];
-  const double ra = 0.5 * (rl + rr);
-  const double v1a = 0.5 * (v1l + v1r), v2a = 0.5 * (v2l + v2r),
+  const R vnl = v1l * n[0] + v2l * n[1] + v3l * n[2];
+  const R vnr = v1r * n[0] + v2r * n[1] + v3r * n[2];
+  const R ra = 0.5 * (rl + rr);
+  const R v1a = 0.5 * (v1l + v1r), v2a = 0.5 * (v2l + v2r),
                v3a = 0.5 * (v3l + v3r);
-  const double vna = 0.5 * (vnl + vnr);
-  const double pa = 0.5 * (pl + pr);
-  const double vsq = 0.5 * (v1l * v1r + v2l * v2r + v3l * v3r);
-  const double f1 = ra * vna;
+  const R vna = 0.5 * (vnl + vnr);
+  const R pa = 0.5 * (pl + pr);
+  const R vsq = 0.5 * (v1l * v1r + v2l * v2r + v3l * v3r);
+  const R f1 = ra * vna;
   f[0] = f1;
   f[1] = f1 * v1a + pa * n[0];
   f[2] = f1 * v2a + pa * n[1];
@@ -125,52 +144,56 @@ void flux_shima(const Eq& eq, const double* ul, const double* ur, const double* 
 
 // Trixi flux_central = average of the pointwise fluxes (equations.jl:273-279
 // restates the same for the covariant form).
-void flux_central(const Eq& eq, const double* ul, const double* ur,
-                  const double* n, double* f) {
-  double fl[NV], fr[NV];
+template <typename R>
+void flux_central(const Eq& eq, const R* ul, const R* ur,
+                  const R* n, R* f) {
+  R fl[NV], fr[NV];
   flux(eq, ul, n, fl);
   flux(eq, ur, n, fr);
   for (int v = 0; v < NV; ++v) f[v] = 0.5 * (fl[v] + fr[v]);
 }
 
 // :538-557 (max_abs_speed with normal direction, the non-naive variant)
-double max_abs_speed(const Eq& eq, const double* ul, const double* ur,
-                     const double* n) {
-  double rl, v1l, v2l, v3l, pl, rr, v1r, v2r, v3r, pr;
+template <typename R>
+R max_abs_speed(const Eq& eq, const R* ul, const R* ur,
+                     const R* n) {
+  R rl, v1l, v2l, v3l, pl, rr, v1r, v2r, v3r, pr;
   cons2prim(eq, ul, rl, v1l, v2l, v3l, pl);
   cons2prim(eq, ur, rr, v1r, v2r, v3r, pr);
-  const double vl = v1l * n[0] + v2l * n[1] + v3l * n[2];
-  const double vr = v1r * n[0] + v2r * n[1] + v3r * n[2];
-  const double cl = std::sqrt(eq.gamma * pl / rl);
-  const double cr = std::sqrt(eq.gamma * pr / rr);
-  const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-  return std::fmax(std::fabs(vl) + cl * nn, std::fabs(vr) + cr * nn);
+  const R vl = v1l * n[0] + v2l * n[1] + v3l * n[2];
+  const R vr = v1r * n[0] + v2r * n[1] + v3r * n[2];
+  const R cl = sqrt(eq.gamma * pl / rl);
+  const R cr = sqrt(eq.gamma * pr / rr);
+  const R nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  return fmax(fabs(vl) + cl * nn, fabs(vr) + cr * nn);
 }
 
 // FluxLaxFriedrichs = FluxPlusDissipation(flux_central,
 // DissipationLocalLaxFriedrichs(max_abs_speed)); equations.jl:256-290 restates
 // the same construction for the covariant equations.
-void flux_llf(const Eq& eq, const double* ul, const double* ur, const double* n,
-              double* f) {
+template <typename R>
+void flux_llf(const Eq& eq, const R* ul, const R* ur, const R* n,
+              R* f) {
   flux_central(eq, ul, ur, n, f);
-  const double lam = max_abs_speed(eq, ul, ur, n);
+  const R lam = max_abs_speed(eq, ul, ur, n);
   for (int v = 0; v < NV; ++v) f[v] += -0.5 * lam * (ur[v] - ul[v]);
 }
 
 // :430-466 (FluxLMARS)
-void flux_lmars(const Eq& eq, double c, const double* ul, const double* ur,
-                const double* n, double* f) {
-  double rl, v1l, v2l, v3l, pl, rr, v1r, v2r, v3r, pr;
+template <typename R>
+void flux_lmars(const Eq& eq, R c, const R* ul, const R* ur,
+                const R* n, R* f) {
+  R rl, v1l, v2l, v3l, pl, rr, v1r, v2r, v3r, pr;
   cons2prim(eq, ul, rl, v1l, v2l, v3l, pl);
   cons2prim(eq, ur, rr, v1r, v2r, v3r, pr);
-  const double vl = v1l * n[0] + v2l * n[1] + v3l * n[2];
-  const double vr = v1r * n[0] + v2r * n[1] + v3r * n[2];
-  const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-  const double rho = 0.5 * (rl + rr);
-  const double p = 0.5 * (pl + pr) - 0.5 * c * rho * (vr - vl) / nn;
-  const double v = 0.5 * (vl + vr) - 1.0 / (2.0 * c * rho) * (pr - pl) * nn;
-  const double* us = v >= 0 ? ul : ur;
-  const double ps = v >= 0 ? pl : pr;
+  const R vl = v1l * n[0] + v2l * n[1] + v3l * n[2];
+  const R vr = v1r * n[0] + v2r * n[1] + v3r * n[2];
+  const R nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  const R rho = 0.5 * (rl + rr);
+  const R p = 0.5 * (pl + pr) - 0.5 * c * rho * (vr - vl) / nn;
+  const R v = 0.5 * (vl + vr) - 1.0 / (2.0 * c * rho) * (pr - pl) * nn;
+  const R* us = v >= 0 ? ul : ur;
+  const R ps = v >= 0 ? pl : pr;
   f[0] = v * us[0];
   f[1] = v * us[1] + p * n[0];
   f[2] = v * us[2] + p * n[1];
@@ -180,25 +203,27 @@ void flux_lmars(const Eq& eq, double c, const double* ul, const double* ur,
 }
 
 // :135-159 (waruszewski: ln_mean density), :206-230 (souza: arithmetic mean)
-void noncons_gravity(int which, const double* ul, const double* ur,
-                     const double* n, double* f) {
+template <typename R>
+void noncons_gravity(int which, const R* ul, const R* ur,
+                     const R* n, R* f) {
   for (int v = 0; v < NV; ++v) f[v] = 0.0;
   if (which == ORA_NC_ZERO) return;
-  const double v1l = ul[1] / ul[0], v2l = ul[2] / ul[0], v3l = ul[3] / ul[0];
-  const double v1r = ur[1] / ur[0], v2r = ur[2] / ur[0], v3r = ur[3] / ur[0];
-  const double v1a = 0.5 * (v1l + v1r), v2a = 0.5 * (v2l + v2r),
+  const R v1l = ul[1] / ul[0], v2l = ul[2] / ul[0], v3l = ul[3] / ul[0];
+  const R v1r = ur[1] / ur[0], v2r = ur[2] / ur[0], v3r = ur[3] / ur[0];
+  const R v1a = 0.5 * (v1l + v1r), v2a = 0.5 * (v2l + v2r),
                v3a = 0.5 * (v3l + v3r);
-  const double ra = which == ORA_NC_WARUSZEWSKI ? ora_ln_mean(ul[0], ur[0])
+  const R ra = which == ORA_NC_WARUSZEWSKI ? ora_ln_mean_t(ul[0], ur[0])
                                                  : 0.5 * (ul[0] + ur[0]);
-  const double jump = ur[5] - ul[5];
+  const R jump = ur[5] - ul[5];
   f[1] = n[0] * ra * jump;
   f[2] = n[1] * ra * jump;
   f[3] = n[2] * ra * jump;
   f[4] = n[0] * ra * v1a * jump + n[1] * ra * v2a * jump + n[2] * ra * v3a * jump;
 }
 
-void volume_flux(const Eq& eq, int which, const double* ul, const double* ur,
-                 const double* n, double* f) {
+template <typename R>
+void volume_flux(const Eq& eq, int which, const R* ul, const R* ur,
+                 const R* n, R* f) {
   switch (which) {
     case ORA_VOL_RANOCHA: flux_ranocha(eq, ul, ur, n, f); break;
     case ORA_VOL_KENNEDY_GRUBER: flux_kennedy_gruber(eq, ul, ur, n, f); break;
@@ -207,19 +232,20 @@ void volume_flux(const Eq& eq, int which, const double* ul, const double* ur,
   }
 }
 
-void surface_flux(const Eq& eq, const ora_e3_params* p, const double* ul,
-                  const double* ur, const double* n, double* f) {
+template <typename R>
+void surface_flux(const Eq& eq, const ora_e3_params* p, const R* ul,
+                  const R* ur, const R* n, R* f) {
   switch (p->surface_flux) {
     case ORA_SURF_LAX_FRIEDRICHS: flux_llf(eq, ul, ur, n, f); break;
     case ORA_SURF_RANOCHA: flux_ranocha(eq, ul, ur, n, f); break;
     case ORA_SURF_KENNEDY_GRUBER: flux_kennedy_gruber(eq, ul, ur, n, f); break;
-    case ORA_SURF_LMARS: flux_lmars(eq, p->lmars_c, ul, ur, n, f); break;
+    case ORA_SURF_LMARS: flux_lmars(eq, R(p->lmars_c), ul, ur, n, f); break;
     default: {
       // FluxPlusDissipation(f#, DissipationLocalLaxFriedrichs(max_abs_speed)):
       // equations.jl:256-290 builds the same for the covariant equations
       if (p->surface_flux == ORA_SURF_KG_PLUS_LLF) flux_kennedy_gruber(eq, ul, ur, n, f);
       else flux_ranocha(eq, ul, ur, n, f);
-      const double lam = max_abs_speed(eq, ul, ur, n);
+      const R lam = max_abs_speed(eq, ul, ur, n);
       for (int v = 0; v < NV; ++v) f[v] += -0.5 * lam * (ur[v] - ul[v]);
       break;
     }
@@ -258,9 +284,10 @@ inline int indices2direction(const int* idx) {
 
 // Trixi get_normal_direction: Ja^{orientation}, negated on odd (Julia) i.e.
 // negative-side directions.
-inline void normal_direction(int dir0, const double* cv, int node, double* nrm) {
+template <typename R>
+inline void normal_direction(int dir0, const R* cv, int node, R* nrm) {
   const int orientation = dir0 / 2;
-  const double* ja = cv + 9 * node + 3 * orientation;
+  const R* ja = cv + 9 * node + 3 * orientation;
   const double s = (dir0 % 2 == 0) ? -1.0 : 1.0;
   nrm[0] = s * ja[0];
   nrm[1] = s * ja[1];
@@ -330,24 +357,25 @@ double ora_e3_entropy(const ora_e3_params* p, const double* u) {
 // `mag` = true accumulates the absolute value of every contribution instead,
 // giving the magnitude sum_terms |term| that bounds the rounding error of the
 // tendency (used by the tests as the error scale of cancelling sums).
+template <typename R>
 static int e3_rhs_impl(const ora_e3_params* p, const ora_connectivity* c,
-                       const double* cv, const double* invJ, const double* u,
-                       double* du, bool mag) {
-  auto M = [mag](double x) { return mag ? std::fabs(x) : x; };
+                       const R* cv, const R* invJ, const R* u,
+                       R* du, bool mag) {
+  auto M = [mag](R x) { return mag ? fabs(x) : x; };
   const Eq eq(p);
   const int n = p->n, n3 = n * n * n, E = p->n_elements;
   const Basis B(n);
   const double* Ds = B.Ds.data();
-  std::vector<double> sfv((size_t)NV * n * n * 6 * E, 0.0);
+  std::vector<R> sfv((size_t)NV * n * n * 6 * E, 0.0);
 
   // --- volume integral: flux_differencing_kernel! (symmetric part, then the
   // nonconservative part with the factor 1/2), per element.
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (kParallel<R>)
   for (int e = 0; e < E; ++e) {
-    double* due = du + (size_t)NV * n3 * e;
-    std::memset(due, 0, sizeof(double) * NV * n3);
-    const double* ue = u + (size_t)NV * n3 * e;
-    const double* cve = cv + (size_t)9 * n3 * e;
+    R* due = du + (size_t)NV * n3 * e;
+    std::fill(due, due + NV * n3, R(0.0));
+    const R* ue = u + (size_t)NV * n3 * e;
+    const R* cve = cv + (size_t)9 * n3 * e;
     auto nid = [n](int i, int j, int k) { return i + n * (j + n * k); };
     for (int k = 0; k < n; ++k)
       for (int j = 0; j < n; ++j)
@@ -357,7 +385,7 @@ static int e3_rhs_impl(const ora_e3_params* p, const ora_connectivity* c,
             const int ia = d == 0 ? i : (d == 1 ? j : k);
             for (int m = ia + 1; m < n; ++m) {
               const int b = d == 0 ? nid(m, j, k) : (d == 1 ? nid(i, m, k) : nid(i, j, m));
-              double Ja[3], f[NV];
+              R Ja[3], f[NV];
               for (int q = 0; q < 3; ++q)
                 Ja[q] = 0.5 * (cve[9 * a + 3 * d + q] + cve[9 * b + 3 * d + q]);
               volume_flux(eq, p->volume_flux, ue + NV * a, ue + NV * b, Ja, f);
@@ -372,12 +400,12 @@ static int e3_rhs_impl(const ora_e3_params* p, const ora_connectivity* c,
       for (int j = 0; j < n; ++j)
         for (int i = 0; i < n; ++i) {
           const int a = nid(i, j, k);
-          double acc[NV] = {0, 0, 0, 0, 0, 0};
+          R acc[NV] = {0, 0, 0, 0, 0, 0};
           for (int d = 0; d < 3; ++d) {
             const int ia = d == 0 ? i : (d == 1 ? j : k);
             for (int m = 0; m < n; ++m) {
               const int b = d == 0 ? nid(m, j, k) : (d == 1 ? nid(i, m, k) : nid(i, j, m));
-              double Ja[3], g[NV];
+              R Ja[3], g[NV];
               for (int q = 0; q < 3; ++q)
                 Ja[q] = 0.5 * (cve[9 * a + 3 * d + q] + cve[9 * b + 3 * d + q]);
               noncons_gravity(p->volume_noncons, ue + NV * a, ue + NV * b, Ja, g);
@@ -390,7 +418,7 @@ static int e3_rhs_impl(const ora_e3_params* p, const ora_connectivity* c,
 
   // --- prolong2interfaces! + calc_interface_flux!: the primary element's
   // outward normal; primary gets f + g(ll,rr)/2, secondary -(f + g(rr,ll)/2).
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (kParallel<R>)
   for (int I = 0; I < c->n_interfaces; ++I) {
     const int ep = c->neighbor_ids[2 * I], es = c->neighbor_ids[2 * I + 1];
     const int* ip = c->node_indices + 6 * I;
@@ -408,29 +436,29 @@ static int e3_rhs_impl(const ora_e3_params* p, const ora_connectivity* c,
         const int si = code_index(is[0], aa, bb, n), sj = code_index(is[1], aa, bb, n),
                   sk = code_index(is[2], aa, bb, n);
         const int pnode = pi + n * (pj + n * pk), snode = si + n * (sj + n * sk);
-        const double* ul = u + (size_t)NV * (pnode + (size_t)n3 * ep);
-        const double* ur = u + (size_t)NV * (snode + (size_t)n3 * es);
-        double nrm[3];
+        const R* ul = u + (size_t)NV * (pnode + (size_t)n3 * ep);
+        const R* ur = u + (size_t)NV * (snode + (size_t)n3 * es);
+        R nrm[3];
         normal_direction(dp, cv, pnode + n3 * ep, nrm);
-        double f[NV], gp[NV], gs[NV];
+        R f[NV], gp[NV], gs[NV];
         surface_flux(eq, p, ul, ur, nrm, f);
         noncons_gravity(p->surface_noncons, ul, ur, nrm, gp);
         noncons_gravity(p->surface_noncons, ur, ul, nrm, gs);
         // secondary surface-node index
         const int sa = code_index(ss[0], aa, bb, n), sb = code_index(ss[1], aa, bb, n);
         (void)sp;
-        double* fp = sfv.data() + (size_t)NV * (aa + n * (bb + n * (dp + 6 * (size_t)ep)));
-        double* fs = sfv.data() + (size_t)NV * (sa + n * (sb + n * (ds + 6 * (size_t)es)));
+        R* fp = sfv.data() + (size_t)NV * (aa + n * (bb + n * (dp + 6 * (size_t)ep)));
+        R* fs = sfv.data() + (size_t)NV * (sa + n * (sb + n * (ds + 6 * (size_t)es)));
         for (int v = 0; v < NV; ++v) {
-          fp[v] = mag ? std::fabs(f[v]) + 0.5 * std::fabs(gp[v]) : f[v] + 0.5 * gp[v];
-          fs[v] = mag ? std::fabs(f[v]) + 0.5 * std::fabs(gs[v]) : -(f[v] + 0.5 * gs[v]);
+          fp[v] = mag ? fabs(f[v]) + 0.5 * fabs(gp[v]) : f[v] + 0.5 * gp[v];
+          fs[v] = mag ? fabs(f[v]) + 0.5 * fabs(gs[v]) : -(f[v] + 0.5 * gs[v]);
         }
       }
   }
 
   // --- prolong2boundaries! + calc_boundary_flux! (slip wall:
   // compressible_euler_energy_with_gravity_3d.jl:93-116)
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (kParallel<R>)
   for (int Bi = 0; Bi < c->n_boundaries; ++Bi) {
     const int e = c->bnd_element[Bi];
     const int* ib = c->bnd_node_indices + 3 * Bi;
@@ -440,30 +468,30 @@ static int e3_rhs_impl(const ora_e3_params* p, const ora_connectivity* c,
         const int ii = code_index(ib[0], aa, bb, n), jj = code_index(ib[1], aa, bb, n),
                   kk = code_index(ib[2], aa, bb, n);
         const int node = ii + n * (jj + n * kk);
-        const double* ui = u + (size_t)NV * (node + (size_t)n3 * e);
-        double nrm[3];
+        const R* ui = u + (size_t)NV * (node + (size_t)n3 * e);
+        R nrm[3];
         normal_direction(dir, cv, node + n3 * e, nrm);
-        const double nn = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
-        const double nh[3] = {nrm[0] / nn, nrm[1] / nn, nrm[2] / nn};
-        const double un = nh[0] * ui[1] + nh[1] * ui[2] + nh[2] * ui[3];
-        const double ub[NV] = {ui[0], ui[1] - 2 * un * nh[0], ui[2] - 2 * un * nh[1],
+        const R nn = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+        const R nh[3] = {nrm[0] / nn, nrm[1] / nn, nrm[2] / nn};
+        const R un = nh[0] * ui[1] + nh[1] * ui[2] + nh[2] * ui[3];
+        const R ub[NV] = {ui[0], ui[1] - 2 * un * nh[0], ui[2] - 2 * un * nh[1],
                                ui[3] - 2 * un * nh[2], ui[4], ui[5]};
-        double f[NV], g[NV];
+        R f[NV], g[NV];
         surface_flux(eq, p, ui, ub, nrm, f);
         noncons_gravity(p->surface_noncons, ui, ub, nrm, g);
-        double* fo = sfv.data() + (size_t)NV * (aa + n * (bb + n * (dir + 6 * (size_t)e)));
+        R* fo = sfv.data() + (size_t)NV * (aa + n * (bb + n * (dir + 6 * (size_t)e)));
         for (int v = 0; v < NV; ++v)
-          fo[v] = mag ? std::fabs(f[v]) + 0.5 * std::fabs(g[v]) : f[v] + 0.5 * g[v];
+          fo[v] = mag ? fabs(f[v]) + 0.5 * fabs(g[v]) : f[v] + 0.5 * g[v];
       }
   }
 
   // --- calc_surface_integral! (outward normals: every face adds f / w_end),
   // then apply_jacobian! (du *= -1/J).
   const double f1 = 1.0 / B.w[0], f2 = 1.0 / B.w[n - 1];
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (kParallel<R>)
   for (int e = 0; e < E; ++e) {
-    double* due = du + (size_t)NV * n3 * e;
-    const double* s = sfv.data() + (size_t)NV * n * n * 6 * e;
+    R* due = du + (size_t)NV * n3 * e;
+    const R* s = sfv.data() + (size_t)NV * n * n * 6 * e;
     auto S = [&](int a, int b, int dir, int v) { return s[NV * (a + n * (b + n * dir)) + v]; };
     for (int m = 0; m < n; ++m)
       for (int l = 0; l < n; ++l)
@@ -476,13 +504,13 @@ static int e3_rhs_impl(const ora_e3_params* p, const ora_connectivity* c,
           due[NV * (l + n * (m + n * (n - 1))) + v] += S(l, m, 5, v) * f2;
         }
     for (int a = 0; a < n3; ++a) {
-      const double fac = mag ? std::fabs(invJ[(size_t)n3 * e + a]) : -invJ[(size_t)n3 * e + a];
+      const R fac = mag ? fabs(invJ[(size_t)n3 * e + a]) : -invJ[(size_t)n3 * e + a];
       for (int v = 0; v < NV; ++v) due[NV * a + v] *= fac;
     }
     // calc_sources!: source_terms_coriolis, -2 Omega e_z x rho v
     // (src/equations/initial_condition_compressible_euler_3d.jl:245-270)
     if (p->coriolis_rate > 0) {
-      const double* ue = u + (size_t)NV * n3 * e;
+      const R* ue = u + (size_t)NV * n3 * e;
       for (int a = 0; a < n3; ++a) {
         due[NV * a + 1] += M(2 * p->coriolis_rate * ue[NV * a + 2]);
         due[NV * a + 2] += M(-2 * p->coriolis_rate * ue[NV * a + 1]);
@@ -493,6 +521,25 @@ static int e3_rhs_impl(const ora_e3_params* p, const ora_connectivity* c,
 }
 
 extern "C" {
+
+// operation counts of one rhs! evaluation (the same code as ora_e3_rhs run
+// with the counting scalar; du receives the same values); counts[4] = +/-,
+// *, /, sqrt/log/exp/pow
+int ora_e3_count_flops(const ora_e3_params* p, const ora_connectivity* c, const double* cv,
+                       const double* invJ, const double* u, double* du, long long* counts) {
+  const size_t n3 = (size_t)p->n * p->n * p->n, E = p->n_elements;
+  std::vector<Counted> cvc(cv, cv + 9 * n3 * E), ijc(invJ, invJ + n3 * E),
+      uc(u, u + 6 * n3 * E), duc(6 * n3 * E);
+  ora_counts() = OraCounts{};
+  const int rc = e3_rhs_impl(p, c, cvc.data(), ijc.data(), uc.data(), duc.data(), false);
+  const OraCounts k = ora_counts();
+  counts[0] = k.add;
+  counts[1] = k.mul;
+  counts[2] = k.div;
+  counts[3] = k.special;
+  for (size_t q = 0; q < duc.size(); ++q) du[q] = duc[q].v;
+  return rc;
+}
 
 int ora_e3_rhs(const ora_e3_params* p, const ora_connectivity* c, const double* cv,
                const double* invJ, const double* u, double* du) {

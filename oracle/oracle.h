@@ -108,6 +108,12 @@ int ora_e3_ck2n54(const ora_e3_params* p, const ora_connectivity* c,
                   double* u, double dt, int n_steps);
 
 /* StepsizeCallback max_dt for P4estMesh{3} (without the CFL factor) */
+/* operation counts of one rhs! (the same code run with an operation-counting
+ * scalar, single-threaded; du receives the same values as ora_e3_rhs):
+ * counts[4] = additions and subtractions, multiplications, divisions,
+ * sqrt/log/exp/pow */
+int ora_e3_count_flops(const ora_e3_params* p, const ora_connectivity* c, const double* cv,
+                       const double* invJ, const double* u, double* du, long long* counts);
 double ora_e3_max_dt(const ora_e3_params* p, const double* contravariant,
                      const double* inverse_jacobian, const double* u);
 
