@@ -218,9 +218,11 @@ struct SweShape {
   static constexpr int NF = 4 * N;  // face nodes per element
 };
 
-// per-node fields staged in LDS: h, hv^1, hv^2, v^1, v^2 (v = hv / h), the
-// covariant velocity G v, J, G_11, G_12, G_22, G^11, G^12, G^22, h_s
-constexpr int kSweFields = 15;
+// per-node fields staged in LDS: h, hv^1, hv^2, v^1, v^2 (v = hv / h), J,
+// G_11, G_12, G_22, G^11, G^12, G^22, h_s (13 doubles per node: with the
+// face buffer 8 KB per wave at polydeg 3, which leaves room for five waves
+// per SIMD)
+constexpr int kSweFields = 13;
 
 }  // namespace
 
@@ -236,8 +238,12 @@ __device__ __forceinline__ void swe_node_speeds(const double* u, double gcon11, 
 // SPEED: the RK stage also writes L.speed_partial (the last stage of a
 // resident step); a variant of its own, so the other stages keep their
 // register budget
+// polydeg 3 fits five waves per SIMD (90-92 VGPRs, 8 KB of LDS per wave, no
+// spills); the other degrees keep the compiler's register choice (a five-wave
+// budget spills there)
 template <int N, int WAVES, bool PERSIST, bool SPEED>
-__global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(N == 4 ? 5 : 1)))
+void swe_stage(SweLaunch L) {
   using S = SweShape<N, WAVES>;
   constexpr int N2 = S::N2, EPB = S::EPB, B = S::BLOCK;
   __shared__ double s_n[EPB][kSweFields][N2];
@@ -265,16 +271,14 @@ __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
       s_n[es][2][q] = hv2;
       s_n[es][3][q] = v1;
       s_n[es][4][q] = v2;
-      s_n[es][5][q] = G11 * v1 + G12 * v2;
-      s_n[es][6][q] = G12 * v1 + G22 * v2;
-      s_n[es][7][q] = gg[0];
-      s_n[es][8][q] = G11;
-      s_n[es][9][q] = G12;
-      s_n[es][10][q] = G22;
-      s_n[es][11][q] = gg[4 * NN];
-      s_n[es][12][q] = gg[5 * NN];
-      s_n[es][13][q] = gg[6 * NN];
-      s_n[es][14][q] = gg[7 * NN];
+      s_n[es][5][q] = gg[0];
+      s_n[es][6][q] = G11;
+      s_n[es][7][q] = G12;
+      s_n[es][8][q] = G22;
+      s_n[es][9][q] = gg[4 * NN];
+      s_n[es][10][q] = gg[5 * NN];
+      s_n[es][11][q] = gg[6 * NN];
+      s_n[es][12][q] = gg[7 * NN];
     }
     __syncthreads();
 
@@ -287,10 +291,10 @@ __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
       for (int m = 0; m < N; ++m) {
         const int p1 = m + N * tj, p2 = ti + N * m;
         double f1[NV], f2[NV];
-        flux_standard(L.gravity, s_n[es][0][p1], s_n[es][1][p1], s_n[es][2][p1], s_n[es][7][p1],
-                      s_n[es][11][p1], s_n[es][12][p1], s_n[es][1][p1], f1);
-        flux_standard(L.gravity, s_n[es][0][p2], s_n[es][1][p2], s_n[es][2][p2], s_n[es][7][p2],
-                      s_n[es][12][p2], s_n[es][13][p2], s_n[es][2][p2], f2);
+        flux_standard(L.gravity, s_n[es][0][p1], s_n[es][1][p1], s_n[es][2][p1], s_n[es][5][p1],
+                      s_n[es][9][p1], s_n[es][10][p1], s_n[es][1][p1], f1);
+        flux_standard(L.gravity, s_n[es][0][p2], s_n[es][1][p2], s_n[es][2][p2], s_n[es][5][p2],
+                      s_n[es][10][p2], s_n[es][11][p2], s_n[es][2][p2], f2);
         const double d1 = L.dhat[ti * N + m], d2 = L.dhat[tj * N + m];
 #pragma unroll
         for (int v = 0; v < NV; ++v) du[v] += d1 * f1[v] + d2 * f2[v];
@@ -298,8 +302,8 @@ __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
     } else if (active) {
       const double hl = s_n[es][0][q], hvl[2] = {s_n[es][1][q], s_n[es][2][q]};
       const double vl[2] = {s_n[es][3][q], s_n[es][4][q]};
-      const double Jl = s_n[es][7][q], hsl = s_n[es][14][q];
-      const double Gi[3] = {s_n[es][11][q], s_n[es][12][q], s_n[es][13][q]};
+      const double Jl = s_n[es][5][q], hsl = s_n[es][12][q];
+      const double Gi[3] = {s_n[es][9][q], s_n[es][10][q], s_n[es][11][q]};
 #pragma unroll
       for (int o = 0; o < 2; ++o) {
         const int own = o == 0 ? ti : tj;
@@ -309,8 +313,10 @@ __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
           const int pq = o == 0 ? m + N * tj : ti + N * m;
           const double hr = s_n[es][0][pq], hvr_o = s_n[es][1 + o][pq];
           const double vr[2] = {s_n[es][3][pq], s_n[es][4][pq]};
-          const double vc0 = s_n[es][5][pq], vc1 = s_n[es][6][pq];
-          const double Jr = s_n[es][7][pq];
+          // the partner's covariant velocity G v from its staged G and v
+          const double vc0 = s_n[es][6][pq] * vr[0] + s_n[es][7][pq] * vr[1];
+          const double vc1 = s_n[es][7][pq] * vr[0] + s_n[es][8][pq] * vr[1];
+          const double Jr = s_n[es][5][pq];
           double f[NV];
           if (L.volume_flux == 0) {  // flux_ec (:100-122)
             const double mass = 0.5 * (Jl * hvl[o] + Jr * hvr_o);
@@ -324,7 +330,7 @@ __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
             f[2] = 0.5 * (ml * vl[1] + mr * vr[1]);
           }
           // flux_nonconservative_ec (:157-187) of (own, partner)
-          const double hs_jump = s_n[es][14][pq] - hsl;
+          const double hs_jump = s_n[es][12][pq] - hsl;
           const double geo0 = 0.5 * hvl[o] * ((Gi[0] * vc0 + Gi[1] * vc1) - vr[0]);
           const double geo1 = 0.5 * hvl[o] * ((Gi[1] * vc0 + Gi[2] * vc1) - vr[1]);
           const double Gio0 = o == 0 ? Gi[0] : Gi[1], Gio1 = o == 0 ? Gi[1] : Gi[2];
@@ -350,10 +356,10 @@ __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
       const double ul[NV] = {s_n[fe][0][qn], s_n[fe][1][qn], s_n[fe][2][qn]};
       // the own node's J, G, G^-1, h_s (aux rows 12-19) from LDS
       double xa[NA] = {};
-      xa[12] = s_n[fe][7][qn];
+      xa[12] = s_n[fe][5][qn];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) xa[13 + c] = s_n[fe][8 + c][qn];
-      xa[19] = s_n[fe][14][qn];
+      for (int c = 0; c < 6; ++c) xa[13 + c] = s_n[fe][6 + c][qn];
+      xa[19] = s_n[fe][12][qn];
       const AuxV ax{xa, 1};
       const size_t fnode = ((size_t)(e0 + fe) * 4 + f) * N + k;
       const int link = L.nbr[fnode];
@@ -385,7 +391,7 @@ __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
         for (int v = 0; v < NV; ++v) du[v] += s_sf[es][v][f * N + k];
       }
       const size_t g = (size_t)e * N2 + q;
-      const double fac = -1.0 / s_n[es][7][q];
+      const double fac = -1.0 / s_n[es][5][q];
       const double uo[NV] = {s_n[es][0][q], s_n[es][1][q], s_n[es][2][q]};
 #pragma unroll
       for (int v = 0; v < NV; ++v) du[v] *= fac;
@@ -393,9 +399,9 @@ __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
         const double* gg = L.geo + g;
         const long long NN = L.n_nodes;
         double xa[NA] = {};
-        xa[12] = s_n[es][7][q];
+        xa[12] = s_n[es][5][q];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) xa[13 + c] = s_n[es][8 + c][q];
+        for (int c = 0; c < 6; ++c) xa[13 + c] = s_n[es][6 + c][q];
 #pragma unroll
         for (int c = 0; c < 6; ++c) xa[20 + c] = gg[(8 + c) * NN];
         double sv[NV];
@@ -423,7 +429,7 @@ __global__ __launch_bounds__(64 * WAVES) void swe_stage(SweLaunch L) {
           // the node's speeds go to its first two LDS fields, which only this
           // lane reads after the face phase and which it has read already
           double l[2];
-          swe_node_speeds(un, s_n[es][11][q], s_n[es][13][q], L.gravity, l);
+          swe_node_speeds(un, s_n[es][9][q], s_n[es][11][q], L.gravity, l);
           s_n[es][0][q] = l[0];
           s_n[es][1][q] = l[1];
         }
