@@ -368,14 +368,35 @@ void swe_stage(SweLaunch L) {
       // then global2contravariant: covariant_shallow_water.jl:140-153)
       const double* M = L.fxf + fnode * 4;
       const double tr[NV] = {ur[0], M[0] * ur[1] + M[1] * ur[2], M[2] * ur[1] + M[3] * ur[2]};
-      const int o = f / 2;
       double fs[NV];
-      if (f % 2 == 0) {  // negative-side face: -(f*(u_nbr, u_own) + g/2)
-        surface_terms(L, tr, ul, ax, o, fs);
+      if constexpr (N == 4) {
+        // polydeg 3 keeps everything in registers with the orientation as a
+        // run-time value, and runs faster so (measured)
+        const int o = f / 2;
+        if (f % 2 == 0) {  // negative-side face: -(f*(u_nbr, u_own) + g/2)
+          surface_terms(L, tr, ul, ax, o, fs);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) fs[v] = -fs[v];
+          for (int v = 0; v < NV; ++v) fs[v] = -fs[v];
+        } else {
+          surface_terms(L, ul, tr, ax, o, fs);
+        }
       } else {
-        surface_terms(L, ul, tr, ax, o, fs);
+        // the orientation as a compile-time constant in each branch: every
+        // index into the node arrays (u[1 + o], G^{io}) is then fixed and they
+        // stay in registers (with a run-time orientation they went to scratch)
+        if (f == 0) {  // negative-side faces: -(f*(u_nbr, u_own) + g/2)
+          surface_terms(L, tr, ul, ax, 0, fs);
+#pragma unroll
+          for (int v = 0; v < NV; ++v) fs[v] = -fs[v];
+        } else if (f == 2) {
+          surface_terms(L, tr, ul, ax, 1, fs);
+#pragma unroll
+          for (int v = 0; v < NV; ++v) fs[v] = -fs[v];
+        } else if (f == 1) {
+          surface_terms(L, ul, tr, ax, 0, fs);
+        } else {
+          surface_terms(L, ul, tr, ax, 1, fs);
+        }
       }
 #pragma unroll
       for (int v = 0; v < NV; ++v) s_sf[fe][v][r] = fs[v] * inv_w;
