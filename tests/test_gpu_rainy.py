@@ -79,7 +79,7 @@ def test_rhs_bubble(oracle, vol, surf, periodic_y):
     check_rhs(oracle, mesh, u.reshape(9, 4, 4, -1, order="F"), vol, surf, "source_terms_rainy")
 
 
-@pytest.mark.parametrize("polydeg", [1, 2, 4, 5])
+@pytest.mark.parametrize("polydeg", [1, 2, 4, 5, 6, 7])
 def test_rhs_polydeg(oracle, polydeg):
     mesh, u = bubble_state(nx=4, ny=3, polydeg=polydeg)
     n = polydeg + 1
