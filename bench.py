@@ -74,62 +74,77 @@ def bench_covariant(steps, warmup, trees=32, polydeg=3, n_members=32):
     """covariant shallow water on the cubed sphere (radius 6.371e6 m, g = 9.81,
     polydeg 3, flux_ec volume, EC + LLF surface): one member of 6 x 32^2
     elements, and an ensemble of 32 members with 6 k^2 elements each, k drawn
-    from {8, 16, 24, 32}, advanced by one launch per RK stage"""
+    from {8, 16, 24, 32}, advanced by one launch per RK stage; each with FP64
+    geometry and with FP32 geometry storage (FP64 arithmetic)"""
     import numpy as np
     import trixiatmo_hip as T
     W = T.workloads
+
+    def timed(semi, run):
+        run(warmup)
+        semi.synchronize()
+        t0 = time.perf_counter()
+        run(steps)
+        semi.synchronize()
+        el = time.perf_counter() - t0
+        semi.enable_timing(True)
+        run(max(2, steps // 2))
+        launches, total_ms = semi.kernel_times(0)
+        semi.enable_timing(False)
+        return el, total_ms / launches / 1e3
+
     out = {}
-    case = W.zonal_jet(trees, polydeg, seed=0)
-    semi = case.semi()
-    u0 = case.u0()
-    semi.set_state(u0)
-    dt = semi.max_dt(0.4)
-    semi.step(dt, warmup)
-    semi.synchronize()
-    t0 = time.perf_counter()
-    semi.step(dt, steps)
-    semi.synchronize()
-    el = time.perf_counter() - t0
-    semi.enable_timing(True)
-    semi.step(dt, max(2, steps // 2))
-    launches, total_ms = semi.kernel_times(0)
-    semi.enable_timing(False)
-    nodes = semi.ndofs
-    avg_s = total_ms / launches / 1e3
-    # algorithmic bytes per node and stage: u, k read, k and u_out written
-    # (3 FP64 each), the 15 geometry values the stage reads (J, G, G^-1, h_s,
-    # the Christoffel symbols, z / r), and per face node (4 n of them per n^2
-    # nodes) the 2 x 2 basis map (FP64) and the neighbour link (int32)
-    bpn = 8 * (4 * 3 + 15) + (8 * 4 + 4) * 4 / (polydeg + 1)
-    out["single"] = {"workload": f"6x{trees}^2 elements, polydeg {polydeg}", "nodes": nodes,
-                     "value": nodes * 5 * steps / el / 1e9, "unit": "GDOF*RHS/s",
-                     "ms_per_step": el / steps * 1e3, "avg_launch_us": avg_s * 1e6,
-                     "hbm_achieved_gbs": bpn * nodes / avg_s / 1e9, "bytes_per_node": bpn,
-                     "finite": bool(np.all(np.isfinite(semi.get_state())))}
-    del semi
-    members, eq, solver, src = W.zonal_jet_ensemble(n_members=n_members, polydeg=polydeg, seed=0)
-    ens = T.SemidiscretizationEnsemble(members, eq, solver, source_terms=src)
-    u0 = ens.compute_coefficients()
-    ens.set_state(u0)
-    dts = ens.max_dt_members(0.4)
-    ens.step_members(dts, warmup)
-    ens.synchronize()
-    t0 = time.perf_counter()
-    ens.step_members(dts, steps)
-    ens.synchronize()
-    el = time.perf_counter() - t0
-    ens.enable_timing(True)
-    ens.step_members(dts, max(2, steps // 2))
-    launches, total_ms = ens.kernel_times(0)
-    ens.enable_timing(False)
-    nodes = ens.ndofs
-    avg_s = total_ms / launches / 1e3
-    out["ensemble"] = {"workload": f"{n_members} members, 6k^2 elements, k in {{8,16,24,32}}",
-                       "sizes": W.ensemble_sizes(n_members, 0), "nodes": nodes,
-                       "value": nodes * 5 * steps / el / 1e9, "unit": "GDOF*RHS/s",
-                       "ms_per_step": el / steps * 1e3, "avg_launch_us": avg_s * 1e6,
-                       "hbm_achieved_gbs": bpn * nodes / avg_s / 1e9,
-                       "finite": bool(np.all(np.isfinite(ens.get_state())))}
+    ref_du = {}
+    for prec, tag in (("fp64", ""), ("fp32_geometry", "_fp32_geometry")):
+        # algorithmic bytes per node and stage: u, k read, k and u_out written
+        # (3 FP64 each), the 15 geometry values the stage reads (J, G, G^-1,
+        # h_s, the Christoffel symbols, z / r), and per face node (4 n of them
+        # per n^2 nodes) the 2 x 2 basis map and the neighbour link (int32)
+        gb = 8 if prec == "fp64" else 4
+        bpn = 8 * 4 * 3 + gb * 15 + (gb * 4 + 4) * 4 / (polydeg + 1)
+        case = W.zonal_jet(trees, polydeg, seed=0, precision=prec)
+        semi = case.semi()
+        u0 = case.u0()
+        # rhs on a perturbed state, against the FP64 form (which matches the
+        # oracle to 1e-12 on the scale of the terms)
+        rng = np.random.default_rng(5)
+        up = u0 * (1.0 + 0.01 * rng.standard_normal(u0.shape))
+        du = semi.rhs(None, up).reshape(3, -1, order="F")
+        if prec == "fp64":
+            ref_du["single"] = du
+        semi.set_state(u0)
+        dt = semi.max_dt(0.4)
+        el, avg_s = timed(semi, lambda k: semi.step(dt, k))
+        nodes = semi.ndofs
+        rec = {"workload": f"6x{trees}^2 elements, polydeg {polydeg}", "precision": prec,
+               "nodes": nodes, "value": nodes * 5 * steps / el / 1e9, "unit": "GDOF*RHS/s",
+               "ms_per_step": el / steps * 1e3, "avg_launch_us": avg_s * 1e6,
+               "hbm_achieved_gbs": bpn * nodes / avg_s / 1e9, "bytes_per_node": bpn,
+               "finite": bool(np.all(np.isfinite(semi.get_state())))}
+        if prec != "fp64":
+            d = np.abs(du - ref_du["single"]).max(axis=1)
+            rec["rhs_rel_err_vs_fp64"] = (d / np.abs(ref_du["single"]).max(axis=1)).tolist()
+            rec["speedup_vs_fp64"] = out["single"]["avg_launch_us"] / rec["avg_launch_us"]
+        out["single" + tag] = rec
+        del semi
+        members, eq, solver, src = W.zonal_jet_ensemble(n_members=n_members, polydeg=polydeg,
+                                                        seed=0, precision=prec)
+        ens = T.SemidiscretizationEnsemble(members, eq, solver, source_terms=src)
+        u0 = ens.compute_coefficients()
+        ens.set_state(u0)
+        dts = ens.max_dt_members(0.4)
+        el, avg_s = timed(ens, lambda k: ens.step_members(dts, k))
+        nodes = ens.ndofs
+        rec = {"workload": f"{n_members} members, 6k^2 elements, k in {{8,16,24,32}}",
+               "precision": prec, "sizes": W.ensemble_sizes(n_members, 0), "nodes": nodes,
+               "value": nodes * 5 * steps / el / 1e9, "unit": "GDOF*RHS/s",
+               "ms_per_step": el / steps * 1e3, "avg_launch_us": avg_s * 1e6,
+               "hbm_achieved_gbs": bpn * nodes / avg_s / 1e9,
+               "finite": bool(np.all(np.isfinite(ens.get_state())))}
+        if prec != "fp64":
+            rec["speedup_vs_fp64"] = out["ensemble"]["avg_launch_us"] / rec["avg_launch_us"]
+        out["ensemble" + tag] = rec
+        del ens
     return out
 
 

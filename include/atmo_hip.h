@@ -57,6 +57,8 @@ extern "C" {
 #define ATMO_PREC_FP64 0          /* everything in FP64 */
 #define ATMO_PREC_FP32_KAHAN 1    /* FP32 fluxes, Kahan-compensated FP32 sums */
 #define ATMO_PREC_FP32 2          /* FP32 fluxes, plain FP32 sums */
+#define ATMO_PREC_FP32_GEOMETRY 3 /* covariant form: metric terms and face maps
+                                     stored in FP32, all arithmetic FP64 */
 
 /* node-local source terms (added after the Jacobian, as calc_sources!) */
 #define ATMO_SRC_NONE 0
@@ -212,7 +214,7 @@ typedef struct {
   int32_t surface_flux;     /* ATMO_SWE_SURF_* */
   int32_t surface_noncons;  /* ATMO_SWE_NC_* */
   int32_t source_terms;     /* 1: source_terms_geometric_coriolis, 2: the same of the standard form */
-  int32_t precision;        /* ATMO_PREC_FP64 */
+  int32_t precision;        /* ATMO_PREC_FP64 or ATMO_PREC_FP32_GEOMETRY */
   double gravity, rotation_rate; /* (Split)CovariantShallowWaterEquations2D(gravity, rotation_rate) */
 } atmo_swe_config;
 

@@ -553,6 +553,8 @@ void atmo_destroy(atmo_handle* h) {
   double* swe_bufs[] = {h->d_aux, h->d_xyz, h->d_dt_member, h->d_geo, h->d_fxf};
   for (double* p : swe_bufs)
     if (p) hipFree(p);
+  if (h->d_geo32) hipFree(h->d_geo32);
+  if (h->d_fxf32) hipFree(h->d_fxf32);
   if (h->d_clock) hipFree(h->d_clock);
   if (h->d_step_dt) hipFree(h->d_step_dt);
   if (h->e3_graph) hipGraphExecDestroy(h->e3_graph);

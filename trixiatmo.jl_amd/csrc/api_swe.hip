@@ -307,8 +307,10 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
       standard != (cfg->surface_flux == ATMO_SWE_SURF_LAX_FRIEDRICHS_STANDARD) ||
       (cfg->source_terms != 0 && standard != (cfg->source_terms == 2)))
     return fail(nullptr, ATMO_ERR_INVALID, "unsupported flux or source choice");
-  if (cfg->precision != ATMO_PREC_FP64)
-    return fail(nullptr, ATMO_ERR_INVALID, "the covariant form runs in FP64");
+  if (cfg->precision != ATMO_PREC_FP64 && cfg->precision != ATMO_PREC_FP32_GEOMETRY)
+    return fail(nullptr, ATMO_ERR_INVALID,
+                "the covariant form runs in FP64, with FP64 or FP32 geometry storage");
+  const bool geo32 = cfg->precision == ATMO_PREC_FP32_GEOMETRY;
   if (!(cfg->gravity > 0)) return fail(nullptr, ATMO_ERR_INVALID, "gravity must be positive");
   if (!g->aux_node_vars || !g->node_coordinates || g->n_interfaces < 0 ||
       (g->n_interfaces > 0 && (!g->interface_neighbor_ids || !g->interface_node_indices)))
@@ -378,6 +380,13 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
             M[2 * r + c] = bo[r] * an[3 * c] + bo[r + 2] * an[1 + 3 * c] + bo[r + 4] * an[2 + 3 * c];
       }
 
+  // FP32 storage: each value rounded to nearest once, here
+  std::vector<float> geo32v, fxf32v;
+  if (geo32) {
+    geo32v.assign(geo.begin(), geo.end());
+    fxf32v.assign(fxf.begin(), fxf.end());
+  }
+
   auto* h = new atmo_handle;
   h->eq = ATMO_EQ_COVARIANT_SWE_2D;
   h->swe = *cfg;
@@ -403,8 +412,10 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
       hipMalloc(&h->d_uio, sb) != hipSuccess || hipMalloc(&h->d_duio, sb) != hipSuccess ||
       hipMalloc(&h->d_aux, sizeof(double) * 26 * n2 * (size_t)E) != hipSuccess ||
       hipMalloc(&h->d_xyz, sizeof(double) * 3 * n2 * (size_t)E) != hipSuccess ||
-      hipMalloc(&h->d_geo, sizeof(double) * kSweGeo * n2 * (size_t)E) != hipSuccess ||
-      hipMalloc(&h->d_fxf, sizeof(double) * 16 * n * (size_t)E) != hipSuccess ||
+      (!geo32 && hipMalloc(&h->d_geo, sizeof(double) * geo.size()) != hipSuccess) ||
+      (!geo32 && hipMalloc(&h->d_fxf, sizeof(double) * fxf.size()) != hipSuccess) ||
+      (geo32 && hipMalloc(&h->d_geo32, sizeof(float) * geo.size()) != hipSuccess) ||
+      (geo32 && hipMalloc(&h->d_fxf32, sizeof(float) * fxf.size()) != hipSuccess) ||
       hipMalloc(&h->d_nbr, sizeof(int32_t) * nbr.size()) != hipSuccess ||
       hipMalloc(&h->d_member, sizeof(int32_t) * E) != hipSuccess ||
       hipMalloc(&h->d_member_first, sizeof(int32_t) * (M + 1)) != hipSuccess ||
@@ -416,10 +427,14 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
                 hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->d_xyz, g->node_coordinates, sizeof(double) * 3 * n2 * (size_t)E,
                 hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->d_geo, geo.data(), sizeof(double) * geo.size(), hipMemcpyHostToDevice) !=
-          hipSuccess ||
-      hipMemcpy(h->d_fxf, fxf.data(), sizeof(double) * fxf.size(), hipMemcpyHostToDevice) !=
-          hipSuccess ||
+      (!geo32 && (hipMemcpy(h->d_geo, geo.data(), sizeof(double) * geo.size(),
+                            hipMemcpyHostToDevice) != hipSuccess ||
+                  hipMemcpy(h->d_fxf, fxf.data(), sizeof(double) * fxf.size(),
+                            hipMemcpyHostToDevice) != hipSuccess)) ||
+      (geo32 && (hipMemcpy(h->d_geo32, geo32v.data(), sizeof(float) * geo32v.size(),
+                           hipMemcpyHostToDevice) != hipSuccess ||
+                 hipMemcpy(h->d_fxf32, fxf32v.data(), sizeof(float) * fxf32v.size(),
+                           hipMemcpyHostToDevice) != hipSuccess)) ||
       hipMemcpy(h->d_nbr, nbr.data(), sizeof(int32_t) * nbr.size(), hipMemcpyHostToDevice) !=
           hipSuccess ||
       hipMemcpy(h->d_member, member.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice) !=
@@ -448,6 +463,8 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
   L.xyz = h->d_xyz;
   L.geo = h->d_geo;
   L.fxf = h->d_fxf;
+  L.geo32 = h->d_geo32;
+  L.fxf32 = h->d_fxf32;
   L.n_nodes = (long long)NN;
   L.nbr = h->d_nbr;
   L.member = h->d_member;

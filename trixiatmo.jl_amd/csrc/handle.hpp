@@ -51,6 +51,7 @@ struct atmo_handle {
   atmo::SweLaunch swe_base{};
   double *d_aux = nullptr, *d_xyz = nullptr, *d_dt_member = nullptr;
   double *d_geo = nullptr, *d_fxf = nullptr;  // stage-kernel geometry (compact, face transforms)
+  float *d_geo32 = nullptr, *d_fxf32 = nullptr;  // the same in FP32 (ATMO_PREC_FP32_GEOMETRY)
   int *d_member = nullptr, *d_member_first = nullptr;
   int n_members = 1;
   bool swe_tuned = false;                  // workgroup shape chosen by the autotuner

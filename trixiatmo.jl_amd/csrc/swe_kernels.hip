@@ -224,6 +224,18 @@ struct SweShape {
 // per SIMD)
 constexpr int kSweFields = 13;
 
+// the stage's geometry rows and face maps in their storage type GT (double, or
+// float under ATMO_PREC_FP32_GEOMETRY); a float operand widens to double in
+// every expression it enters
+template <typename GT>
+__device__ __forceinline__ const GT* geo_rows(const SweLaunch& L) {
+  if constexpr (sizeof(GT) == 4) return L.geo32; else return L.geo;
+}
+template <typename GT>
+__device__ __forceinline__ const GT* face_maps(const SweLaunch& L) {
+  if constexpr (sizeof(GT) == 4) return L.fxf32; else return L.fxf;
+}
+
 }  // namespace
 
 // StepsizeCallback's scaled wave speeds at one node (stepsize_dg2d.jl:47-71):
@@ -241,7 +253,7 @@ __device__ __forceinline__ void swe_node_speeds(const double* u, double gcon11, 
 // polydeg 3 fits five waves per SIMD (90-92 VGPRs, 8 KB of LDS per wave, no
 // spills); the other degrees keep the compiler's register choice (a five-wave
 // budget spills there)
-template <int N, int WAVES, bool PERSIST, bool SPEED>
+template <int N, int WAVES, bool PERSIST, bool SPEED, typename GT>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(N == 4 ? 5 : 1)))
 void swe_stage(SweLaunch L) {
   using S = SweShape<N, WAVES>;
@@ -261,7 +273,7 @@ void swe_stage(SweLaunch L) {
     if (active) {
       const size_t g = (size_t)e * N2 + q;
       const double* un = L.u + g * NV;
-      const double* gg = L.geo + g;
+      const GT* gg = geo_rows<GT>(L) + g;
       const long long NN = L.n_nodes;
       const double h = un[0], hv1 = un[1], hv2 = un[2];
       const double v1 = hv1 / h, v2 = hv2 / h;
@@ -366,7 +378,7 @@ void swe_stage(SweLaunch L) {
       const double* ur = L.u + (size_t)link * NV;
       // neighbour state in this element's contravariant basis (contravariant2global,
       // then global2contravariant: covariant_shallow_water.jl:140-153)
-      const double* M = L.fxf + fnode * 4;
+      const GT* M = face_maps<GT>(L) + fnode * 4;
       const double tr[NV] = {ur[0], M[0] * ur[1] + M[1] * ur[2], M[2] * ur[1] + M[3] * ur[2]};
       double fs[NV];
       if constexpr (N == 4) {
@@ -417,7 +429,7 @@ void swe_stage(SweLaunch L) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) du[v] *= fac;
       if (L.source_terms) {
-        const double* gg = L.geo + g;
+        const GT* gg = geo_rows<GT>(L) + g;
         const long long NN = L.n_nodes;
         double xa[NA] = {};
         xa[12] = s_n[es][5][q];
@@ -473,18 +485,20 @@ void swe_stage(SweLaunch L) {
 }
 
 // per-element maximum of the scaled wave speeds (stepsize_dg2d.jl:47-71)
-template <int N>
+// G^11 and G^22 come from the stage's geometry rows 4 and 6 (aux rows 16 and
+// 18) in their storage type, the values the fused last stage uses
+template <int N, typename GT>
 __global__ __launch_bounds__(64) void swe_max_speed(SweLaunch L, double* partial) {
   constexpr int N2 = N * N;
   const int e = blockIdx.x, t = threadIdx.x;
   __shared__ double s1[64], s2[64];
   double l1 = 0.0, l2 = 0.0;
+  const GT* gr = geo_rows<GT>(L);
   for (int q = t; q < N2; q += 64) {
     const size_t g = (size_t)e * N2 + q;
     const double* u = L.u + g * NV;
-    const AuxV a{L.aux + g * NA, 1};
     double l[2];
-    swe_node_speeds(u, a.Gcon(0, 0), a.Gcon(1, 1), L.gravity, l);
+    swe_node_speeds(u, gr[4 * L.n_nodes + g], gr[6 * L.n_nodes + g], L.gravity, l);
     l1 = speed_max(l1, l[0]);
     l2 = speed_max(l2, l[1]);
   }
@@ -525,10 +539,21 @@ void launch_variant(const SweLaunch& L, hipStream_t s) {
   const int groups = (L.n_elements + S::EPB - 1) / S::EPB;
   // persistent: a few workgroups per CU (256 CUs), each looping over groups
   const int grid = PERSIST ? (groups < 256 * 8 ? groups : 256 * 8) : groups;
-  if (L.speed_partial && L.mode == 1)
-    hipLaunchKernelGGL((swe_stage<N, WAVES, PERSIST, true>), dim3(grid), dim3(S::BLOCK), 0, s, L);
-  else
-    hipLaunchKernelGGL((swe_stage<N, WAVES, PERSIST, false>), dim3(grid), dim3(S::BLOCK), 0, s, L);
+  const bool speed = L.speed_partial && L.mode == 1;
+  if (L.geo32) {
+    if (speed)
+      hipLaunchKernelGGL((swe_stage<N, WAVES, PERSIST, true, float>), dim3(grid), dim3(S::BLOCK), 0,
+                         s, L);
+    else
+      hipLaunchKernelGGL((swe_stage<N, WAVES, PERSIST, false, float>), dim3(grid), dim3(S::BLOCK),
+                         0, s, L);
+  } else if (speed) {
+    hipLaunchKernelGGL((swe_stage<N, WAVES, PERSIST, true, double>), dim3(grid), dim3(S::BLOCK), 0,
+                       s, L);
+  } else {
+    hipLaunchKernelGGL((swe_stage<N, WAVES, PERSIST, false, double>), dim3(grid), dim3(S::BLOCK), 0,
+                       s, L);
+  }
 }
 
 template <int N>
@@ -569,7 +594,12 @@ bool swe_launch_member_max_speed(const SweLaunch& L, double* partial, const int*
                                  int n_members, double* out, hipStream_t s) {
 #define ATMO_SWE_CASE(NN)                                                                       \
   case NN:                                                                                      \
-    hipLaunchKernelGGL(swe_max_speed<NN>, dim3(L.n_elements), dim3(64), 0, s, L, partial);     \
+    if (L.geo32)                                                                                \
+      hipLaunchKernelGGL((swe_max_speed<NN, float>), dim3(L.n_elements), dim3(64), 0, s, L,     \
+                         partial);                                                              \
+    else                                                                                        \
+      hipLaunchKernelGGL((swe_max_speed<NN, double>), dim3(L.n_elements), dim3(64), 0, s, L,    \
+                         partial);                                                              \
     break;
   switch (L.n) {
     ATMO_SWE_CASE(2)

@@ -29,6 +29,11 @@ struct SweLaunch {
   // contravariant basis at face node k of face f
   const double* geo;
   const double* fxf;
+  // the same two arrays rounded to FP32 (ATMO_PREC_FP32_GEOMETRY; NULL in
+  // FP64): the stage and the max-speed kernels then read these, and every
+  // operation stays FP64
+  const float* geo32;
+  const float* fxf32;
   long long n_nodes;
   const int* nbr;       // [E, 4, n] neighbour global node of each face node
   int mode;             // 0 rhs, 1 RK stage

@@ -14,7 +14,7 @@ from .equations import (EARTH_ROTATION_RATE, SWE_SURFACE_FLUXES, SWE_SURFACE_LLF
                         SWE_SURFACE_NONCONS, SWE_VOLUME_FLUXES, SWE_VOLUME_NONCONS,
                         SWE_VOLUME_WEAK_FORM, CovariantShallowWaterEquations2D,
                         SplitCovariantShallowWaterEquations2D)
-from .solvers import VolumeIntegralWeakForm
+from .solvers import PRECISIONS, VolumeIntegralWeakForm
 
 ATMO_EQ_EULER_ENERGY_GRAVITY_3D = 1
 ATMO_BC_SLIP_WALL = 0
@@ -67,11 +67,12 @@ class MetricTermsCovariantSphere:
 
 
 def _swe_config(equations, solver, n_elements, source_terms):
-    if solver.precision != "fp64":
-        raise ValueError("the covariant form runs in FP64")
+    if solver.precision not in ("fp64", "fp32_geometry"):
+        raise ValueError("the covariant form runs in FP64 (precision fp64 or fp32_geometry)")
     vol, sf, snc, src = swe_codes(equations, solver, source_terms)
     return AtmoSweConfig(polydeg=solver.polydeg, n_elements=n_elements, volume_flux=vol,
-                         surface_flux=sf, surface_noncons=snc, source_terms=src, precision=0,
+                         surface_flux=sf, surface_noncons=snc, source_terms=src,
+                         precision=PRECISIONS[solver.precision],
                          gravity=equations.gravity, rotation_rate=equations.rotation_rate)
 
 

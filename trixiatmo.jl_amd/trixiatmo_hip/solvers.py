@@ -4,7 +4,7 @@ from typing import Optional, Tuple, Union
 
 from .equations import (FluxLMARS, NONCONSERVATIVE_FLUXES, SURFACE_FLUXES, VOLUME_FLUXES)
 
-PRECISIONS = {"fp64": 0, "fp32_kahan": 1, "fp32": 2}
+PRECISIONS = {"fp64": 0, "fp32_kahan": 1, "fp32": 2, "fp32_geometry": 3}
 
 
 @dataclass(frozen=True)
@@ -24,6 +24,8 @@ class DGSEM:
 
     `precision` selects the arithmetic of the flux evaluations and volume sums
     ("fp64", "fp32_kahan" or "fp32"); RK registers stay FP64 in every form.
+    The covariant form takes "fp64" or "fp32_geometry" (its metric terms,
+    Christoffel symbols and face maps stored in FP32, every operation FP64).
     """
     polydeg: int
     surface_flux: Union[Tuple[Union[str, FluxLMARS], str], str]

@@ -179,13 +179,14 @@ SWE_RADIUS = 6.371e6
 SWE_GRAVITY = 9.81
 
 
-def zonal_jet(trees_per_face=32, polydeg=3, seed=0):
+def zonal_jet(trees_per_face=32, polydeg=3, seed=0, precision="fp64"):
     """Covariant shallow water on the cubed sphere of radius 6.371e6 m with
     g = 9.81: a seeded balanced zonal jet (u_0 in [20, 40] m/s, mean depth
     5960 m) plus three Gaussian height bumps; flux_ec +
     flux_nonconservative_ec volume, FluxPlusDissipation(flux_ec,
     DissipationLocalLaxFriedrichs()) + flux_nonconservative_surface_simplified
-    surface, source_terms_geometric_coriolis, FP64."""
+    surface, source_terms_geometric_coriolis; FP64 arithmetic with FP64 or
+    (precision "fp32_geometry") FP32 metric storage."""
     from . import initial_conditions as ic
     from .equations import EARTH_ROTATION_RATE, SplitCovariantShallowWaterEquations2D
     from .meshes import P4estMeshCubedSphere2D
@@ -194,7 +195,7 @@ def zonal_jet(trees_per_face=32, polydeg=3, seed=0):
     solver = DGSEM(polydeg=polydeg,
                    surface_flux=(SWE_EC_LLF, "flux_nonconservative_surface_simplified"),
                    volume_integral=VolumeIntegralFluxDifferencing(
-                       ("flux_ec", "flux_nonconservative_ec")))
+                       ("flux_ec", "flux_nonconservative_ec")), precision=precision)
     jet = ic.initial_condition_zonal_jet_bumps(seed, radius=SWE_RADIUS, gravity=SWE_GRAVITY)
     return CovariantCase(mesh, eq, solver, ic.transform_initial_condition(jet),
                          "source_terms_geometric_coriolis")
@@ -207,7 +208,8 @@ def ensemble_sizes(n_members=32, seed=0, choices=(8, 16, 24, 32)):
     return [int(k) for k in rng.choice(choices, size=n_members)]
 
 
-def zonal_jet_ensemble(n_members=32, polydeg=3, seed=0, choices=(8, 16, 24, 32)):
+def zonal_jet_ensemble(n_members=32, polydeg=3, seed=0, choices=(8, 16, 24, 32),
+                       precision="fp64"):
     """Ensemble of independent zonal-jet members with uneven element counts
     6 k^2, k drawn from `choices`; member m draws its jet and bumps from seed
     1000 * seed + m + 1.  Returns (members [(mesh, ic, None)], equations,
@@ -222,7 +224,7 @@ def zonal_jet_ensemble(n_members=32, polydeg=3, seed=0, choices=(8, 16, 24, 32))
         jet = ic.initial_condition_zonal_jet_bumps(1000 * seed + m + 1, radius=SWE_RADIUS,
                                                    gravity=SWE_GRAVITY)
         members.append((meshes[k], ic.transform_initial_condition(jet), None))
-    base = zonal_jet(1, polydeg)
+    base = zonal_jet(1, polydeg, precision=precision)
     return members, base.equations, base.solver, base.source_terms
 
 
