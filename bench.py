@@ -113,6 +113,7 @@ def bench_covariant(steps, warmup, trees=32, polydeg=3, n_members=32):
         if prec == "fp64":
             ref_du["single"] = du
         semi.set_state(u0)
+        shape, shape_ms = semi.autotune()
         dt = semi.max_dt(0.4)
         el, avg_s = timed(semi, lambda k: semi.step(dt, k))
         nodes = semi.ndofs
@@ -120,6 +121,7 @@ def bench_covariant(steps, warmup, trees=32, polydeg=3, n_members=32):
                "nodes": nodes, "value": nodes * 5 * steps / el / 1e9, "unit": "GDOF*RHS/s",
                "ms_per_step": el / steps * 1e3, "avg_launch_us": avg_s * 1e6,
                "hbm_achieved_gbs": bpn * nodes / avg_s / 1e9, "bytes_per_node": bpn,
+               "shape": shape, "shape_us": [round(x * 1e3, 2) for x in shape_ms],
                "finite": bool(np.all(np.isfinite(semi.get_state())))}
         if prec != "fp64":
             d = np.abs(du - ref_du["single"]).max(axis=1)
@@ -132,6 +134,7 @@ def bench_covariant(steps, warmup, trees=32, polydeg=3, n_members=32):
         ens = T.SemidiscretizationEnsemble(members, eq, solver, source_terms=src)
         u0 = ens.compute_coefficients()
         ens.set_state(u0)
+        shape, shape_ms = ens.autotune()
         dts = ens.max_dt_members(0.4)
         el, avg_s = timed(ens, lambda k: ens.step_members(dts, k))
         nodes = ens.ndofs
@@ -140,6 +143,7 @@ def bench_covariant(steps, warmup, trees=32, polydeg=3, n_members=32):
                "value": nodes * 5 * steps / el / 1e9, "unit": "GDOF*RHS/s",
                "ms_per_step": el / steps * 1e3, "avg_launch_us": avg_s * 1e6,
                "hbm_achieved_gbs": bpn * nodes / avg_s / 1e9,
+               "shape": shape, "shape_us": [round(x * 1e3, 2) for x in shape_ms],
                "finite": bool(np.all(np.isfinite(ens.get_state())))}
         if prec != "fp64":
             rec["speedup_vs_fp64"] = out["ensemble"]["avg_launch_us"] / rec["avg_launch_us"]

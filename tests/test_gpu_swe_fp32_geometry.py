@@ -142,3 +142,22 @@ def test_ensemble_matches_members():
         single.step(dts[m], 3)
         np.testing.assert_array_equal(single.get_state().reshape(single.shape, order="F"),
                                       u_ens[..., sl])
+
+
+@pytest.mark.gpu
+def test_resident_ensemble_members_take_their_own_steps():
+    members, eq, solver, src = W.zonal_jet_ensemble(n_members=3, polydeg=3, seed=8,
+                                                    choices=(2, 3), precision=GEO)
+    ens = T.SemidiscretizationEnsemble(members, eq, solver, source_terms=src)
+    u0 = ens.compute_coefficients()
+    ens.set_state(u0)
+    t_end = 8.5 * ens.max_dt_members(0.4).max()
+    print(f"case: fp32-geometry resident ensemble of {ens.n_members}, t_end {t_end}", flush=True)
+    u_ens, steps = T.solve(ens, u0, (0.0, t_end), 0.4, resident=True, check_every=4)
+    first = ens.member_first_element
+    for m, (mesh, icm, _) in enumerate(members):
+        single = T.SemidiscretizationHyperbolic(mesh, eq, solver, source_terms=src,
+                                                initial_condition=icm)
+        um, n = T.solve(single, single.compute_coefficients(), (0.0, t_end), 0.4)
+        assert steps[m] == n
+        np.testing.assert_array_equal(u_ens[..., first[m]:first[m + 1]], um)

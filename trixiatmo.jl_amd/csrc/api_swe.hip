@@ -147,10 +147,20 @@ int swe_autotune(atmo_handle* h, double* ms_per_variant) {
   hipEvent_t a, b;
   HIP_TRY(h, hipEventCreate(&a));
   HIP_TRY(h, hipEventCreate(&b));
+  // the shapes are timed on the RK stage, the launch the time loops make
+  // (its traffic differs from the rhs's: the register is read and written):
+  // the state is read and the scratch buffers d_u2 (the stage output) and d_du
+  // (as the register) are written; the resident state is left untouched
   SweLaunch L = h->swe_base;
-  L.mode = 0;
+  L.mode = 1;
   L.u = h->d_u;
-  L.out = h->d_du;
+  L.out = h->d_u2;
+  L.k = h->d_du;
+  L.rk_a = 0.0;
+  L.rk_dt = 0.0;
+  L.rk_b = 0.0;
+  L.dt_member = nullptr;
+  L.speed_partial = nullptr;
   // two rounds over all shapes, 5 timed launches each; a shape's time is its
   // best round (the rounds interleave the shapes, so clock and cache drift
   // affect them alike)
