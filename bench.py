@@ -97,11 +97,12 @@ def bench_covariant(steps, warmup, trees=32, polydeg=3, n_members=32):
     ref_du = {}
     for prec, tag in (("fp64", ""), ("fp32_geometry", "_fp32_geometry")):
         # algorithmic bytes per node and stage: u, k read, k and u_out written
-        # (3 FP64 each), the 15 geometry values the stage reads (J, G, G^-1,
-        # h_s, the Christoffel symbols, z / r), and per face node (4 n of them
+        # (3 FP64 each), the geometry values the stage reads (G, h_s, the
+        # Christoffel symbols, z / r, and with FP32 storage J; G^-1, and in
+        # FP64 J, are recomputed from G), and per face node (4 n of them
         # per n^2 nodes) the 2 x 2 basis map and the neighbour link (int32)
         gb = 8 if prec == "fp64" else 4
-        bpn = 8 * 4 * 3 + gb * 15 + (gb * 4 + 4) * 4 / (polydeg + 1)
+        bpn = 8 * 4 * 3 + (8 * 11 if gb == 8 else 4 * 12) + (gb * 4 + 4) * 4 / (polydeg + 1)
         case = W.zonal_jet(trees, polydeg, seed=0, precision=prec)
         semi = case.semi()
         u0 = case.u0()

@@ -222,3 +222,21 @@ def test_autotuner_shapes_and_graph_replay_are_bit_identical():
         semi.set_state(u0)
         semi.step(dt, 4)
         np.testing.assert_array_equal(semi.get_state(), a)
+
+
+def test_inconsistent_metric_terms_are_refused(monkeypatch):
+    """the stage recomputes J = sqrt(det G) and G^-1 from G: auxiliary
+    variables whose J or G^-1 disagree with their G are refused at creation"""
+    from trixiatmo_hip import semidiscretization as S
+    case = W.zonal_jet(2, 3, seed=0)
+    good = S.covariant_aux
+    for row in (12, 16, 17, 18):
+        def bad(mesh, metric_terms=None, auxiliary_field=None, row=row):
+            aux = good(mesh, metric_terms, auxiliary_field)
+            aux[row, 1, 2, 5] += 1e-6 * abs(aux[16 if row > 12 else 12, 1, 2, 5])
+            return aux
+        monkeypatch.setattr(S, "covariant_aux", bad)
+        with pytest.raises(T.AtmoError, match="sqrt\\(det G\\)"):
+            case.semi()
+    monkeypatch.setattr(S, "covariant_aux", good)
+    case.semi()

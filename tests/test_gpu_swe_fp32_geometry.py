@@ -9,11 +9,12 @@ Its errors are stated against the FP64 oracle, not hidden in a tolerance:
   magnitude mode), the rounding of the geometry entering linearly;
 - state after 100 RK steps of the zonal jet: <= 1e-5 of the change of the
   state over the run;
-- mass over the 100 steps: the quadrature with the FP32 Jacobian the scheme
-  computes with drifts 1.2e-13 relative (<= 1e-12): the two sides of a face
-  rotate the neighbour's momentum with independently rounded FP32 maps, so
-  their mass fluxes agree to FP32 rounding, not FP64; the analysis integral
-  (FP64 Jacobian) drifts 1.2e-12 (<= 1e-11).  The FP64 form conserves both to
+- mass over the 100 steps: the quadrature with the Jacobian the scheme
+  computes with (the area element rounded to FP32) drifts about 1e-13
+  relative (<= 1e-12): the two sides of a face rotate the neighbour's momentum with
+  independently rounded FP32 maps, so their mass fluxes agree to FP32
+  rounding, not FP64; the analysis integral (FP64 Jacobian) drifts 1.5e-12
+  (<= 1e-11).  The FP64 form conserves both to
   round-off.
 The resident loop and ensembles keep their exact bar: the fused speeds read
 the same FP32 values as the max-speed reduction, so the resident loop
@@ -95,7 +96,7 @@ def test_state_after_100_steps_and_mass():
     rel = np.abs((b - a).reshape(3, -1, order="F")).max(axis=1) / change
     drift = abs(s32.integrals()[0] - m0) / m0
     own = abs((wJ32 * b[0]).sum() - own0) / own0
-    print("  100-step state err / state change:", rel, " mass drift (FP32 Jacobian):", own,
+    print("  100-step state err / state change:", rel, " mass drift (scheme's Jacobian):", own,
           " (FP64 Jacobian):", drift, flush=True)
     assert np.all(rel <= 1e-5), rel
     assert own <= 1e-12, own

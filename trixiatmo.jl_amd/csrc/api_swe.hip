@@ -373,10 +373,22 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
   std::vector<double> geo(kSweGeo * NN), fxf((size_t)16 * n * E);
   for (size_t gq = 0; gq < NN; ++gq) {
     const double* a = g->aux_node_vars + gq * 26;
-    for (int c = 0; c < 8; ++c) geo[c * NN + gq] = a[12 + c];
-    for (int c = 0; c < 6; ++c) geo[(8 + c) * NN + gq] = a[20 + c];
+    for (int c = 0; c < 3; ++c) geo[c * NN + gq] = a[13 + c];
+    geo[3 * NN + gq] = a[19];
+    for (int c = 0; c < 6; ++c) geo[(4 + c) * NN + gq] = a[20 + c];
     const double* x = g->node_coordinates + gq * 3;
-    geo[14 * NN + gq] = x[2] / std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    geo[10 * NN + gq] = x[2] / std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    geo[11 * NN + gq] = a[12];
+    // the kernels recompute J and G^-1 from G: the given ones must agree
+    double J, c11, c12, c22;
+    metric_from_gcov(a[13], a[14], a[15], J, c11, c12, c22);
+    const double cs = std::fabs(c11) + std::fabs(c22);
+    if (!(std::fabs(J - a[12]) <= 1e-10 * std::fabs(a[12])) ||
+        !(std::fabs(c11 - a[16]) <= 1e-10 * cs) || !(std::fabs(c12 - a[17]) <= 1e-10 * cs) ||
+        !(std::fabs(c22 - a[18]) <= 1e-10 * cs))
+      return fail(nullptr, ATMO_ERR_INVALID,
+                  "aux_node_vars: J or the inverse metric is not sqrt(det G) or G^-1 at node " +
+                      std::to_string(gq));
   }
   for (int e = 0; e < E; ++e)
     for (int f = 0; f < 4; ++f)

@@ -277,20 +277,26 @@ void swe_stage(SweLaunch L) {
       const long long NN = L.n_nodes;
       const double h = un[0], hv1 = un[1], hv2 = un[2];
       const double v1 = hv1 / h, v2 = hv2 / h;
-      const double G11 = gg[NN], G12 = gg[2 * NN], G22 = gg[3 * NN];
+      const double G11 = gg[0], G12 = gg[NN], G22 = gg[2 * NN];
+      double J, c11, c12, c22;
+      metric_from_gcov(G11, G12, G22, J, c11, c12, c22);
+      // FP32 storage: J is the stored area element rounded once; it is the
+      // same number on both sides of a face (G is not: it is expressed in each
+      // element's own basis), so the two mass fluxes there agree
+      if constexpr (sizeof(GT) == 4) J = gg[11 * NN];
       s_n[es][0][q] = h;
       s_n[es][1][q] = hv1;
       s_n[es][2][q] = hv2;
       s_n[es][3][q] = v1;
       s_n[es][4][q] = v2;
-      s_n[es][5][q] = gg[0];
+      s_n[es][5][q] = J;
       s_n[es][6][q] = G11;
       s_n[es][7][q] = G12;
       s_n[es][8][q] = G22;
-      s_n[es][9][q] = gg[4 * NN];
-      s_n[es][10][q] = gg[5 * NN];
-      s_n[es][11][q] = gg[6 * NN];
-      s_n[es][12][q] = gg[7 * NN];
+      s_n[es][9][q] = c11;
+      s_n[es][10][q] = c12;
+      s_n[es][11][q] = c22;
+      s_n[es][12][q] = gg[3 * NN];
     }
     __syncthreads();
 
@@ -436,12 +442,12 @@ void swe_stage(SweLaunch L) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) xa[13 + c] = s_n[es][6 + c][q];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) xa[20 + c] = gg[(8 + c) * NN];
+        for (int c = 0; c < 6; ++c) xa[20 + c] = gg[(4 + c) * NN];
         double sv[NV];
         if (L.source_terms == 2)
-          source_standard(L.gravity, L.rotation_rate, uo, gg[14 * NN], AuxV{xa, 1}, sv);
+          source_standard(L.gravity, L.rotation_rate, uo, gg[10 * NN], AuxV{xa, 1}, sv);
         else
-          source_split(L.rotation_rate, uo, gg[14 * NN], AuxV{xa, 1}, sv);
+          source_split(L.rotation_rate, uo, gg[10 * NN], AuxV{xa, 1}, sv);
 #pragma unroll
         for (int v = 0; v < NV; ++v) du[v] += sv[v];
       }
@@ -485,8 +491,8 @@ void swe_stage(SweLaunch L) {
 }
 
 // per-element maximum of the scaled wave speeds (stepsize_dg2d.jl:47-71)
-// G^11 and G^22 come from the stage's geometry rows 4 and 6 (aux rows 16 and
-// 18) in their storage type, the values the fused last stage uses
+// G^11 and G^22 are recomputed from the stage's G rows in their storage type,
+// as the fused last stage does
 template <int N, typename GT>
 __global__ __launch_bounds__(64) void swe_max_speed(SweLaunch L, double* partial) {
   constexpr int N2 = N * N;
@@ -497,8 +503,9 @@ __global__ __launch_bounds__(64) void swe_max_speed(SweLaunch L, double* partial
   for (int q = t; q < N2; q += 64) {
     const size_t g = (size_t)e * N2 + q;
     const double* u = L.u + g * NV;
-    double l[2];
-    swe_node_speeds(u, gr[4 * L.n_nodes + g], gr[6 * L.n_nodes + g], L.gravity, l);
+    double J, c11, c12, c22, l[2];
+    metric_from_gcov(gr[g], gr[L.n_nodes + g], gr[2 * L.n_nodes + g], J, c11, c12, c22);
+    swe_node_speeds(u, c11, c22, L.gravity, l);
     l1 = speed_max(l1, l[0]);
     l2 = speed_max(l2, l[1]);
   }

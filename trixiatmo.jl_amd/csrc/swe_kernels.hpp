@@ -22,9 +22,9 @@ struct SweLaunch {
   double* k;
   const double* aux;    // [26, n, n, E]
   const double* xyz;    // [3, n, n, E]
-  // the stage kernel's geometry: geo[c * n_nodes + node] for c = J, G_11, G_12,
-  // G_22, G^11, G^12, G^22, h_s, the six Christoffel symbols (aux rows 12-25)
-  // and z / r; fxf[((e * 4 + f) * n + k) * 4 + 2 r + s] = (B_own A_nbr)_rs, the
+  // the stage kernel's geometry: geo[c * n_nodes + node] for c = G_11, G_12,
+  // G_22, h_s, the six Christoffel symbols (aux rows 13-15, 19-25), z / r and
+  // J (G^-1, and in FP64 J, are recomputed from G where they are needed); fxf[((e * 4 + f) * n + k) * 4 + 2 r + s] = (B_own A_nbr)_rs, the
   // 2 x 2 map of a neighbour's contravariant momentum into this element's
   // contravariant basis at face node k of face f
   const double* geo;
@@ -45,7 +45,21 @@ struct SweLaunch {
   int variant;              // workgroup shape: 0..2 = 1, 2, 4 waves; +3 persistent grid
 };
 constexpr int kSweVariants = 6;
-constexpr int kSweGeo = 15;
+constexpr int kSweGeo = 12;
+
+// J = sqrt(det G) and G^-1 = adj(G) / det G from the covariant metric tensor
+// G = A^T A (the identities aux rows 12 and 16-18 satisfy: J is the area
+// element |a_1 x a_2|); one sequence of operations, with an explicit fused
+// multiply-add, on the host and in every kernel, so each use gets the same bits
+__host__ __device__ inline void metric_from_gcov(double g11, double g12, double g22, double& J,
+                                                 double& c11, double& c12, double& c22) {
+  const double det = fma(g11, g22, -(g12 * g12));
+  const double inv = 1.0 / det;
+  J = sqrt(det);
+  c11 = g22 * inv;
+  c12 = -g12 * inv;
+  c22 = g11 * inv;
+}
 
 bool swe_launch_stage(const SweLaunch& L, hipStream_t s);
 // per-member max over elements of the summed scaled wave speeds -> out[n_members]
