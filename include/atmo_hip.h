@@ -233,7 +233,10 @@ typedef struct {
 /* SemidiscretizationHyperbolic for the covariant form on P4estMesh{2}
  * (src/semidiscretization/semidiscretization_hyperbolic_2d_manifold_in_3d.jl:78-111,
  * cache of src/solvers/dgsem_p4est/dg.jl:5-32); an ensemble of independent
- * members is one handle whose stage launches cover all members. */
+ * members is one handle whose stage launches cover all members.  The kernels
+ * recompute G^-1 (and, in FP64, J) from G: aux rows 12 and 16-18 must equal
+ * sqrt(det G) and G^-1 of rows 13-15 to 1e-10 relative (as the reference's
+ * auxiliary variables do), else ATMO_ERR_INVALID. */
 int atmo_swe_create(const atmo_swe_config* config, const atmo_geometry2d* geometry,
                     atmo_handle** handle);
 
