@@ -162,3 +162,34 @@ def test_resident_ensemble_members_take_their_own_steps():
         um, n = T.solve(single, single.compute_coefficients(), (0.0, t_end), 0.4)
         assert steps[m] == n
         np.testing.assert_array_equal(u_ens[..., first[m]:first[m + 1]], um)
+
+
+def _rhs_err(oracle, case, u):
+    semi = case.semi()
+    so = SweOracle.for_semi(oracle, semi)
+    du_ref = so.rhs(u)
+    mag = np.abs(so.rhs_magnitude(u).reshape(3, -1, order="F")).max(axis=1)
+    du = semi.rhs(None, u)
+    return np.abs((du - du_ref).reshape(3, -1, order="F")).max(axis=1) / mag
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("polydeg", [1, 2, 4, 5, 6, 7])
+def test_rhs_error_every_polydeg(oracle, polydeg):
+    case = W.zonal_jet(3, polydeg, seed=2, precision=GEO)
+    err = _rhs_err(oracle, case, perturbed(case))
+    print(f"  polydeg {polydeg}: fp32 geometry rel err (term scale)", err, flush=True)
+    assert np.all(err <= 1e-6), err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,over", [
+    ("geostrophic_balance", {}),                        # standard form, weak form
+    ("rossby_haurwitz", {"volume_flux": ("flux_central", "flux_nonconservative_ec")}),
+    ("well_balanced", {}),                              # bottom topography
+])
+def test_rhs_error_other_forms(oracle, name, over):
+    case = W.covariant_elixir(name, cells=3, polydeg=3, precision=GEO, **over)
+    err = _rhs_err(oracle, case, perturbed(case))
+    print(f"  {name}: fp32 geometry rel err (term scale)", err, flush=True)
+    assert np.all(err <= 1e-6), err

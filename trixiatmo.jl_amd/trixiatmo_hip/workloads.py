@@ -164,12 +164,14 @@ def covariant_elixir(name, cells=5, polydeg=3, **override):
         eq = CovariantShallowWaterEquations2D(EARTH_GRAVITATIONAL_ACCELERATION,
                                               EARTH_ROTATION_RATE)
         solver = DGSEM(polydeg=polydeg, surface_flux=spec["surface_flux"],
-                       volume_integral=VolumeIntegralWeakForm())
+                       volume_integral=VolumeIntegralWeakForm(),
+                       precision=spec.get("precision", "fp64"))
     else:
         eq = SplitCovariantShallowWaterEquations2D(EARTH_GRAVITATIONAL_ACCELERATION,
                                                    EARTH_ROTATION_RATE)
         solver = DGSEM(polydeg=polydeg, surface_flux=tuple(spec["surface_flux"]),
-                       volume_integral=VolumeIntegralFluxDifferencing(tuple(spec["volume_flux"])))
+                       volume_integral=VolumeIntegralFluxDifferencing(tuple(spec["volume_flux"])),
+                       precision=spec.get("precision", "fp64"))
     return CovariantCase(mesh, eq, solver, ic.transform_initial_condition(spec["ic"]),
                          "source_terms_geometric_coriolis", spec["field"],
                          spec.get("christoffel", "ChristoffelSymbolsAutodiff()"), spec["cfl"])
