@@ -491,6 +491,9 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
   L.nbr = h->d_nbr;
   L.member = h->d_member;
   L.dt_member = nullptr;
+  // without bottom topography (h_s = 0 at every node) the stage skips its row
+  L.topography = 0;
+  for (size_t gq = 0; gq < NN && !L.topography; ++gq) L.topography = geo[3 * NN + gq] != 0.0;
   L.variant = 0;
   *out = h;
   return ATMO_OK;

@@ -42,6 +42,7 @@ struct SweLaunch {
   double* speed_partial;    // RK stage, if set: per element the StepsizeCallback sum over
                             // directions of the maximum scaled speed of the new state
   const int* member;        // [E] member of each element
+  int topography;           // 0: h_s is zero at every node and its row is not read
   int variant;              // workgroup shape: 0..2 = 1, 2, 4 waves; +3 persistent grid
 };
 constexpr int kSweVariants = 6;
