@@ -6,7 +6,13 @@ Pin: test/test_2d_rainy_euler.jl:7-31 (examples/euler/precipitation/tests/
 convergence_test.jl: 6x6 periodic StructuredMesh on [0, 2]^2, polydeg 3, weak
 form, flux_lax_friedrichs, manufactured source, NonlinearSolveDG with tol
 1e-8 as the stage limiter of CarpenterKennedy2N54, StepsizeCallback(cfl = 1),
-t = 2).  Closed forms: saturated states of that test and unsaturated states
+t = 2); test/test_2d_rainy_euler.jl:79-114 (the rainy bubble of
+examples/euler/precipitation/buoyancy/elixir_hoeck_bubble.jl, 16 x 16 cells,
+10 s: flux_ec_rain, LLF, source_terms_rainy, the simple slip wall and the
+stage limiter together; with the reference's adaptive SSPRK43 restated, and
+with this build's fixed-step integrator within the measured integrator gap);
+test/test_unit.jl:526-544 (flux_ec_rain against flux on the reference's
+state and normals).  Closed forms: saturated states of that test and unsaturated states
 solve the saturation system exactly; the Jacobian and the derivative of the
 saturation pressure match finite differences of the functions they
 differentiate.  Invariants: conservation of dry air, of total water and of
@@ -174,6 +180,22 @@ def test_flux_ec_rain_consistency_and_symmetry(oracle):
                                    atol=1e-14 * np.abs(f_llf).max())
 
 
+def test_flux_ec_rain_reference_consistency_pattern(oracle):
+    """the reference's own known answer (test/test_unit.jl:526-544): on the
+    state (1.0, 0.2, 0.1, 0.5, -0.4, 2.2, 0.1, 0.1, 300), whose energy and
+    temperature do not agree, flux_ec_rain(u, u, n) equals flux(u, n) under
+    Julia's isapprox (rtol sqrt(eps)) in every row except the energy row, for
+    the four normals of that test"""
+    ro = RainyOracle(oracle, conv_mesh())
+    u = np.array([1.0, 0.2, 0.1, 0.5, -0.4, 2.2, 0.1, 0.1, 300.0])
+    rtol = np.sqrt(np.finfo(float).eps)
+    expected = [True, True, True, True, True, False, True, True, True]
+    for n in ([1.0, 0.0], [0.0, 1.0], [0.5, -0.5], [-1.2, 0.3]):
+        a, b = ro.flux_ec(u, u, np.array(n)), ro.flux(u, np.array(n))
+        equal = np.abs(a - b) <= rtol * np.maximum(np.abs(a), np.abs(b))
+        assert equal.tolist() == expected, (n, a, b)
+
+
 def _bubble(nx=8, ny=6, periodic_y=False, seed=3):
     mesh = StructuredMesh2D((nx, ny), (0.0, 0.0), (4000.0, 3000.0), polydeg=3,
                             periodicity=(True, periodic_y))
@@ -309,3 +331,103 @@ def test_mlp_oracle_forward_matches_its_loss(oracle):
     yt = (y.astype(np.float64) - norm[6:8]) / scale
     yo = (out - norm[6:8]) / scale
     np.testing.assert_allclose(((yo - yt) ** 2).sum(axis=1).mean(), L.value, rtol=1e-12)
+
+
+# --- test/test_2d_rainy_euler.jl:79-114: the rainy bubble (elixir_hoeck_bubble.jl,
+# 16 x 16 cells, 10 s): flux_ec_rain volume, flux_lax_friedrichs surface,
+# source_terms_rainy, boundary_condition_simple_slip_wall, NonlinearSolveDG
+
+# bars of the oracle run with the reference's integrator restated: three times
+# the largest difference found (1.4e-6 l2, 3.5e-5 linf; rain 1e-4 / 6.5e-4)
+BUBBLE_RTOL_L2, BUBBLE_RTOL_LINF = 3e-5, 1e-4
+BUBBLE_RAIN_RTOL_L2, BUBBLE_RAIN_RTOL_LINF = 5e-4, 3e-3
+
+
+def _bubble_oracle(oracle):
+    import rainy_bubble_pin as B
+    mesh = B.bubble_mesh()
+    ro = RainyOracle(oracle, mesh, volume_integral=1, source="source_terms_rainy")
+    u0 = B.exact(mesh.node_coordinates, 0.0)
+    return B, mesh, ro, u0
+
+
+def _bubble_bars(rain_l2, rain_linf, rest_l2, rest_linf):
+    l2 = np.full(9, rest_l2)
+    linf = np.full(9, rest_linf)
+    l2[2], linf[2] = rain_l2, rain_linf
+    return l2, linf
+
+
+def test_bubble_initial_condition_solves_its_residuals():
+    """every node of the bubble's initial state is a root of the reference's
+    perturbation residual (saturated in the bubble core, relative humidity 0.2
+    outside), and the hydrostatic layers are roots of theirs"""
+    import rainy_bubble_pin as B
+    L = B.layers()
+    mesh = B.bubble_mesh()
+    u = B.exact(mesh.node_coordinates, 0.0)
+    T_ = u[8]
+    es = EQ.saturation_vapour_pressure(T_)
+    rvs = es / (EQ.R_vapour * T_)
+    eps = EQ.R_dry_air / EQ.R_vapour
+    x, z = mesh.node_coordinates
+    r = np.sqrt((x - 1200) ** 2 + (z - 800) ** 2)
+    p = (u[0] * EQ.R_dry_air + u[1] * EQ.R_vapour) * T_
+    H = u[1] * (u[0] + rvs / eps) / (rvs * (u[0] + u[1] / eps))  # humidity residual solved for H
+    assert np.allclose(H[r > 300], 0.2, rtol=1e-12)
+    assert np.allclose(H[r < 200], 1.0, rtol=1e-12)
+    assert np.all(np.diff(L.layer_data[:, 0]) < 0)  # pressure falls with height
+    d = L.layer_data
+    # hydrostatic residual of each layer on the one below (dz = 1)
+    res1 = (d[1:, 0] - d[:-1, 0]) + (d[1:, 1] + d[1:, 2]) * EQ.gravity
+    assert np.abs(res1).max() < 1e-9 * d[0, 0]
+    assert np.abs(d[:, 0] - d[:, 3] * (d[:, 1] * EQ.R_dry_air + d[:, 2] * EQ.R_vapour)).max() \
+        < 1e-10 * d[0, 0]
+    assert np.all(p > 0)
+
+
+def test_bubble_regression_with_the_reference_integrator(oracle):
+    """the oracle integrated with the reference's adaptive SSPRK43 (default
+    tolerances, stage limiter after every stage) reproduces the regression
+    values of all nine variables"""
+    B, mesh, ro, u0 = _bubble_oracle(oracle)
+    u, steps = B.ssprk43_adaptive(lambda v, t: ro.rhs(v, t), lambda v: ro.newton(v)[0],
+                                  ro._u(u0), B.T_END)
+    u = u.reshape(u0.shape, order="F")
+    l2, linf = calc_error_norms_2d(u, mesh, B.exact, B.T_END)
+    print("SSPRK43 steps", steps, "\nl2", (l2 - B.BUBBLE_L2) / B.BUBBLE_L2,
+          "\nlinf", (linf - B.BUBBLE_LINF) / B.BUBBLE_LINF)
+    bl2, blinf = _bubble_bars(BUBBLE_RAIN_RTOL_L2, BUBBLE_RAIN_RTOL_LINF, BUBBLE_RTOL_L2,
+                              BUBBLE_RTOL_LINF)
+    assert np.all(np.abs(l2 - B.BUBBLE_L2) <= bl2 * B.BUBBLE_L2)
+    assert np.all(np.abs(linf - B.BUBBLE_LINF) <= blinf * B.BUBBLE_LINF)
+
+
+def bubble_ck2n54_bars(oracle):
+    """the oracle's fixed-step CarpenterKennedy2N54 run (CFL 0.25, the
+    integrator of this build) and the bars of its regression values: three
+    times the measured gap between that run and the restated reference
+    integrator, plus 1e-6 relative; returns (u, steps, l2 bar, linf bar)"""
+    B, mesh, ro, u0 = _bubble_oracle(oracle)
+    u_ck, steps = ro.solve(u0, B.T_END, B.CFL)
+    u_ck = u_ck.reshape(u0.shape, order="F")
+    u_ss, _ = B.ssprk43_adaptive(lambda v, t: ro.rhs(v, t), lambda v: ro.newton(v)[0],
+                                 ro._u(u0), B.T_END)
+    u_ss = u_ss.reshape(u0.shape, order="F")
+    l2c, lic = calc_error_norms_2d(u_ck, mesh, B.exact, B.T_END)
+    l2s, lis = calc_error_norms_2d(u_ss, mesh, B.exact, B.T_END)
+    bar_l2 = 3 * np.abs(l2c - l2s) + 1e-6 * np.abs(l2s)
+    bar_linf = 3 * np.abs(lic - lis) + 1e-6 * np.abs(lis)
+    return u_ck, steps, bar_l2, bar_linf
+
+
+def test_bubble_regression_fixed_steps(oracle):
+    """the oracle with this build's integrator (fixed-step CK2N54 at CFL
+    0.25): the regression values within the derived integrator gap"""
+    import rainy_bubble_pin as B
+    u, steps, bar_l2, bar_linf = bubble_ck2n54_bars(oracle)
+    l2, linf = calc_error_norms_2d(u, B.bubble_mesh(), B.exact, B.T_END)
+    print("CK2N54 steps", steps, "\nl2 diff/bar", np.abs(l2 - B.BUBBLE_L2) / bar_l2,
+          "\nlinf diff/bar", np.abs(linf - B.BUBBLE_LINF) / bar_linf)
+    assert np.all(np.abs(l2 - B.BUBBLE_L2) <= bar_l2)
+    assert np.all(np.abs(linf - B.BUBBLE_LINF) <= bar_linf)

@@ -467,3 +467,135 @@ def initial_condition_rainy_bubble(mesh, equations, seed=0, theta=300.0, radius=
         rv * eq.ref_latent_heat_vap_temp
     u = np.stack([rd, rv, zero, zero, zero, E, rv, zero, T])
     return u, {"relative_humidity": rh, "centre": (xc, zc)}
+
+
+# ---------------------------------------------------------------------------
+# rainy bubble of examples/euler/precipitation/buoyancy/elixir_hoeck_bubble.jl
+
+def rainy_theta_d(z, equations):
+    """hydrostatic dry potential temperature of the rainy bubble
+    (src/equations/compressible_rainy_euler_2d.jl:1066-1083): 283 K at the
+    surface pressure 85000 Pa, stratification 1.3e-5 / m"""
+    eq = equations
+    theta0 = 283.0 * (eq.ref_pressure / 85000.0) ** (eq.R_dry_air / eq.c_dry_air_const_pressure)
+    return theta0 * np.exp(1.3e-5 * np.asarray(z, dtype=float))
+
+
+def _svp_and_derivative(T, eq):
+    """saturation vapour pressure and its temperature derivative
+    (compressible_rainy_euler_2d.jl:397-446)"""
+    k1 = (eq.c_vapour_const_pressure - eq.c_liquid_water) / eq.R_vapour
+    k2 = (eq.ref_latent_heat_vap_temp -
+          (eq.c_vapour_const_pressure - eq.c_liquid_water) * eq.ref_temperature) / eq.R_vapour
+    es = eq.ref_saturation_pressure * (T / eq.ref_temperature) ** k1 * \
+        np.exp(k2 * (1.0 / eq.ref_temperature - 1.0 / T))
+    return es, es * (k1 / T + k2 / (T * T))
+
+
+class AtmosphereLayersRainyBubble:
+    """`AtmosphereLayersRainyBubble(equations; total_height, precision = 1)`
+    (src/equations/compressible_rainy_euler_2d.jl:1086-1131): hydrostatic
+    layers of (pressure, rho_dry, rho_vapour, T) at relative humidity 0.2,
+    one every `precision` metres, each the root of the four residuals of
+    generate_hydrostatic_residual (:997-1031) given the layer below: the
+    first at z = 0.01 above the surface pressure 85000 Pa, then up in steps
+    of `precision`.  The reference finds the roots with NLsolve to a residual
+    tolerance (1e-8, then 1e-10); here Newton's method with the analytic
+    Jacobian goes to the root (the two agree to ~1e-15 relative in the
+    pressure).  Rows keep the reference's indexing: row n (0-based n - 1 in
+    its 1-based table) is used as the layer at height (n - 1) precision."""
+
+    def __init__(self, equations, total_height, precision=1.0):
+        eq = equations
+        self.total_height = float(total_height)
+        self.precision = float(precision)
+        n = int(round(self.total_height / self.precision))
+        data = np.zeros((n + 1, 4))
+        data[0] = self._solve(85000.0, 0.01, 0.01, np.array([85000.0, 1.4, 0.04, 300.0]), eq)
+        z, dz = 0.01, self.precision
+        for i in range(n):
+            z += dz
+            data[i + 1] = self._solve(data[i, 0], z, dz, data[i].copy(), eq)
+        self.layer_data = data
+
+    @staticmethod
+    def _solve(p_lower, z, dz, x, eq):
+        R_d, R_v, eps = eq.R_dry_air, eq.R_vapour, eq.R_dry_air / eq.R_vapour
+        kappa = R_d / eq.c_dry_air_const_pressure
+        H, g = 0.2, eq.gravity
+        th = rainy_theta_d(z, eq)
+        for _ in range(60):
+            p, rd, rv, T = x
+            es, des = _svp_and_derivative(T, eq)
+            rvs = es / (R_v * T)
+            drvs = (des * T - es) / (R_v * T * T)
+            pk = (eq.ref_pressure / p) ** kappa
+            r = np.array([(p - p_lower) / dz + (rd + rv) * g,
+                          p - T * (rd * R_d + rv * R_v),
+                          th - T * pk,
+                          1000.0 * (rvs * (rd + rv / eps) * H - rv * (rd + rvs / eps))])
+            J = np.array([[1.0 / dz, g, g, 0.0],
+                          [1.0, -T * R_d, -T * R_v, -(rd * R_d + rv * R_v)],
+                          [T * kappa * pk / p, 0.0, 0.0, -pk],
+                          [0.0, 1000.0 * (rvs * H - rv), 1000.0 * (rvs * H / eps - rd - rvs / eps),
+                           1000.0 * (drvs * (rd + rv / eps) * H - rv * drvs / eps)]])
+            step = np.linalg.solve(J, r)
+            x = x - step
+            if np.all(np.abs(step) <= 4e-16 * np.abs(x)):
+                break
+        return x
+
+
+def initial_condition_bubble_rainy(x, t, equations, atmosphere_layers):
+    """`initial_condition_bubble_rainy` (src/equations/compressible_rainy_euler_2d.jl:1133-1206)
+    at points x [2, ...]: relative humidity 0.2 outside a bubble centred at
+    (1200, 800) m, 1 inside radius 200 m with a cos^2 transition to 300 m;
+    the layer pressure interpolated linearly in height; (rho_dry,
+    rho_vapour, T) the root of generate_perturbation_residual (:1033-1064)
+    from the layer's values (Newton's method to the root; the reference's
+    NLsolve stops at a residual of 1e-9); at rest, no cloud or rain."""
+    eq = equations
+    R_d, R_v, eps = eq.R_dry_air, eq.R_vapour, eq.R_dry_air / eq.R_vapour
+    kappa = R_d / eq.c_dry_air_const_pressure
+    xs = np.asarray(x[0], dtype=float)
+    z = np.asarray(x[1], dtype=float)
+    r = np.sqrt((xs - 1200.0) ** 2 + (z - 800.0) ** 2)
+    H = np.where(r > 300.0, 0.2,
+                 np.where(r > 200.0,
+                          0.2 + 0.8 * np.cos(np.pi * (r - 200.0) / 200.0) ** 2, 1.0))
+    L = atmosphere_layers
+    dz = L.precision
+    n = np.floor((z + np.finfo(float).eps) / dz).astype(np.int64) + 1  # 1-based, as the reference
+    z_lower = (n - 1) * dz
+    z_upper = n * dz
+    top = z_lower == L.total_height
+    z_upper = np.where(top, z_lower + dz, z_upper)
+    n = np.where(top, n - 1, n)
+    n = np.where(n == 0, 1, n)
+    if np.any((z > L.total_height) & ~np.isclose(z, L.total_height)):
+        raise ValueError("The atmosphere does not match the simulation domain")
+    data = L.layer_data
+    p_lo, p_up = data[n - 1, 0], data[n, 0]
+    ph = (p_up * (z - z_lower) + p_lo * (z_upper - z)) / dz
+    # the root: T from the potential-temperature residual, then Newton on
+    # (rho_dry, rho_vapour) for the humidity and pressure residuals
+    T = rainy_theta_d(z, eq) / (eq.ref_pressure / ph) ** kappa
+    es, _ = _svp_and_derivative(T, eq)
+    rvs = es / (R_v * T)
+    rd, rv = data[n - 1, 1].copy(), data[n - 1, 2].copy()
+    for _ in range(60):
+        f1 = 30.0 * (rvs * (rd + rv / eps) * H - rv * (rd + rvs / eps))
+        f2 = ph - (rd * R_d + rv * R_v) * T
+        a11 = 30.0 * (rvs * H - rv)
+        a12 = 30.0 * (rvs * H / eps - rd - rvs / eps)
+        a21, a22 = -R_d * T, -R_v * T
+        det = a11 * a22 - a12 * a21
+        d_rd = (f1 * a22 - a12 * f2) / det
+        d_rv = (a11 * f2 - a21 * f1) / det
+        rd, rv = rd - d_rd, rv - d_rv
+        if np.all(np.abs(d_rd) <= 4e-16 * np.abs(rd)) and np.all(np.abs(d_rv) <= 4e-16 * np.abs(rv)):
+            break
+    E = (eq.c_dry_air_const_volume * rd + eq.c_vapour_const_volume * rv) * T + \
+        rv * eq.ref_latent_heat_vap_temp
+    zero = np.zeros_like(rd)
+    return np.stack([rd, rv, zero, zero, zero, E, rv, zero, T])
