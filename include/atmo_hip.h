@@ -403,6 +403,29 @@ int atmo_max_dt(atmo_handle* handle, double cfl, double* dt);
 int atmo_solve(atmo_handle* handle, double t_end, double cfl, int32_t check_every,
                int64_t* steps, double* t_reached);
 
+/* Allocation-free max_dt for the 3D Euler equations.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight time loops, harnesses that forbid hipMalloc).  It
+ * computes what atmo_max_dt computes on a handle -- dt = cfl * 2 / (n * max
+ * over elements of the summed maximum scaled speeds) -- for a state u
+ * [6,n,n,n,E], contravariant vectors [3,3,n,n,n,E] and inverse Jacobian
+ * [n,n,n,E] in device memory owned by the caller (config: equations,
+ * polydeg, n_elements, c_p, c_v; the other fields are not read), and writes
+ * dt to device memory (*dt_device; NaN when a wave speed is not finite,
+ * where atmo_max_dt returns ATMO_ERR_NONFINITE).  Its kernels keep their
+ * per-element partials in a fixed scratch array of the library (one per GPU,
+ * held until the process ends), so n_elements is at most
+ * ATMO_NOALLOC_MAX_ELEMENTS (ATMO_ERR_INVALID beyond).  It calls no hipMalloc
+ * or hipFree, enqueues its kernels on `stream` (a hipStream_t; NULL: the
+ * null stream) and returns without waiting for them.  Calls on one GPU share
+ * that GPU's scratch array: calls that overlap on one GPU race, so the
+ * caller serialises them. */
+#define ATMO_NOALLOC_MAX_ELEMENTS 262144
+int atmo_max_dt3d_noalloc(const atmo_config* config, const double* u_device,
+                          const double* contravariant_vectors_device,
+                          const double* inverse_jacobian_device, double cfl,
+                          double* dt_device, void* stream);
+
 /* Integrals on the resident state (analysis_covariant.jl:53-105 analogues):
  * out[0] mass, out[1] entropy, out[2] dS/dt = int w(u).du of the last
  * computed rhs (or of a fresh one when `fresh_rhs`), out[3..] integral of each

@@ -123,8 +123,11 @@ __global__ __launch_bounds__(256) void reduce_columns(const double* __restrict__
 template <int N>
 static void launch_red(const E3Launch& L, int which, const double* du, double* partial,
                        hipStream_t s) {
-  WeightTable wt;
-  for (int q = 0; q < N; ++q) wt.w[q] = L.weights[q];
+  // the quadrature weights enter the integrals only (the max-speed reduction
+  // may be launched without them)
+  WeightTable wt{};
+  if (which != 0)
+    for (int q = 0; q < N; ++q) wt.w[q] = L.weights[q];
   if (which == 0)
     hipLaunchKernelGGL((e3_max_speed<N>), dim3(L.n_elements), dim3(Shape<N>::BLOCK), 0, s, L.q,
                        L.u, L.cv, L.invJ, partial);

@@ -169,6 +169,9 @@ SIGNATURES = [
     ("atmo_get_state", ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     ("atmo_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]),
     ("atmo_max_dt", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, c_double_p]),
+    ("atmo_max_dt3d_noalloc", ctypes.c_int, [ctypes.POINTER(AtmoConfig), ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                             ctypes.c_void_p, ctypes.c_void_p]),
     ("atmo_solution_variables", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int32, c_double_p, c_int32_p]),
     ("atmo_metric_terms3d", ctypes.c_int,
