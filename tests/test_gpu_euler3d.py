@@ -193,3 +193,39 @@ def test_mixed_precision_rhs_error(oracle, precision, case):
             assert rel[v] <= MIXED_RTOL, (v, rel)
         else:  # round-off-only tendency (state at rest)
             assert diff[v] <= 1e-10 * umax[v], (v, diff, umax)
+
+
+@pytest.mark.parametrize("case", ["shell_p4", "cube_odd_p4", "cube_odd_p3"])
+def test_autotuner_shapes_are_bit_identical(case):
+    """the 3D stage's workgroup shapes (one or two elements per workgroup,
+    chosen by the autotuner) give the same bits: rhs!, RK steps replayed from
+    the captured graph, and the resident loop; an odd element count leaves the
+    last two-element workgroup half filled"""
+    if case == "shell_p4":
+        mesh, eq, solver, bcs, u0 = T.workloads.shell(3, 2, polydeg=4, seed=1)
+        u0 = u0.copy()
+        u0[1:4] += 10.0 * np.sin(mesh.node_coordinates / 1e6) * u0[0]
+    else:
+        p = 4 if case.endswith("p4") else 3
+        mesh, eq, solver, bcs, u0 = T.workloads.periodic_cube(trees=3, polydeg=p, seed=2)
+    print(f"case: {case}, E={mesh.n_elements}", flush=True)
+    semi = T.SemidiscretizationHyperbolic(mesh, eq, solver, bcs)
+    semi.set_state(u0)
+    chosen, ms = semi.autotune()
+    print("autotune", chosen, ms[:2], flush=True)
+    assert chosen in (0, 1) and ms[0] > 0 and ms[1] > 0
+    dt = semi.max_dt(0.5)
+    rhs, steps, solved = [], [], []
+    for v in (0, 1):
+        semi.set_kernel_variant(v)
+        rhs.append(semi.rhs(None, u0))
+        semi.set_state(u0)
+        semi.step(dt, 4)
+        steps.append(semi.get_state())
+        semi.set_state(u0)
+        semi.set_time(0.0)
+        semi.solve_resident(3.5 * dt, 0.5, 2)
+        solved.append(semi.get_state())
+    np.testing.assert_array_equal(rhs[0], rhs[1])
+    np.testing.assert_array_equal(steps[0], steps[1])
+    np.testing.assert_array_equal(solved[0], solved[1])

@@ -125,7 +125,15 @@ E3Launch base_launch(atmo_handle* h) {
   L.bg.p0 = h->cfg.background_p0;
   L.bg.inv_RT0 = h->bg_on ? 1.0 / ((h->cfg.c_p - h->cfg.c_v) * h->cfg.background_T0) : 0.0;
   L.bg.B = h->d_B;
+  L.epb = h->e3_epb;
   return L;
+}
+
+bool launch_stage_only(atmo_handle* h, const E3Launch& L) {
+  const int prec = h->cfg.precision;
+  return prec == ATMO_PREC_FP64         ? e3_launch_stage_fp64(L, h->stream)
+         : prec == ATMO_PREC_FP32_KAHAN ? e3_launch_stage_fp32_kahan(L, h->stream)
+                                        : e3_launch_stage_fp32(L, h->stream);
 }
 
 int launch_stage(atmo_handle* h, const E3Launch& L) {
@@ -195,7 +203,7 @@ int launch_rk_step(atmo_handle* h, E3Launch& L, double* speed_partial = nullptr)
 // two RK steps (ten stage launches, after which the ping-pong buffers are
 // back in place) captured once per state buffer, dt read from d_step_dt
 int ensure_e3_graph(atmo_handle* h) {
-  if (h->e3_graph && h->e3_graph_u == h->d_u) return ATMO_OK;
+  if (h->e3_graph && h->e3_graph_u == h->d_u && h->e3_graph_epb == h->e3_epb) return ATMO_OK;
   if (h->e3_graph) {
     HIP_TRY(h, hipGraphExecDestroy(h->e3_graph));
     h->e3_graph = nullptr;
@@ -220,8 +228,66 @@ int ensure_e3_graph(atmo_handle* h) {
   (void)hipGraphDestroy(g);
   HIP_TRY(h, ei);
   h->e3_graph_u = h->d_u;
+  h->e3_graph_epb = h->e3_epb;
   return ATMO_OK;
 }
+
+}  // namespace
+
+namespace atmo {
+
+// Autotuner of the 3D stage (no counterpart in the reference): where the
+// stage has a two-element workgroup shape, times one and two elements per
+// workgroup on the RK stage (the launch the time loops make) and keeps the
+// faster.  The resident state is read; the scratch buffers d_u2 (the stage
+// output) and d_du (as the register) are written.  Every shape gives the same
+// bits (each element's arithmetic does not depend on its workgroup).
+int e3_autotune(atmo_handle* h, double* ms_per_variant) {
+  if (ms_per_variant) ms_per_variant[0] = ms_per_variant[1] = 0.0;
+  if (!e3_has_two_element_groups(h->n, h->cfg.precision)) {
+    h->e3_epb = 1;
+    h->e3_tuned = true;
+    return ATMO_OK;
+  }
+  hipEvent_t a, b;
+  HIP_TRY(h, hipEventCreate(&a));
+  HIP_TRY(h, hipEventCreate(&b));
+  E3Launch L = base_launch(h);
+  L.mode = 1;
+  L.u = h->d_u;
+  L.out = h->d_u2;
+  L.k = h->d_du;
+  L.rk_a = L.rk_dt = L.rk_b = 0.0;
+  float best_of[2] = {1e30f, 1e30f};
+  for (int round = 0; round < 3; ++round)
+    for (int v = 0; v < 2; ++v) {
+      L.epb = v + 1;
+      if (!launch_stage_only(h, L)) continue;
+      HIP_TRY(h, hipEventRecord(a, h->stream));
+      for (int r = 0; r < 5; ++r) launch_stage_only(h, L);
+      HIP_TRY(h, hipEventRecord(b, h->stream));
+      HIP_TRY(h, hipEventSynchronize(b));
+      float ms = 0.0f;
+      HIP_TRY(h, hipEventElapsedTime(&ms, a, b));
+      ms /= 5.0f;
+      if (ms < best_of[v]) best_of[v] = ms;
+    }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  HIP_TRY(h, hipGetLastError());
+  h->e3_epb = best_of[1] < best_of[0] ? 2 : 1;
+  if (ms_per_variant) {
+    ms_per_variant[0] = best_of[0];
+    ms_per_variant[1] = best_of[1];
+  }
+  h->e3_tuned = true;
+  h->du_valid = false;
+  return ATMO_OK;
+}
+
+}  // namespace atmo
+
+namespace {
 
 }  // namespace
 
@@ -517,7 +583,8 @@ int atmo_create(const atmo_config* cfg, const atmo_geometry3d* g, atmo_handle** 
           hipSuccess ||
       hipMemcpy(h->d_face_info, face_info.data(), sizeof(int32_t) * face_info.size(),
                 hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(h->d_u, 0, sb) != hipSuccess || hipMemset(h->d_k, 0, sb) != hipSuccess) {
+      hipMemset(h->d_u, 0, sb) != hipSuccess || hipMemset(h->d_u2, 0, sb) != hipSuccess ||
+      hipMemset(h->d_k, 0, sb) != hipSuccess) {
     fail(nullptr, ATMO_ERR_DEVICE, "device upload failed");
     return cleanup_fail(ATMO_ERR_DEVICE);
   }
@@ -555,6 +622,7 @@ void atmo_destroy(atmo_handle* h) {
     if (p) hipFree(p);
   if (h->d_geo32) hipFree(h->d_geo32);
   if (h->d_fxf32) hipFree(h->d_fxf32);
+  if (h->d_hs64) hipFree(h->d_hs64);
   if (h->d_clock) hipFree(h->d_clock);
   if (h->d_step_dt) hipFree(h->d_step_dt);
   if (h->e3_graph) hipGraphExecDestroy(h->e3_graph);
@@ -601,6 +669,10 @@ int atmo_rhs_device(atmo_handle* h, double* du, const double* u, double t) {
 int atmo_set_state(atmo_handle* h, const double* u) {
   if (!h || !u) return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "null argument");
   HIP_TRY(h, hipMemcpyAsync(h->d_u, u, sizeof(double) * h->len, hipMemcpyHostToDevice, h->stream));
+  // the RK stages of the 3D form never write phi: both state buffers hold it
+  if (h->eq == ATMO_EQ_EULER_ENERGY_GRAVITY_3D)
+    HIP_TRY(h, hipMemcpyAsync(h->d_u2, h->d_u, sizeof(double) * h->len, hipMemcpyDeviceToDevice,
+                              h->stream));
   HIP_TRY(h, hipMemsetAsync(h->d_k, 0, sizeof(double) * h->len, h->stream));
   int rc = compute_background(h, h->d_u);
   if (rc != ATMO_OK) return rc;
@@ -625,6 +697,10 @@ int atmo_step(atmo_handle* h, double dt, int32_t n_steps) {
     if (rc != ATMO_OK) return rc;
     for (int s = 0; s < n_steps; ++s) h->time += dt;
     return ATMO_OK;
+  }
+  if (!h->e3_tuned) {
+    const int rc = e3_autotune(h, nullptr);
+    if (rc != ATMO_OK) return rc;
   }
   E3Launch L = base_launch(h);
   L.mode = 1;
@@ -676,7 +752,7 @@ int atmo_max_dt(atmo_handle* h, double cfl, double* dt) {
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   if (!std::isfinite(mx)) return fail(&h->err, ATMO_ERR_NONFINITE, "non-finite wave speed in state");
   if (mx < 5e-324) mx = 5e-324;
-  *dt = cfl * 2.0 / (h->n * mx);
+  *dt = cfl * (2.0 / (h->n * mx));  // max_dt (stepsize_dg2d.jl:43), then StepsizeCallback's cfl factor
   return ATMO_OK;
 }
 

@@ -44,7 +44,12 @@ struct E3Launch {
   ResidentLink res;      // resident time loop: device step size, speeds of the new state
   double coriolis;  // 2 Omega, 0 = no source
   Background bg;
+  int epb;          // elements per workgroup of the direct stage form: 1, or 2 where built
 };
+
+// whether the stage kernel of this polydeg / precision has a two-element
+// workgroup shape (the 3D autotuner's choice)
+bool e3_has_two_element_groups(int n, int precision);
 
 // background state from the geopotential of u: (rho_bg, 0, 0, 0, p_bg/(gamma-1), phi)
 void e3_launch_background_state(const E3Launch& L, const double* u, double* ubg, hipStream_t s);

@@ -79,6 +79,20 @@ __device__ __forceinline__ T ln_mean_logs_branch(T x, T y, T lx, T ly) {
   return div_(d, ly - lx);
 }
 
+// ln_mean for the two states of a face node: they describe one physical
+// point, so the series branch is the common one and the logarithms are taken
+// only in the other branch (log y - log x there, which differs from log(y/x)
+// by rounding only)
+template <typename T>
+__device__ __forceinline__ T ln_mean_face(T x, T y) {
+  const T d = y - x, s = x + y;
+  if (d * d < T(1.0e-4) * (s * s)) {
+    const T f2 = (d * d) * rcp_refined(s * s);
+    return div_(s, T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0))));
+  }
+  return div_(d, log_(y) - log_(x));
+}
+
 template <typename T>
 __device__ __forceinline__ T inv_ln_mean_logs(T x, T y, T lx_minus_ly) {
   const T d = y - x, s = x + y;

@@ -277,69 +277,89 @@ __device__ __forceinline__ void surface_momentum_split(const E3Params& q, int su
 // the neighbour's state) overlap the volume phase; from polydeg 4 on only the
 // first pass (every lane of the workgroup) is: polydeg 4's partial second
 // pass then loads in place, which keeps 14 registers free during the volume
-// phase (5 % faster there; polydeg 3, one wave per element, prefetches both)
-template <int N>
+// phase (5 % faster there; polydeg 3, one wave per element, prefetches both).
+// A workgroup holds EPB elements (slots es = 0..EPB-1 of Shape<N>::BLOCK
+// lanes each); its EPB * 6 n^2 face nodes are spread over all its lanes,
+// face node fg = t + ps * EPB * BLOCK of pass ps being node fg mod 6 n^2 of
+// slot fg / (6 n^2).
+template <int N, int EPB = 1>
 struct FacePrefetch {
-  static constexpr int PASSES = (6 * Shape<N>::N2 + Shape<N>::BLOCK - 1) / Shape<N>::BLOCK;
+  static constexpr int GB = EPB * Shape<N>::BLOCK;
+  static constexpr int NF = 6 * Shape<N>::N2;
+  static constexpr int PASSES = (EPB * NF + GB - 1) / GB;
   static constexpr int PF = N >= 5 ? 1 : PASSES;
   int link[PF], info[PF];
   double ur[PF][6];
 };
 
-// link (neighbour's global node, < 0 on a boundary) and face_info of pass ps
-template <int N>
-__device__ __forceinline__ void face_link(const FacePrefetch<N>& pf, int e, int t, int ps,
-                                          const int* __restrict__ nbr,
+// link (neighbour's global node, < 0 on a boundary) and face_info of pass ps;
+// elements e0 .. e0 + n_valid - 1 are the workgroup's
+template <int N, int EPB = 1>
+__device__ __forceinline__ void face_link(const FacePrefetch<N, EPB>& pf, int e0, int n_valid,
+                                          int t, int ps, const int* __restrict__ nbr,
                                           const int* __restrict__ face_info, int& link,
                                           int& info) {
-  constexpr int N2 = Shape<N>::N2, NF = 6 * N2;
-  if (ps < FacePrefetch<N>::PF) {
-    link = pf.link[ps];
-    info = pf.info[ps];
+  using F = FacePrefetch<N, EPB>;
+  constexpr int N2 = Shape<N>::N2, NF = F::NF;
+  if (ps < F::PF) {
+    link = pf.link[ps < F::PF ? ps : 0];
+    info = pf.info[ps < F::PF ? ps : 0];
     return;
   }
-  const int fid = t + ps * Shape<N>::BLOCK;
+  const int fg = t + ps * F::GB;
   link = -1;
   info = 0;
-  if (fid < NF) {
+  const int es = EPB == 1 ? 0 : fg / NF;
+  if (fg < EPB * NF && es < n_valid) {
+    const int fid = fg - es * NF;
     const int f = fid / N2, ab = fid - f * N2;
-    link = nbr[((size_t)e * 6 + f) * N2 + ab];
-    info = face_info[(size_t)e * 6 + f];
+    const size_t e = (size_t)(e0 + es);
+    link = nbr[(e * 6 + f) * N2 + ab];
+    info = face_info[e * 6 + f];
   }
 }
 
-template <int N>
-__device__ __forceinline__ void face_prefetch(FacePrefetch<N>& pf, int e, int t,
+template <int N, int EPB = 1>
+__device__ __forceinline__ void face_prefetch(FacePrefetch<N, EPB>& pf, int e0, int n_valid, int t,
                                               const double* __restrict__ u,
                                               const int* __restrict__ nbr,
                                               const int* __restrict__ face_info) {
-  constexpr int N2 = Shape<N>::N2, NF = 6 * N2;
+  using F = FacePrefetch<N, EPB>;
+  constexpr int N2 = Shape<N>::N2, NF = F::NF;
 #pragma unroll
-  for (int ps = 0; ps < FacePrefetch<N>::PF; ++ps) {
-    const int fid = t + ps * Shape<N>::BLOCK;
+  for (int ps = 0; ps < F::PF; ++ps) {
+    const int fg = t + ps * F::GB;
+    const int es = EPB == 1 ? 0 : fg / NF;
     pf.link[ps] = -1;
     pf.info[ps] = 0;
-    if (fid < NF) {
+    if (fg < EPB * NF && es < n_valid) {
+      const int fid = fg - es * NF;
       const int f = fid / N2, ab = fid - f * N2;
-      pf.link[ps] = nbr[((size_t)e * 6 + f) * N2 + ab];
-      pf.info[ps] = face_info[(size_t)e * 6 + f];
+      const size_t e = (size_t)(e0 + es);
+      pf.link[ps] = nbr[(e * 6 + f) * N2 + ab];
+      pf.info[ps] = face_info[e * 6 + f];
     }
   }
 #pragma unroll
-  for (int ps = 0; ps < FacePrefetch<N>::PF; ++ps) {
+  for (int ps = 0; ps < F::PF; ++ps) {
     const int link = pf.link[ps];
 #pragma unroll
     for (int v = 0; v < 6; ++v) pf.ur[ps][v] = link >= 0 ? u[(size_t)link * 6 + v] : 0.0;
   }
 }
 
-// face-node geometry of pass ps: face f, direction d, own node; false when the
-// lane has no face node in that pass
-template <int N>
-__device__ __forceinline__ bool face_node_of(int t, int ps, int& fid, int& f, int& d, int& node) {
-  constexpr int N2 = Shape<N>::N2, NF = 6 * N2;
-  fid = t + ps * Shape<N>::BLOCK;
-  if (fid >= NF) return false;
+// face-node geometry of pass ps: element slot es, face node fid of that
+// element, face f, direction d, own node; false when the lane has no face
+// node in that pass
+template <int N, int EPB = 1>
+__device__ __forceinline__ bool face_node_of(int t, int ps, int n_valid, int& es, int& fid, int& f,
+                                             int& d, int& node) {
+  using F = FacePrefetch<N, EPB>;
+  constexpr int N2 = Shape<N>::N2, NF = F::NF;
+  const int fg = t + ps * F::GB;
+  es = EPB == 1 ? 0 : fg / NF;
+  if (fg >= EPB * NF || es >= n_valid) return false;
+  fid = fg - es * NF;
   f = fid / N2;
   const int ab = fid - f * N2;
   const int a = ab % N, b = ab / N;
@@ -352,35 +372,36 @@ __device__ __forceinline__ bool face_node_of(int t, int ps, int& fid, int& f, in
 // normal of every face pass: the own outward contravariant vector on a primary
 // or boundary face, minus the primary's on a secondary face (Trixi evaluates
 // an interface with the primary element's normal).  Read before the face
-// fluxes overwrite the LDS copy of the own vectors (s_cv = [9][n^3], or null
-// to read them from memory).
-template <int N, typename T>
+// fluxes overwrite the LDS copy of the own vectors (s_cv = [9][n^3] per slot,
+// slot stride cv_stride, or null to read them from memory).
+template <int N, typename T, int EPB = 1>
 struct FaceNormals {
-  T n[FacePrefetch<N>::PASSES][3];
+  T n[FacePrefetch<N, EPB>::PASSES][3];
 };
 
-template <int N, typename T>
-__device__ __forceinline__ void face_normals(FaceNormals<N, T>& fn, int e, int t,
+template <int N, typename T, int EPB = 1>
+__device__ __forceinline__ void face_normals(FaceNormals<N, T, EPB>& fn, int e0, int n_valid, int t,
                                              const T* __restrict__ cv, const T* s_cv,
-                                             const FacePrefetch<N>& pf,
+                                             int cv_stride, const FacePrefetch<N, EPB>& pf,
                                              const int* __restrict__ nbr,
                                              const int* __restrict__ face_info) {
   constexpr int N3 = Shape<N>::N3;
 #pragma unroll
-  for (int ps = 0; ps < FacePrefetch<N>::PASSES; ++ps) {
-    int fid, f, d, node;
+  for (int ps = 0; ps < FacePrefetch<N, EPB>::PASSES; ++ps) {
+    int es, fid, f, d, node;
     fn.n[ps][0] = fn.n[ps][1] = fn.n[ps][2] = T(0);
-    if (!face_node_of<N>(t, ps, fid, f, d, node)) continue;
+    if (!face_node_of<N, EPB>(t, ps, n_valid, es, fid, f, d, node)) continue;
     // face_info: 0 = primary side or boundary; 8 | dp = secondary side of an
     // interface whose primary face direction is dp
     int link, info;
-    face_link<N>(pf, e, t, ps, nbr, face_info, link, info);
+    face_link<N, EPB>(pf, e0, n_valid, t, ps, nbr, face_info, link, info);
     if (info == 0) {
       const T sgn = (f % 2 == 0) ? T(-1) : T(1);
+      const size_t e = (size_t)(e0 + es);
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        fn.n[ps][c] = sgn * (s_cv ? s_cv[(3 * d + c) * N3 + node]
-                                  : cv[((size_t)e * N3 + node) * 9 + 3 * d + c]);
+        fn.n[ps][c] = sgn * (s_cv ? s_cv[es * cv_stride + (3 * d + c) * N3 + node]
+                                  : cv[(e * N3 + node) * 9 + 3 * d + c]);
     } else {
       const int dp = info & 7;
       const T sgn = (dp % 2 == 0) ? T(1) : T(-1);  // minus the primary's outward normal
@@ -391,17 +412,25 @@ __device__ __forceinline__ void face_normals(FaceNormals<N, T>& fn, int e, int t
   }
 }
 
-template <int N, typename T>
+// the surface fluxes of the workgroup's face nodes, lifted by 1/w_end, into
+// s_sf[es * sf_stride + v * 6 n^2 + fid]; the own primitives come from the
+// slot's node fields s_w[es * w_stride + ...] (and s_pb[es * N3 + ...])
+template <int N, typename T, int EPB = 1>
 __device__ __forceinline__ void face_fluxes_to_lds(
-    const E3Params& q, int e, int t, double inv_w, const double* __restrict__ u,
-    const FacePrefetch<N>& pf, const FaceNormals<N, T>& fn, const int* __restrict__ nbr,
-    const int* __restrict__ face_info, int surf, int snc, const Background& bg, const T* s_w,
-    const double* s_pb, T* s_sf) {
+    const E3Params& q, int e0, int n_valid, int t, double inv_w, const double* __restrict__ u,
+    const FacePrefetch<N, EPB>& pf, const FaceNormals<N, T, EPB>& fn,
+    const int* __restrict__ nbr, const int* __restrict__ face_info, int surf, int snc,
+    const Background& bg, const T* s_w_all, int w_stride, const double* s_pb_all,
+    T* s_sf_all, int sf_stride) {
   constexpr int N2 = Shape<N>::N2, N3 = Shape<N>::N3, NF = 6 * N2;
 #pragma unroll
-  for (int ps = 0; ps < FacePrefetch<N>::PASSES; ++ps) {
-    int fid, f, d, node;
-    if (!face_node_of<N>(t, ps, fid, f, d, node)) continue;
+  for (int ps = 0; ps < FacePrefetch<N, EPB>::PASSES; ++ps) {
+    int es, fid, f, d, node;
+    if (!face_node_of<N, EPB>(t, ps, n_valid, es, fid, f, d, node)) continue;
+    const T* s_w = s_w_all + es * w_stride;
+    const double* s_pb = s_pb_all + es * N3;
+    T* s_sf = s_sf_all + es * sf_stride;
+    const int e = e0 + es;
     double uo[6];
     if constexpr (sizeof(T) == 8) {
       // the own state from the primitives in LDS (rounding-level differences
@@ -420,14 +449,14 @@ __device__ __forceinline__ void face_fluxes_to_lds(
       for (int v = 0; v < 6; ++v) uo[v] = u[gn * 6 + v];
     }
     int link, info;
-    face_link<N>(pf, e, t, ps, nbr, face_info, link, info);
+    face_link<N, EPB>(pf, e0, n_valid, t, ps, nbr, face_info, link, info);
     const T n0 = fn.n[ps][0], n1 = fn.n[ps][1], n2 = fn.n[ps][2];
     double ur[6];
     if (link >= 0) {
 #pragma unroll
       for (int v = 0; v < 6; ++v)
-        ur[v] = ps < FacePrefetch<N>::PF ? pf.ur[ps < FacePrefetch<N>::PF ? ps : 0][v]
-                                         : u[(size_t)link * 6 + v];
+        ur[v] = ps < FacePrefetch<N, EPB>::PF ? pf.ur[ps < FacePrefetch<N, EPB>::PF ? ps : 0][v]
+                                              : u[(size_t)link * 6 + v];
     } else {
       // boundary_condition_slip_wall (:93-116): mirror the normal momentum
       const double m0 = n0, m1 = n1, m2 = n2;
@@ -473,7 +502,7 @@ __device__ __forceinline__ void face_fluxes_to_lds(
     if (snc != 0) {
       // gravity term with the phi jump formed in FP64
       const T dphi = T(ur[5] - uo[5]);
-      const T ra = snc == 1 ? ln_mean(own.rho, pr.rho) : T(0.5) * (own.rho + pr.rho);
+      const T ra = snc == 1 ? ln_mean_face(own.rho, pr.rho) : T(0.5) * (own.rho + pr.rho);
       const T sj = ra * dphi;
       T sm = sj;
       if (bg.on) {
@@ -605,12 +634,16 @@ __device__ __forceinline__ void stage_epilogue(
   // calc_sources!: source_terms_coriolis (-2 Omega e_z x rho v), after the Jacobian
   du[1] += coriolis * uo[2];
   du[2] += -coriolis * uo[1];
+  // phi has no tendency: in an RK stage its register stays 0 and its state
+  // entry is the same in both state buffers (copied when the state is set),
+  // so the update leaves both untouched
+  un[NV - 1] = uo[NV - 1];
   if (mode == 0) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) out[gnode * NV + v] = du[v];
   } else {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
+    for (int v = 0; v < NV - 1; ++v) {
       const double kn = rk_a * kreg[gnode * NV + v] + rk_dt * du[v];
       kreg[gnode * NV + v] = kn;
       un[v] = uo[v] + rk_b * kn;
@@ -621,13 +654,15 @@ __device__ __forceinline__ void stage_epilogue(
 
 // per element: the sum over directions of the maximum scaled speed of the
 // new states un (the resident loop's next step size), reduced over the
-// workgroup with speed_max (exact, so any order gives the max-speed
-// reduction's value); every thread of the workgroup takes part
-template <int B>
+// element's slot of BE lanes with speed_max (exact, so any order gives the
+// max-speed reduction's value); every thread of the workgroup takes part
+template <int BE, int EPB = 1>
 __device__ __forceinline__ void element_speed(const E3Params& q, bool active, const double* un,
                                               const double* __restrict__ cv64,
                                               const double* __restrict__ invJ, size_t gnode,
-                                              int t, double (*red)[3], double* partial) {
+                                              int t, int e0, int n_valid, double (*red)[3],
+                                              double* partial) {
+  constexpr int WPS = BE / 64;  // waves per element slot
   double l[3] = {0.0, 0.0, 0.0};
   if (active) node_scaled_speeds(q, un, cv64 + gnode * 9, fabs(invJ[gnode]), l);
 #pragma unroll
@@ -639,11 +674,13 @@ __device__ __forceinline__ void element_speed(const E3Params& q, bool active, co
     for (int d = 0; d < 3; ++d) red[t >> 6][d] = l[d];
   }
   __syncthreads();
-  if (t == 0) {
-    double m[3] = {red[0][0], red[0][1], red[0][2]};
-    for (int w = 1; w < B / 64; ++w)
-      for (int d = 0; d < 3; ++d) m[d] = speed_max(m[d], red[w][d]);
-    *partial = m[0] + m[1] + m[2];
+  const int es = t / BE;
+  if (t - es * BE == 0 && es < n_valid) {
+    const int w0 = es * WPS;
+    double m[3] = {red[w0][0], red[w0][1], red[w0][2]};
+    for (int w = 1; w < WPS; ++w)
+      for (int d = 0; d < 3; ++d) m[d] = speed_max(m[d], red[w0 + w][d]);
+    partial[e0 + es] = m[0] + m[1] + m[2];
   }
 }
 
@@ -687,7 +724,7 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
   if (active) node_fields_to_lds<N, VOL, NC, T>(q, u, gnode, t, s_w, s_phi, s_pb, bg);
 
   FacePrefetch<N> pf;
-  face_prefetch<N>(pf, e, t, u, nbr, face_info);
+  face_prefetch<N>(pf, e, 1, t, u, nbr, face_info);
 
   Acc<T, KAHAN> acc[5], acc_nc[4];
   for (int d = 0; d < 3; ++d) {
@@ -758,10 +795,10 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
     }
   }
   FaceNormals<N, T> fn;
-  face_normals<N, T>(fn, e, t, cv, nullptr, pf, nbr, face_info);
+  face_normals<N, T>(fn, e, 1, t, cv, nullptr, 0, pf, nbr, face_info);
   __syncthreads();  // pair terms consumed; the buffer now takes the face fluxes
-  face_fluxes_to_lds<N, T>(q, e, t, inv_w, u, pf, fn, nbr, face_info, surf, snc, bg, s_w, s_pb,
-                           s_fl);
+  face_fluxes_to_lds<N, T>(q, e, 1, t, inv_w, u, pf, fn, nbr, face_info, surf, snc, bg, s_w, 0,
+                           s_pb, s_fl, 0);
   __syncthreads();
   double un[6];
   if (active)
@@ -769,7 +806,7 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
                                 rk_a, res.dt ? res.dt[0] : rk_dt, rk_b, coriolis, bg, un);
   if (res.speed_partial) {
     __shared__ double s_red[B / 64][3];
-    element_speed<B>(q, active, un, res.cv64, invJ, gnode, t, s_red, res.speed_partial + e);
+    element_speed<B>(q, active, un, res.cv64, invJ, gnode, t, e, 1, s_red, res.speed_partial);
   }
 }
 
@@ -780,8 +817,8 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
 // pair-record form's exactly; in exchange for twice the flux arithmetic the
 // volume phase needs no pair buffer and no barrier, which leaves LDS for more
 // resident workgroups.
-template <int N, int VOL, int NC, typename T, bool KAHAN>
-__global__ __launch_bounds__(Shape<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+template <int N, int VOL, int NC, typename T, bool KAHAN, int EPB = 1>
+__global__ __launch_bounds__(EPB * Shape<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 void e3_stage_direct(
     E3Params q, DsplitTable dst, double inv_w, int n_elements, const double* __restrict__ u,
     double* __restrict__ out, double* __restrict__ kreg, const T* __restrict__ cv,
@@ -789,50 +826,66 @@ void e3_stage_direct(
     const int* __restrict__ face_info, int surf, int snc, int mode, double rk_a, double rk_dt,
     ResidentLink res, double rk_b, double coriolis, Background bg) {
   using S = Shape<N>;
-  constexpr int N2 = S::N2, N3 = S::N3;
+  constexpr int N2 = S::N2, N3 = S::N3, BE = S::BLOCK;
   constexpr bool MIXED = sizeof(T) == 4;
   constexpr int NW = NodeFields<VOL, NC, T>::NW;
   constexpr int BUF = 9 * N3 > 36 * N2 ? 9 * N3 : 36 * N2;
   // flux_kennedy_gruber (with or without a gravity term) takes the sum form
   // below
   constexpr bool KG_SUMS = VOL == 1;
-  __shared__ T s_w[NW * N3];
-  __shared__ double s_phi[MIXED ? N3 : 1];
-  __shared__ double s_pb[MIXED ? N3 : 1];
-  __shared__ T s_buf[BUF];  // contravariant vectors [9][n^3], later face fluxes [6][6 n^2]
+  // EPB element slots of BE lanes each; slot es owns s_w_all[es], s_buf_all[es]
+  __shared__ T s_w_all[EPB][NW * N3];
+  __shared__ double s_phi_all[EPB][MIXED ? N3 : 1];
+  __shared__ double s_pb_all[EPB][MIXED ? N3 : 1];
+  __shared__ T s_buf_all[EPB][BUF];  // contravariant vectors [9][n^3], later face fluxes [6][6 n^2]
   __shared__ T s_ds[N * N];
 
   if constexpr (!MIXED) bg.on = 0;
-  const int e = xcd_element(blockIdx.x, n_elements);
-  if (e >= n_elements) return;
+  const int n_groups = (n_elements + EPB - 1) / EPB;
+  const int grp = xcd_element(blockIdx.x, n_groups);
+  if (grp >= n_groups) return;  // whole workgroup: padding of the XCD mapping
+  const int e0 = grp * EPB;
+  const int n_valid = EPB == 1 ? 1 : (n_elements - e0 < EPB ? n_elements - e0 : EPB);
   const int t = threadIdx.x;
-  const bool active = t < N3;
-  const int ti = t % N, tj = (t / N) % N, tk = t / N2;
-  const size_t gnode = (size_t)e * N3 + t;
+  // element slot and lane within it (one slot: the lane itself)
+  const int es = EPB == 1 ? 0 : t / BE, tl = EPB == 1 ? t : t - es * BE;
+  const int e = e0 + es;
+  const bool active = tl < N3 && es < n_valid;
+  const int ti = tl % N, tj = (tl / N) % N, tk = tl / N2;
+  const size_t gnode = (size_t)e * N3 + tl;
+  T* s_w = s_w_all[es];
+  double* s_phi = s_phi_all[es];
+  double* s_pb = s_pb_all[es];
+  T* s_buf = s_buf_all[es];
   if (t < N * N) s_ds[t] = T(dst.ds[t]);
   if (active) {
-    node_fields_to_lds<N, VOL, NC, T>(q, u, gnode, t, s_w, s_phi, s_pb, bg);
+    node_fields_to_lds<N, VOL, NC, T>(q, u, gnode, tl, s_w, s_phi, s_pb, bg);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) s_buf[c * N3 + t] = cv[gnode * 9 + c];
+    for (int c = 0; c < 9; ++c) s_buf[c * N3 + tl] = cv[gnode * 9 + c];
   }
-  FacePrefetch<N> pf;
-  face_prefetch<N>(pf, e, t, u, nbr, face_info);
+  FacePrefetch<N, EPB> pf;
+  face_prefetch<N, EPB>(pf, e0, n_valid, t, u, nbr, face_info);
   __syncthreads();
 
   Acc<T, KAHAN> acc[5], acc_nc[4];
   if (active) {
-    const FastPrim<T> l = load_fast_prim<VOL, NC, N3, T>(s_w, t, bg);
+    const FastPrim<T> l = load_fast_prim<VOL, NC, N3, T>(s_w, tl, bg);
     double phi_own = 0.0;
-    if constexpr (MIXED) phi_own = s_phi[t];
+    if constexpr (MIXED) phi_own = s_phi[tl];
     for (int d = 0; d < 3; ++d) {
       const int stride = d == 0 ? 1 : (d == 1 ? N : N2);
       const int own_d = d == 0 ? ti : (d == 1 ? tj : tk);
       const T* ja = s_buf + 3 * d * N3;
-      const T jo0 = ja[t], jo1 = ja[N3 + t], jo2 = ja[2 * N3 + t];
-#pragma unroll
-      for (int m = 0; m < N; ++m) {
-        if (m == own_d) continue;
-        const int pn = t + (m - own_d) * stride;  // the partner node
+      const T jo0 = ja[tl], jo1 = ja[N3 + tl], jo2 = ja[2 * N3 + tl];
+      // the partners in cyclic order own_d + 1, ..., own_d + n - 1 (mod n):
+      // every lane has a partner in each of the n - 1 iterations (a loop over
+      // the line's n nodes that skips the own one leaves a lane in n idle in
+      // every iteration)
+#pragma unroll 1
+      for (int j = 1; j < N; ++j) {
+        const bool wrap = own_d + j >= N;
+        const int m = wrap ? own_d + j - N : own_d + j;
+        const int pn = wrap ? tl + (j - N) * stride : tl + j * stride;  // the partner node
         const FastPrim<T> r = load_fast_prim<VOL, NC, N3, T>(s_w, pn, bg);
         T dphi;
         if constexpr (MIXED) dphi = T(s_phi[pn] - phi_own);
@@ -904,20 +957,20 @@ void e3_stage_direct(
       for (int v = 0; v < 4; ++v) acc_nc[v].scale(scn[v]);
     }
   }
-  FaceNormals<N, T> fn;
-  face_normals<N, T>(fn, e, t, cv, s_buf, pf, nbr, face_info);
+  FaceNormals<N, T, EPB> fn;
+  face_normals<N, T, EPB>(fn, e0, n_valid, t, cv, s_buf_all[0], BUF, pf, nbr, face_info);
   __syncthreads();  // contravariant vectors consumed; the buffer takes the face fluxes
-  face_fluxes_to_lds<N, T>(q, e, t, inv_w, u, pf, fn, nbr, face_info, surf, snc, bg, s_w, s_pb,
-                           s_buf);
+  face_fluxes_to_lds<N, T, EPB>(q, e0, n_valid, t, inv_w, u, pf, fn, nbr, face_info, surf, snc, bg,
+                                s_w_all[0], NW * N3, s_pb_all[0], s_buf_all[0], BUF);
   __syncthreads();
   double un[6];
   if (active)
     stage_epilogue<N, T, KAHAN>(acc, acc_nc, s_buf, ti, tj, tk, gnode, u, out, kreg, invJ, mode,
                                 rk_a, res.dt ? res.dt[0] : rk_dt, rk_b, coriolis, bg, un);
   if (res.speed_partial) {
-    __shared__ double s_red[S::BLOCK / 64][3];
-    element_speed<S::BLOCK>(q, active, un, res.cv64, invJ, gnode, t, s_red,
-                            res.speed_partial + e);
+    __shared__ double s_red[EPB * BE / 64][3];
+    element_speed<BE, EPB>(q, active, un, res.cv64, invJ, gnode, t, e0, n_valid, s_red,
+                           res.speed_partial);
   }
 }
 
@@ -931,23 +984,45 @@ constexpr bool use_direct_form() {
   return sizeof(T) == 8 || N != 5;
 }
 
+// two elements per workgroup is built where it can pay: polydeg 3 and 4 of
+// the direct form, whose 6 n^2 face nodes leave a partly filled last face
+// pass (polydeg 4: 150 face nodes on 128 lanes; two elements put 300 on 256)
+template <int N, typename T>
+constexpr bool has_two_element_groups() {
+  return (N == 4 || N == 5) && use_direct_form<N, 1, T>();
+}
+
+template <int N, int VOL, int NC, typename T, bool KAHAN, int EPB>
+void launch_direct(const E3Launch& L, const DsplitTable& dst, hipStream_t s) {
+  const int n_groups = (L.n_elements + EPB - 1) / EPB;
+  const int grid = 8 * ((n_groups + 7) / 8);
+  hipLaunchKernelGGL((e3_stage_direct<N, VOL, NC, T, KAHAN, EPB>), dim3(grid),
+                     dim3(EPB * Shape<N>::BLOCK), 0, s, L.q, dst, L.inv_w, L.n_elements, L.u,
+                     L.out, L.k, reinterpret_cast<const T*>(L.cv_t), L.invJ, L.nbr, L.face_info,
+                     L.surface_flux, L.surface_noncons, L.mode, L.rk_a, L.rk_dt, L.res, L.rk_b,
+                     L.coriolis, L.bg);
+}
+
 template <int N, int VOL, int NC, typename T, bool KAHAN>
 void launch_stage_inst(const E3Launch& L, hipStream_t s) {
   DsplitTable dst;
   for (int q = 0; q < N * N; ++q) dst.ds[q] = L.dsplit[q];
   const int grid = 8 * ((L.n_elements + 7) / 8);
-  if constexpr (use_direct_form<N, VOL, T>())
-    hipLaunchKernelGGL((e3_stage_direct<N, VOL, NC, T, KAHAN>), dim3(grid),
-                       dim3(Shape<N>::BLOCK), 0, s, L.q, dst, L.inv_w, L.n_elements, L.u, L.out,
-                       L.k, reinterpret_cast<const T*>(L.cv_t), L.invJ, L.nbr, L.face_info,
-                       L.surface_flux, L.surface_noncons, L.mode, L.rk_a, L.rk_dt, L.res, L.rk_b,
-                       L.coriolis, L.bg);
-  else
+  if constexpr (use_direct_form<N, VOL, T>()) {
+    if constexpr (has_two_element_groups<N, T>()) {
+      if (L.epb == 2) {
+        launch_direct<N, VOL, NC, T, KAHAN, 2>(L, dst, s);
+        return;
+      }
+    }
+    launch_direct<N, VOL, NC, T, KAHAN, 1>(L, dst, s);
+  } else {
     hipLaunchKernelGGL((e3_stage<N, VOL, NC, T, KAHAN>), dim3(grid), dim3(Shape<N>::BLOCK), 0, s,
                        L.q, dst, L.inv_w, L.n_elements, L.u, L.out, L.k,
                        reinterpret_cast<const T*>(L.cv_t), L.invJ, L.nbr, L.face_info,
                        L.surface_flux, L.surface_noncons, L.mode, L.rk_a, L.rk_dt, L.res, L.rk_b,
                        L.coriolis, L.bg);
+  }
 }
 
 template <typename T, bool KAHAN, int N, int VOL>

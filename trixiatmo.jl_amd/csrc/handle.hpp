@@ -52,6 +52,7 @@ struct atmo_handle {
   double *d_aux = nullptr, *d_xyz = nullptr, *d_dt_member = nullptr;
   double *d_geo = nullptr, *d_fxf = nullptr;  // stage-kernel geometry (compact, face transforms)
   float *d_geo32 = nullptr, *d_fxf32 = nullptr;  // the same in FP32 (ATMO_PREC_FP32_GEOMETRY)
+  double* d_hs64 = nullptr;  // FP32 geometry with bottom topography: h_s kept in FP64
   int *d_member = nullptr, *d_member_first = nullptr;
   int n_members = 1;
   bool swe_tuned = false;                  // workgroup shape chosen by the autotuner
@@ -66,8 +67,13 @@ struct atmo_handle {
   // 3D atmo_step: two RK steps captured for replay, their step size read from d_step_dt
   hipGraphExec_t e3_graph = nullptr;
   double* e3_graph_u = nullptr;            // state buffer the graph was captured with
+  int e3_graph_epb = 0;                    // and the stage shape
+  int e3_epb = 1;                          // elements per workgroup of the 3D stage
+  bool e3_tuned = false;
   double* d_step_dt = nullptr;
-  unsigned long long* d_stats = nullptr;   // Newton statistics
+  // Newton statistics: per-wave slots [n_stat_slots][4], then the reduced [4]
+  unsigned long long* d_stats = nullptr;
+  int n_stat_slots = 0;
   atmo_mlp* mlp = nullptr;                 // seed of the saturation solve
   bool du_valid = false;
   bool timing = false;
@@ -94,6 +100,10 @@ int swe_step(atmo_handle* h, const double* dt_members, double dt, int n_steps);
 int swe_max_dt(atmo_handle* h, double cfl, double* dt_members);
 int swe_integrals(atmo_handle* h, int fresh_rhs, double* out);
 int swe_autotune(atmo_handle* h, double* ms_per_variant);
+// 3D stage: one or two elements per workgroup (ms_per_variant[2] may be NULL)
+int e3_autotune(atmo_handle* h, double* ms_per_variant);
+// whether the 3D stage of this polydeg / precision has the two-element shape
+bool e3_has_two_element_groups(int n, int precision);
 // rainy Euler paths of the generic entry points
 int rainy_rhs_device(atmo_handle* h, double* du, const double* u, double t);
 int rainy_step(atmo_handle* h, double dt, int n_steps);

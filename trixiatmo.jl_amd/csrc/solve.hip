@@ -28,7 +28,7 @@ namespace atmo {
 namespace {
 
 // the step just taken is added to t, then the next step size is
-// dt = min(cfl * 2 / (n * max speed), t_end - t) as atmo_max_dt and the host
+// dt = min(cfl * (2 / (n * max speed)), t_end - t) as atmo_max_dt and the host
 // loop compute it (same operations in the same order), or 0 once t >= t_end
 __device__ __forceinline__ void step_control_kernel_body(double max_speed, int m, int M, int n,
                                                          double* clock, double* dt) {
@@ -41,7 +41,7 @@ __device__ __forceinline__ void step_control_kernel_body(double max_speed, int m
       clock[3 * M] = 1.0;  // every member that finds one stores the same value
     } else {
       if (mx < 5e-324) mx = 5e-324;
-      d = fmin(cfl * 2.0 / (n * mx), t_end - t);
+      d = fmin(cfl * (2.0 / (n * mx)), t_end - t);
       clock[2 * M + m] += 1.0;
     }
   }
@@ -203,6 +203,10 @@ int atmo_solve(atmo_handle* h, double t_end, double cfl, int32_t check_every, in
       const int rc = atmo::swe_autotune(h, nullptr);
       if (rc != ATMO_OK) return rc;
     }
+  }
+  if (h->eq == ATMO_EQ_EULER_ENERGY_GRAVITY_3D && !h->e3_tuned) {
+    const int rc = atmo::e3_autotune(h, nullptr);
+    if (rc != ATMO_OK) return rc;
   }
   h->du_valid = false;
   bool done = !(h->time < t_end);
