@@ -3,6 +3,7 @@
 // points, the saturation Newton solve) and for the MLP that seeds that solve.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -165,7 +166,7 @@ int rainy_max_dt(atmo_handle* h, double cfl, double* dt) {
   HIP_TRY(h, hipMemcpyAsync(&mx, h->d_result, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   if (!std::isfinite(mx)) return fail(&h->err, ATMO_ERR_NONFINITE, "non-finite wave speed");
-  *dt = cfl * 2.0 / (h->n * mx);
+  *dt = cfl * (2.0 / (h->n * mx));  // max_dt (stepsize_dg2d.jl:43), then StepsizeCallback's cfl factor
   return ATMO_OK;
 }
 
@@ -227,6 +228,13 @@ int atmo_rainy_create(const atmo_rainy_config* cfg, atmo_handle** out) {
   if (hipGetDevice(&h->device) != hipSuccess) return cleanup_fail(ATMO_ERR_DEVICE, "no HIP device");
   Basis b(n);
   const size_t sb = sizeof(double) * h->len;
+  // one statistics slot per wave: the stage launches at most one wave per
+  // element, the Newton kernel four per 256 nodes
+  {
+    const size_t nodes = (size_t)n * n * E;
+    h->n_stat_slots = (int)std::max<size_t>((size_t)E, 4 * ((nodes + 255) / 256));
+  }
+  const size_t stat_bytes = sizeof(unsigned long long) * (4 * (size_t)h->n_stat_slots + 4);
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&h->d_u, sb) != hipSuccess || hipMalloc(&h->d_u2, sb) != hipSuccess ||
       hipMalloc(&h->d_k, sb) != hipSuccess || hipMalloc(&h->d_du, sb) != hipSuccess ||
@@ -234,12 +242,12 @@ int atmo_rainy_create(const atmo_rainy_config* cfg, atmo_handle** out) {
       hipMalloc(&h->d_dt_member, sizeof(double) * 16) != hipSuccess ||
       hipMalloc(&h->d_partial, sizeof(double) * 9 * (size_t)E) != hipSuccess ||
       hipMalloc(&h->d_result, sizeof(double) * 16) != hipSuccess ||
-      hipMalloc(&h->d_stats, sizeof(unsigned long long) * 4) != hipSuccess)
+      hipMalloc(&h->d_stats, stat_bytes) != hipSuccess)
     return cleanup_fail(ATMO_ERR_DEVICE, "device allocation failed");
   if (hipMemcpy(h->d_dt_member, b.w.data(), sizeof(double) * n, hipMemcpyHostToDevice) !=
           hipSuccess ||
       hipMemset(h->d_u, 0, sb) != hipSuccess || hipMemset(h->d_k, 0, sb) != hipSuccess ||
-      hipMemset(h->d_stats, 0, sizeof(unsigned long long) * 4) != hipSuccess)
+      hipMemset(h->d_stats, 0, stat_bytes) != hipSuccess)
     return cleanup_fail(ATMO_ERR_DEVICE, "device upload failed");
   RainyLaunch& L = h->rainy_base;
   std::memset(&L, 0, sizeof(L));
@@ -286,21 +294,28 @@ int atmo_saturation_newton(atmo_handle* h, int32_t seeded, int32_t* iterations,
   if (!h) return fail(nullptr, ATMO_ERR_INVALID, "null handle");
   if (h->eq != ATMO_EQ_RAINY_EULER_2D)
     return fail(&h->err, ATMO_ERR_INVALID, "the saturation solve belongs to rainy handles");
-  if (seeded && !h->mlp) return fail(&h->err, ATMO_ERR_INVALID, "no MLP attached");
+  if (seeded < 0 || seeded > 2) return fail(&h->err, ATMO_ERR_INVALID, "seeded must be 0, 1 or 2");
+  if (seeded == 1 && !h->mlp) return fail(&h->err, ATMO_ERR_INVALID, "no MLP attached");
   if (h->rainy.newton_max_iterations < 1)
     return fail(&h->err, ATMO_ERR_INVALID, "the handle has no Newton settings");
   RainyLaunch L = h->rainy_base;
-  HIP_TRY(h, hipMemsetAsync(h->d_stats, 0, sizeof(unsigned long long) * 4, h->stream));
+  // counts accumulated by earlier stage limiters are discarded: the slots
+  // then hold this solve's counts only
+  unsigned long long* out = h->d_stats + 4 * (size_t)h->n_stat_slots;
+  rainy_launch_stats_reduce(h->d_stats, h->n_stat_slots, out, h->stream);
   L.stats = h->d_stats;
+  L.closed_form_seed = seeded == 2;
   MlpView mv{};
-  if (seeded) mv = MlpView{h->mlp->d_params, h->mlp->d_norm};
+  if (seeded == 1) mv = MlpView{h->mlp->d_params, h->mlp->d_norm};
   TimerPair tp;
   if (h->timing) timer_begin(h, 1, &tp);
-  rainy_launch_newton(L, h->d_u, seeded ? &mv : nullptr, iterations, h->stream);
+  rainy_launch_newton(L, h->d_u, seeded == 1 ? &mv : nullptr, iterations, h->stream);
   if (h->timing) timer_end(h, &tp);
   HIP_TRY(h, hipGetLastError());
+  rainy_launch_stats_reduce(h->d_stats, h->n_stat_slots, out, h->stream);
+  HIP_TRY(h, hipGetLastError());
   unsigned long long st[4];
-  HIP_TRY(h, hipMemcpyAsync(st, h->d_stats, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipMemcpyAsync(st, out, sizeof(st), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   if (stats) {
     stats->nodes = h->len / 9;
@@ -316,7 +331,10 @@ int atmo_saturation_newton(atmo_handle* h, int32_t seeded, int32_t* iterations,
 int atmo_enable_newton_stats(atmo_handle* h, int32_t enable) {
   if (!h || h->eq != ATMO_EQ_RAINY_EULER_2D)
     return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "not a rainy handle");
-  HIP_TRY(h, hipMemsetAsync(h->d_stats, 0, sizeof(unsigned long long) * 4, h->stream));
+  // clear the slots (the reduction clears what it reads)
+  rainy_launch_stats_reduce(h->d_stats, h->n_stat_slots, h->d_stats + 4 * (size_t)h->n_stat_slots,
+                            h->stream);
+  HIP_TRY(h, hipGetLastError());
   h->rainy_base.stats = enable ? h->d_stats : nullptr;
   return ATMO_OK;
 }
@@ -324,15 +342,17 @@ int atmo_enable_newton_stats(atmo_handle* h, int32_t enable) {
 int atmo_newton_stats_read(atmo_handle* h, atmo_newton_stats* stats) {
   if (!h || !stats || h->eq != ATMO_EQ_RAINY_EULER_2D)
     return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "not a rainy handle");
+  unsigned long long* out = h->d_stats + 4 * (size_t)h->n_stat_slots;
+  rainy_launch_stats_reduce(h->d_stats, h->n_stat_slots, out, h->stream);
+  HIP_TRY(h, hipGetLastError());
   unsigned long long st[4];
-  HIP_TRY(h, hipMemcpyAsync(st, h->d_stats, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipMemcpyAsync(st, out, sizeof(st), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   stats->nodes = h->len / 9;
   stats->updates = (int64_t)st[0];
   stats->failures = (int64_t)st[1];
   stats->max_updates = (int64_t)st[2];
   stats->fallbacks = (int64_t)st[3];
-  HIP_TRY(h, hipMemsetAsync(h->d_stats, 0, sizeof(st), h->stream));
   return ATMO_OK;
 }
 
@@ -357,8 +377,10 @@ int atmo_rainy_attach_mlp(atmo_handle* h, atmo_mlp* m) {
 int atmo_rainy_seed_stage_limiter(atmo_handle* h, int32_t enable) {
   if (!h || h->eq != ATMO_EQ_RAINY_EULER_2D)
     return fail(h ? &h->err : nullptr, ATMO_ERR_INVALID, "not a rainy handle");
-  if (enable && !h->mlp) return fail(&h->err, ATMO_ERR_INVALID, "no MLP attached");
-  h->rainy_base.mlp = enable ? MlpView{h->mlp->d_params, h->mlp->d_norm} : MlpView{nullptr, nullptr};
+  if (enable < 0 || enable > 2) return fail(&h->err, ATMO_ERR_INVALID, "enable must be 0, 1 or 2");
+  if (enable == 1 && !h->mlp) return fail(&h->err, ATMO_ERR_INVALID, "no MLP attached");
+  h->rainy_base.mlp = enable == 1 ? MlpView{h->mlp->d_params, h->mlp->d_norm} : MlpView{nullptr, nullptr};
+  h->rainy_base.closed_form_seed = enable == 2;
   return ATMO_OK;
 }
 

@@ -437,22 +437,31 @@ struct RShape {
   static constexpr int NF = 4 * N;
 };
 
-__device__ __forceinline__ void accumulate_stats(unsigned long long* stats, int it) {
-  // wave-level sums keep the number of atomics small
+// the Newton statistics of one wave (updates, failures, largest update count,
+// fallbacks), added to the wave's own slot stats[4 * slot ..] by its lane 0:
+// no two waves of a launch share a slot and launches on the handle's stream
+// follow each other, so no atomics are needed (one 64-bit atomic per wave on
+// three shared counters serialised in the L2 and cost ~0.3 ms per launch)
+__device__ __forceinline__ void accumulate_stats(unsigned long long* stats, size_t slot, int it,
+                                                 int fallback) {
   unsigned long long upd = it > 0 ? (unsigned long long)it : 0ull;
   unsigned long long fail = it < 0 ? 1ull : 0ull;
+  unsigned long long fb = fallback ? 1ull : 0ull;
   unsigned long long mx = upd;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     upd += __shfl_xor(upd, o, 64);
     fail += __shfl_xor(fail, o, 64);
+    fb += __shfl_xor(fb, o, 64);
     const unsigned long long other = __shfl_xor(mx, o, 64);
     mx = other > mx ? other : mx;
   }
   if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&stats[0], upd);
-    atomicAdd(&stats[1], fail);
-    atomicMax(&stats[2], mx);
+    unsigned long long* s = stats + 4 * slot;
+    s[0] += upd;
+    s[1] += fail;
+    s[2] = s[2] > mx ? s[2] : mx;
+    s[3] += fb;
   }
 }
 
@@ -498,6 +507,52 @@ __device__ __forceinline__ bool mlp_seed(const double* u, const float* y, double
   x[1] = rc;
   x[2] = (e_int - rv * L_ref) / (c_vd * u[0] + c_vv * rv + c_l * (rc + u[2]));
   return x[2] > 0.0;
+}
+
+// closed-form guess without a network: all moisture as vapour with T from
+// the energy row (exact below saturation: no update is then needed); where
+// that state is supersaturated, one saturation adjustment at that
+// temperature (vapour at its saturation density, the rest cloud) with T from
+// the energy row again.  false when T is not positive.
+__device__ __forceinline__ bool closed_form_seed(const double* u, double* x) {
+  const double rho = u[0] + u[1] + u[2];
+  const double inv = 1.0 / rho;
+  const double v1 = u[3] * inv, v2 = u[4] * inv;
+  const double e_int = u[5] - rho * 0.5 * (v1 * v1 + v2 * v2);
+  double rv = u[1], rc = 0.0;
+  double T = (e_int - rv * L_ref) / (c_vd * u[0] + c_vv * rv + c_l * u[2]);
+  if (T > 0.0) {
+    const double rvs = svp(T) / (R_v * T);
+    if (rv > rvs) {
+      rv = rvs;
+      rc = u[1] - rv;
+      T = (e_int - rv * L_ref) / (c_vd * u[0] + c_vv * rv + c_l * (rc + u[2]));
+    }
+  }
+  x[0] = rv;
+  x[1] = rc;
+  x[2] = T;
+  return T > 0.0;
+}
+
+// the saturation solve of one node from the closed-form guess, redone from
+// the default guess u[7:9] (x on entry) where it fails; returns the update
+// count (-1: failed) and sets *fallback
+__device__ __forceinline__ int closed_form_solve(const double* u, double* x, double tol,
+                                                 int max_iter, int* fallback) {
+  const double x0[3] = {x[0], x[1], x[2]};
+  const bool ok = closed_form_seed(u, x);
+  int it = ok ? newton_node(u, x, tol, max_iter) : -1;
+  if (it < 0 || !(x[2] > 0.0)) {
+    const int used = ok ? max_iter : 0;
+    x[0] = x0[0];
+    x[1] = x0[1];
+    x[2] = x0[2];
+    const int it2 = newton_node(u, x, tol, max_iter);
+    it = it2 < 0 ? -1 : used + it2;
+    *fallback = 1;
+  }
+  return it;
 }
 
 // t0 + c dt rounded as two separate operations (the host's order, no fused
@@ -691,7 +746,7 @@ void rainy_stage(RainyLaunch L) {
     for (int v = 0; v < NV; ++v) s_sf(fe, v, r) = sgn * fl[v];
   }
   __syncthreads();
-  int it_stat = 0;
+  int it_stat = 0, fallback_stat = 0;
   double un[NV];  // the RK-updated state (stage mode)
 #pragma unroll
   for (int v = 0; v < NV; ++v) un[v] = 0.0;
@@ -768,10 +823,13 @@ void rainy_stage(RainyLaunch L) {
             x[2] = un[8];
             const int it2 = newton_node(un, x, L.newton_tol, L.newton_max_iter);
             it_stat = it2 < 0 ? -1 : used + it2;
-            if (L.stats) atomicAdd(&L.stats[3], 1ull);
+            fallback_stat = 1;
           }
         } else {
-          it_stat = newton_node(un, x, L.newton_tol, L.newton_max_iter);
+          if (L.closed_form_seed)
+            it_stat = closed_form_solve(un, x, L.newton_tol, L.newton_max_iter, &fallback_stat);
+          else
+            it_stat = newton_node(un, x, L.newton_tol, L.newton_max_iter);
         }
         if (un[2] < 0.0) un[2] = 0.0;
         un[6] = x[0];
@@ -804,7 +862,8 @@ void rainy_stage(RainyLaunch L) {
     }
   }
   // every lane of the (single-wave) workgroup takes part in the reduction
-  if (L.stats && L.mode == 1 && L.newton_max_iter > 0) accumulate_stats(L.stats, it_stat);
+  if (L.stats && L.mode == 1 && L.newton_max_iter > 0)
+    accumulate_stats(L.stats, blockIdx.x, it_stat, fallback_stat);
 }
 
 // per-element maximum of the scaled wave speeds (stepsize_dg2d.jl:5-43 pattern)
@@ -898,7 +957,7 @@ __global__ __launch_bounds__(256) void rainy_newton_kernel(RainyLaunch L, double
                                                            unsigned long long* stats) {
   const size_t nodes = (size_t)L.n_elements * L.n * L.n;
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int it = 0;
+  int it = 0, fallback = 0;
   const bool valid = q < nodes;
   double uc[NV];
 #pragma unroll
@@ -913,7 +972,6 @@ __global__ __launch_bounds__(256) void rainy_newton_kernel(RainyLaunch L, double
   if (valid) {
     double* un = u + q * NV;
     double x[3] = {uc[6], uc[7], uc[8]};
-    int fallback = 0;
     if (use_mlp) {
       const bool ok = mlp_seed(uc, ypred, x);
       it = ok ? newton_node(uc, x, L.newton_tol, L.newton_max_iter) : -1;
@@ -927,6 +985,8 @@ __global__ __launch_bounds__(256) void rainy_newton_kernel(RainyLaunch L, double
         const int it2 = newton_node(uc, x, L.newton_tol, L.newton_max_iter);
         it = it2 < 0 ? -1 : used + it2;
       }
+    } else if (L.closed_form_seed) {
+      it = closed_form_solve(uc, x, L.newton_tol, L.newton_max_iter, &fallback);
     } else {
       it = newton_node(uc, x, L.newton_tol, L.newton_max_iter);
     }
@@ -935,11 +995,9 @@ __global__ __launch_bounds__(256) void rainy_newton_kernel(RainyLaunch L, double
     un[7] = x[1];
     un[8] = x[2];
     if (iterations) iterations[q] = it;
-    if (stats && fallback) atomicAdd(&stats[3], 1ull);
-  } else {
-    it = 0;
   }
-  if (stats) accumulate_stats(stats, q < nodes ? it : 0);
+  // every lane of the wave takes part (a lane past the last node adds nothing)
+  if (stats) accumulate_stats(stats, q / 64, it, fallback);
 }
 
 // features for training: x = (rho, internal energy, total water), y = (T, q_v)
@@ -1010,6 +1068,42 @@ bool rainy_launch_integrals(const RainyLaunch& L, const double* w, double* parti
 #undef ATMO_CASE
   hipLaunchKernelGGL(rainy_sum_columns, dim3(1), dim3(256), 0, s, partial, L.n_elements, out);
   return true;
+}
+
+// sums (max for the largest update count) of the per-wave statistics slots
+// into out[4], clearing the slots for the next accumulation
+__global__ __launch_bounds__(256) void rainy_stats_reduce(unsigned long long* __restrict__ slots,
+                                                          int n_slots,
+                                                          unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long s[4][256];
+  const int t = threadIdx.x;
+  unsigned long long a[4] = {0ull, 0ull, 0ull, 0ull};
+  for (int i = t; i < n_slots; i += 256) {
+    unsigned long long* p = slots + 4 * (size_t)i;
+    a[0] += p[0];
+    a[1] += p[1];
+    a[2] = a[2] > p[2] ? a[2] : p[2];
+    a[3] += p[3];
+    p[0] = p[1] = p[2] = p[3] = 0ull;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) s[k][t] = a[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      s[0][t] += s[0][t + w];
+      s[1][t] += s[1][t + w];
+      s[2][t] = s[2][t] > s[2][t + w] ? s[2][t] : s[2][t + w];
+      s[3][t] += s[3][t + w];
+    }
+    __syncthreads();
+  }
+  if (t < 4) out[t] = s[t][0];
+}
+
+void rainy_launch_stats_reduce(unsigned long long* slots, int n_slots, unsigned long long* out,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(rainy_stats_reduce, dim3(1), dim3(256), 0, s, slots, n_slots, out);
 }
 
 void rainy_launch_newton(const RainyLaunch& L, double* u, const MlpView* mlp, int* iterations,

@@ -111,11 +111,20 @@ struct RainyLaunch {
   double* speed_partial;
   double newton_tol;
   int newton_max_iter;
-  unsigned long long* stats;  // optional: [sum of updates, failures, max updates, fallbacks]
+  // optional Newton statistics: per wave of a launch (slot = global wave
+  // index, fewer than n_elements in both the stage and the Newton kernel)
+  // [sum of updates, failures, max updates, fallbacks], accumulated over
+  // launches until rainy_launch_stats_reduce collects and clears them
+  unsigned long long* stats;
   MlpView mlp;                // seeds the stage limiter's solve when mlp.params is set
+  int closed_form_seed;       // else, nonzero: the closed-form guess seeds it
 };
 
 bool rainy_launch_stage(const RainyLaunch& L, hipStream_t s);
+// out[4] = the statistics summed over the slots [n_slots][4] (max of the
+// largest update count); the slots are cleared
+void rainy_launch_stats_reduce(unsigned long long* slots, int n_slots, unsigned long long* out,
+                               hipStream_t s);
 // per-element scaled wave speeds, then their maximum -> out[0]
 bool rainy_launch_max_speed(const RainyLaunch& L, double* partial, double* out, hipStream_t s);
 // integral of each of the 9 variables -> out[9]

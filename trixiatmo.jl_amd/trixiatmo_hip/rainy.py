@@ -150,9 +150,12 @@ class SemidiscretizationRainy:
     # saturation solve
     def newton(self, seeded=False, iterations=None):
         """NonlinearSolveDG on the resident state; `iterations` an int32 device
-        tensor of one entry per node (optional).  Returns the statistics."""
+        tensor of one entry per node (optional).  `seeded`: False (the
+        previous u[7:9]), True or "network" (the attached MLP), or
+        "closed_form" (all vapour with T from the energy row, one saturation
+        adjustment where that is supersaturated).  Returns the statistics."""
         st = AtmoNewtonStats()
-        check(lib().atmo_saturation_newton(self._h, 1 if seeded else 0, _ptr(iterations),
+        check(lib().atmo_saturation_newton(self._h, _seed_mode(seeded), _ptr(iterations),
                                            ctypes.byref(st)), self._h)
         return _stats(st)
 
@@ -174,8 +177,16 @@ class SemidiscretizationRainy:
         check(lib().atmo_rainy_attach_mlp(self._h, mlp.handle if mlp else None), self._h)
 
     def seed_stage_limiter(self, on=True):
-        """seed the Newton solve of every stage with the attached network"""
-        check(lib().atmo_rainy_seed_stage_limiter(self._h, 1 if on else 0), self._h)
+        """seed the Newton solve of every stage: True or "network" with the
+        attached network, "closed_form" with the closed-form guess, False not"""
+        check(lib().atmo_rainy_seed_stage_limiter(self._h, _seed_mode(on)), self._h)
+
+
+def _seed_mode(seeded):
+    modes = {False: 0, True: 1, "network": 1, "closed_form": 2, None: 0}
+    if seeded not in modes:
+        raise ValueError(f"unknown seed {seeded!r}")
+    return modes[seeded]
 
 
 def _stats(st):
