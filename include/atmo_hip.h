@@ -219,7 +219,10 @@ typedef struct {
 } atmo_swe_config;
 
 typedef struct {
-  const double* aux_node_vars;           /* [26,n,n,E] */
+  /* [26,n,n,E], or NULL: the auxiliary variables are then computed on the
+   * device from node_coordinates (as atmo_covariant_aux_device does, with the
+   * two fields at the end of this struct) */
+  const double* aux_node_vars;
   const double* node_coordinates;        /* [3,n,n,E] */
   int32_t n_interfaces;
   const int32_t* interface_neighbor_ids; /* [2,I] */
@@ -228,7 +231,26 @@ typedef struct {
    * [member_first_element[m], member_first_element[m+1]); NULL: one member */
   int32_t n_members;
   const int32_t* member_first_element;   /* [n_members + 1] */
+  /* read only when aux_node_vars is NULL */
+  int32_t christoffel_collocation;       /* 0: exact metric derivatives, 1: collocation */
+  const double* bottom_topography;       /* [n,n,E] host array (aux row 20), or NULL for zero */
 } atmo_geometry2d;
+
+/* init_auxiliary_node_variables! on the device
+ * (src/solvers/dgsem_p4est/containers_2d_manifold_in_3d_covariant.jl:232-489,
+ * GlobalCartesianCoordinates, element_local_mapping = true): the 26
+ * auxiliary variables per node [26,n,n,E] of P4estMeshCubedSphere2D elements
+ * from their node coordinates [3,n,n,E] (the corners of each element and the
+ * radius of the first node, as the reference reads them), with the
+ * Christoffel symbols from the exact metric derivatives (forward-mode dual
+ * numbers) or, when christoffel_collocation is nonzero, the collocation
+ * derivative; bottom_topography [n,n,E] (may be NULL: zero) fills row 20.
+ * Device pointers; complete on return.  Every value equals that of
+ * atmo_mesh2d_cubed_sphere bit for bit except the exact Christoffel symbols,
+ * which agree to rounding (the host builder differentiates by complex step). */
+int atmo_covariant_aux_device(int polydeg, int32_t n_elements, const double* node_coordinates,
+                              int32_t christoffel_collocation, const double* bottom_topography,
+                              double* aux_node_vars);
 
 /* SemidiscretizationHyperbolic for the covariant form on P4estMesh{2}
  * (src/semidiscretization/semidiscretization_hyperbolic_2d_manifold_in_3d.jl:78-111,
@@ -240,15 +262,20 @@ typedef struct {
 int atmo_swe_create(const atmo_swe_config* config, const atmo_geometry2d* geometry,
                     atmo_handle** handle);
 
-/* Autotuner of the covariant stage kernel (no counterpart in the reference):
- * times every workgroup shape (1, 2 or 4 waves per workgroup, each with a
- * full or a persistent grid) on the resident state and keeps the fastest;
- * `ms` (may be NULL) receives the time of each of the ATMO_SWE_VARIANTS
- * shapes.  atmo_step tunes by itself on its first call; every shape gives
+/* Autotuner of the stage kernels (no counterpart in the reference): times
+ * every workgroup shape on the resident state (as an RK stage) and keeps the
+ * fastest; `ms` (may be NULL, else room for ATMO_SWE_VARIANTS values)
+ * receives the time of each shape.  Covariant form: 1, 2 or 4 waves per
+ * workgroup, each with a full or a persistent grid (ATMO_SWE_VARIANTS
+ * shapes).  3D Euler: one or two elements per workgroup (variants 0 and 1;
+ * the second is built for polydeg 3 and 4 of the direct volume form, elsewhere
+ * variant 0 is the only shape and its time is reported as 0).  atmo_step and
+ * atmo_solve tune by themselves on their first call; every shape gives
  * bit-identical results. */
 #define ATMO_SWE_VARIANTS 6
 int atmo_autotune(atmo_handle* handle, int32_t* variant, double* ms);
-/* pin the workgroup shape of the covariant stage kernel (0..ATMO_SWE_VARIANTS-1) */
+/* pin the workgroup shape of the stage kernel (covariant:
+ * 0..ATMO_SWE_VARIANTS-1; 3D: 0 or 1 elements-per-workgroup minus one) */
 int atmo_set_kernel_variant(atmo_handle* handle, int32_t variant);
 
 /* ensemble variants: per-member max_dt (dt[n_members]) and RK steps with
@@ -302,10 +329,12 @@ typedef struct {
 } atmo_newton_stats;
 
 /* the saturation solve of nonlinear_solve_dg_2d! (nonlinear_solve_dg2d.jl:6-45)
- * on the resident state: guess u[7:9] (seeded = 0) or the attached MLP's
- * (T, q_v) prediction (seeded = 1, falling back to u[7:9] where the seeded
- * solve fails); iterations[node] (device pointer, may be NULL) receives the
- * update count (-1: tolerance missed). */
+ * on the resident state: guess u[7:9] (seeded = 0), the attached MLP's
+ * (T, q_v) prediction (seeded = 1) or the closed-form guess (seeded = 2: all
+ * moisture as vapour with T from the energy row, and where that is
+ * supersaturated one saturation adjustment at that temperature), a seeded
+ * solve that fails being redone from u[7:9]; iterations[node] (device
+ * pointer, may be NULL) receives the update count (-1: tolerance missed). */
 int atmo_saturation_newton(atmo_handle* handle, int32_t seeded, int32_t* iterations_device,
                            atmo_newton_stats* stats);
 /* Newton statistics accumulated by atmo_step's stage limiter since the last
@@ -350,8 +379,9 @@ int atmo_mlp_train(atmo_mlp* mlp, const float* x_device, const float* y_device,
 int atmo_mlp_forward(atmo_mlp* mlp, const float* x_device, float* y_device, int64_t n);
 /* the network that seeds atmo_saturation_newton (NULL detaches) */
 int atmo_rainy_attach_mlp(atmo_handle* handle, atmo_mlp* mlp);
-/* seed the stage limiter of atmo_step with the attached network (with the
- * same fallback to u[7:9] per node) */
+/* seed the stage limiter of atmo_step: enable = 1 with the attached
+ * network, 2 with the closed-form guess of atmo_saturation_newton, 0 not
+ * (with the same fallback to u[7:9] per node) */
 int atmo_rainy_seed_stage_limiter(atmo_handle* handle, int32_t enable);
 
 /* SemidiscretizationHyperbolic(mesh, equations, initial_condition, solver;

@@ -193,3 +193,20 @@ def test_rhs_error_other_forms(oracle, name, over):
     err = _rhs_err(oracle, case, perturbed(case))
     print(f"  {name}: fp32 geometry rel err (term scale)", err, flush=True)
     assert np.all(err <= 1e-6), err
+
+
+@pytest.mark.gpu
+def test_lake_at_rest_stays_balanced(oracle):
+    """bottom topography with FP32 geometry storage: h_s stays FP64 (a row of
+    its own), so h + h_s = const is balanced as in the FP64 form: the rhs of
+    the lake at rest (the well_balanced elixir) vanishes to 1e-12 of the
+    scale of its terms (with h_s rounded to FP32 it was 1.5e-9)"""
+    print("case: well_balanced lake at rest, fp32 geometry", flush=True)
+    case = W.covariant_elixir("well_balanced", cells=4, polydeg=3, precision=GEO)
+    semi = case.semi()
+    u0 = case.u0()
+    du = semi.rhs(None, u0)
+    so = SweOracle.for_semi(oracle, semi)
+    mag = np.abs(so.rhs_magnitude(u0)).max()
+    print(f"  max |du| / term scale {np.abs(du).max() / mag:.2e}", flush=True)
+    assert np.abs(du).max() <= 1e-12 * mag

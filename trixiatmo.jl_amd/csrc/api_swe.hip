@@ -237,7 +237,7 @@ int swe_max_dt(atmo_handle* h, double cfl, double* dt_members) {
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   for (int m = 0; m < h->n_members; ++m) {
     if (!std::isfinite(mx[m])) return fail(&h->err, ATMO_ERR_NONFINITE, "non-finite wave speed");
-    dt_members[m] = cfl * 2.0 / (h->n * mx[m]);
+    dt_members[m] = cfl * (2.0 / (h->n * mx[m]));  // max_dt (stepsize_dg2d.jl:70), then the cfl factor
   }
   return ATMO_OK;
 }
@@ -303,6 +303,23 @@ int atmo_mesh2d_arrays(const atmo_mesh2d* mh, const double** xyz, const double**
 
 void atmo_mesh2d_destroy(atmo_mesh2d* mh) { delete mh; }
 
+int atmo_covariant_aux_device(int polydeg, int32_t n_elements, const double* node_coordinates,
+                              int32_t christoffel_collocation, const double* bottom_topography,
+                              double* aux_node_vars) {
+  if (polydeg < 1 || polydeg > 7 || n_elements < 1 || !node_coordinates || !aux_node_vars)
+    return fail(nullptr, ATMO_ERR_INVALID, "bad arguments");
+  const int n = polydeg + 1;
+  Basis b(n);
+  if (!swe_launch_aux(n, n_elements, b.x.data(), b.D.data(), node_coordinates, bottom_topography,
+                      christoffel_collocation, nullptr, nullptr, aux_node_vars, nullptr))
+    return fail(nullptr, ATMO_ERR_INVALID, "no kernel for this polydeg");
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess)
+    return fail(nullptr, ATMO_ERR_DEVICE, std::string("HIP: ") + hipGetErrorString(e));
+  return ATMO_OK;
+}
+
 int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_handle** out) {
   if (!cfg || !g || !out) return fail(nullptr, ATMO_ERR_INVALID, "null argument");
   if (cfg->polydeg < 1 || cfg->polydeg > 7)
@@ -322,9 +339,13 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
                 "the covariant form runs in FP64, with FP64 or FP32 geometry storage");
   const bool geo32 = cfg->precision == ATMO_PREC_FP32_GEOMETRY;
   if (!(cfg->gravity > 0)) return fail(nullptr, ATMO_ERR_INVALID, "gravity must be positive");
-  if (!g->aux_node_vars || !g->node_coordinates || g->n_interfaces < 0 ||
+  if (!g->node_coordinates || g->n_interfaces < 0 ||
       (g->n_interfaces > 0 && (!g->interface_neighbor_ids || !g->interface_node_indices)))
     return fail(nullptr, ATMO_ERR_INVALID, "missing geometry arrays");
+  // no auxiliary variables given: they are computed on the device from the
+  // node coordinates (init_auxiliary_node_variables!), and so are the stage
+  // kernel's compact rows and face maps
+  const bool aux_on_device = g->aux_node_vars == nullptr;
   const int n = cfg->polydeg + 1, n2 = n * n, E = cfg->n_elements;
   const int M = g->member_first_element ? g->n_members : 1;
   std::vector<int32_t> first(M + 1), member(E);
@@ -370,7 +391,11 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
   // neighbour's momentum enters (contravariant2global then global2contravariant,
   // covariant_shallow_water.jl:140-153, as one 2 x 2 product)
   const size_t NN = (size_t)n2 * E;
-  std::vector<double> geo(kSweGeo * NN), fxf((size_t)16 * n * E);
+  const size_t geo_len = kSweGeo * NN, fxf_len = (size_t)16 * n * E;
+  std::vector<double> geo, fxf;
+  if (!aux_on_device) {
+  geo.assign(geo_len, 0.0);
+  fxf.assign(fxf_len, 0.0);
   for (size_t gq = 0; gq < NN; ++gq) {
     const double* a = g->aux_node_vars + gq * 26;
     for (int c = 0; c < 3; ++c) geo[c * NN + gq] = a[13 + c];
@@ -401,13 +426,27 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
           for (int c = 0; c < 2; ++c)
             M[2 * r + c] = bo[r] * an[3 * c] + bo[r + 2] * an[1 + 3 * c] + bo[r + 4] * an[2 + 3 * c];
       }
+  }
 
-  // FP32 storage: each value rounded to nearest once, here
+  // without bottom topography (h_s = 0 at every node) the stage skips its row
+  int topography = 0;
+  if (!aux_on_device) {
+    for (size_t gq = 0; gq < NN && !topography; ++gq) topography = geo[3 * NN + gq] != 0.0;
+  } else if (g->bottom_topography) {
+    for (size_t gq = 0; gq < NN && !topography; ++gq) topography = g->bottom_topography[gq] != 0.0;
+  }
+
+  // FP32 storage: each value rounded to nearest once, here.  h_s stays FP64
+  // (a row of its own): the lake at rest is balanced by h + h_s = const with
+  // h in FP64, which a rounded h_s would break at the FP32 rounding of h_s
   std::vector<float> geo32v, fxf32v;
-  if (geo32) {
+  std::vector<double> hs64;
+  if (geo32 && !aux_on_device) {
     geo32v.assign(geo.begin(), geo.end());
     fxf32v.assign(fxf.begin(), fxf.end());
+    if (topography) hs64.assign(geo.begin() + 3 * NN, geo.begin() + 4 * NN);
   }
+  const bool need_hs64 = geo32 && topography;
 
   auto* h = new atmo_handle;
   h->eq = ATMO_EQ_COVARIANT_SWE_2D;
@@ -434,10 +473,11 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
       hipMalloc(&h->d_uio, sb) != hipSuccess || hipMalloc(&h->d_duio, sb) != hipSuccess ||
       hipMalloc(&h->d_aux, sizeof(double) * 26 * n2 * (size_t)E) != hipSuccess ||
       hipMalloc(&h->d_xyz, sizeof(double) * 3 * n2 * (size_t)E) != hipSuccess ||
-      (!geo32 && hipMalloc(&h->d_geo, sizeof(double) * geo.size()) != hipSuccess) ||
-      (!geo32 && hipMalloc(&h->d_fxf, sizeof(double) * fxf.size()) != hipSuccess) ||
-      (geo32 && hipMalloc(&h->d_geo32, sizeof(float) * geo.size()) != hipSuccess) ||
-      (geo32 && hipMalloc(&h->d_fxf32, sizeof(float) * fxf.size()) != hipSuccess) ||
+      ((!geo32 || aux_on_device) && hipMalloc(&h->d_geo, sizeof(double) * geo_len) != hipSuccess) ||
+      ((!geo32 || aux_on_device) && hipMalloc(&h->d_fxf, sizeof(double) * fxf_len) != hipSuccess) ||
+      (geo32 && hipMalloc(&h->d_geo32, sizeof(float) * geo_len) != hipSuccess) ||
+      (geo32 && hipMalloc(&h->d_fxf32, sizeof(float) * fxf_len) != hipSuccess) ||
+      (need_hs64 && hipMalloc(&h->d_hs64, sizeof(double) * NN) != hipSuccess) ||
       hipMalloc(&h->d_nbr, sizeof(int32_t) * nbr.size()) != hipSuccess ||
       hipMalloc(&h->d_member, sizeof(int32_t) * E) != hipSuccess ||
       hipMalloc(&h->d_member_first, sizeof(int32_t) * (M + 1)) != hipSuccess ||
@@ -445,18 +485,23 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
       hipMalloc(&h->d_partial, sizeof(double) * 3 * (size_t)E) != hipSuccess ||
       hipMalloc(&h->d_result, sizeof(double) * (M > 16 ? M : 16)) != hipSuccess)
     return cleanup_fail(ATMO_ERR_DEVICE, "device allocation failed");
-  if (hipMemcpy(h->d_aux, g->aux_node_vars, sizeof(double) * 26 * n2 * (size_t)E,
-                hipMemcpyHostToDevice) != hipSuccess ||
+  if ((!aux_on_device && hipMemcpy(h->d_aux, g->aux_node_vars, sizeof(double) * 26 * n2 * (size_t)E,
+                                   hipMemcpyHostToDevice) != hipSuccess) ||
       hipMemcpy(h->d_xyz, g->node_coordinates, sizeof(double) * 3 * n2 * (size_t)E,
                 hipMemcpyHostToDevice) != hipSuccess ||
-      (!geo32 && (hipMemcpy(h->d_geo, geo.data(), sizeof(double) * geo.size(),
-                            hipMemcpyHostToDevice) != hipSuccess ||
-                  hipMemcpy(h->d_fxf, fxf.data(), sizeof(double) * fxf.size(),
-                            hipMemcpyHostToDevice) != hipSuccess)) ||
-      (geo32 && (hipMemcpy(h->d_geo32, geo32v.data(), sizeof(float) * geo32v.size(),
-                           hipMemcpyHostToDevice) != hipSuccess ||
-                 hipMemcpy(h->d_fxf32, fxf32v.data(), sizeof(float) * fxf32v.size(),
-                           hipMemcpyHostToDevice) != hipSuccess)) ||
+      (!aux_on_device && !geo32 &&
+       (hipMemcpy(h->d_geo, geo.data(), sizeof(double) * geo.size(), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(h->d_fxf, fxf.data(), sizeof(double) * fxf.size(), hipMemcpyHostToDevice) !=
+            hipSuccess)) ||
+      (!aux_on_device && geo32 &&
+       (hipMemcpy(h->d_geo32, geo32v.data(), sizeof(float) * geo32v.size(),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->d_fxf32, fxf32v.data(), sizeof(float) * fxf32v.size(),
+                  hipMemcpyHostToDevice) != hipSuccess)) ||
+      (!aux_on_device && need_hs64 &&
+       hipMemcpy(h->d_hs64, hs64.data(), sizeof(double) * hs64.size(), hipMemcpyHostToDevice) !=
+           hipSuccess) ||
       hipMemcpy(h->d_nbr, nbr.data(), sizeof(int32_t) * nbr.size(), hipMemcpyHostToDevice) !=
           hipSuccess ||
       hipMemcpy(h->d_member, member.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice) !=
@@ -467,6 +512,47 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
           hipSuccess ||
       hipMemset(h->d_u, 0, sb) != hipSuccess || hipMemset(h->d_k, 0, sb) != hipSuccess)
     return cleanup_fail(ATMO_ERR_DEVICE, "device upload failed");
+  if (aux_on_device) {
+    // the auxiliary variables, the compact rows and the face maps from the
+    // node coordinates, in FP64; FP32 storage narrows them once, on the device
+    // (after every upload above has landed: the handle's stream does not wait
+    // for the null stream)
+    if (hipDeviceSynchronize() != hipSuccess)
+      return cleanup_fail(ATMO_ERR_DEVICE, "device upload failed");
+    double* d_topo = nullptr;
+    if (topography) {
+      if (hipMalloc(&d_topo, sizeof(double) * NN) != hipSuccess)
+        return cleanup_fail(ATMO_ERR_DEVICE, "device allocation failed");
+      if (hipMemcpy(d_topo, g->bottom_topography, sizeof(double) * NN, hipMemcpyHostToDevice) !=
+          hipSuccess) {
+        (void)hipFree(d_topo);
+        return cleanup_fail(ATMO_ERR_DEVICE, "device upload failed");
+      }
+    }
+    bool ok = swe_launch_aux(n, E, b.x.data(), b.D.data(), h->d_xyz, d_topo,
+                             g->christoffel_collocation, h->d_member, h->d_member_first,
+                             h->d_aux, h->stream) &&
+              swe_launch_face_maps(n, E, h->d_aux, h->d_nbr, h->d_fxf, h->stream);
+    if (ok) {
+      swe_launch_compact_rows(h->d_aux, h->d_xyz, (long long)NN, h->d_geo, h->stream);
+      if (geo32) {
+        launch_narrow_to_float(h->d_geo, h->d_geo32, geo_len, h->stream);
+        launch_narrow_to_float(h->d_fxf, h->d_fxf32, fxf_len, h->stream);
+        if (need_hs64)
+          ok = hipMemcpyAsync(h->d_hs64, h->d_geo + 3 * NN, sizeof(double) * NN,
+                              hipMemcpyDeviceToDevice, h->stream) == hipSuccess;
+      }
+    }
+    ok = ok && hipGetLastError() == hipSuccess && hipStreamSynchronize(h->stream) == hipSuccess;
+    if (d_topo) (void)hipFree(d_topo);
+    if (geo32) {  // the FP64 copies served only to narrow from
+      (void)hipFree(h->d_geo);
+      (void)hipFree(h->d_fxf);
+      h->d_geo = nullptr;
+      h->d_fxf = nullptr;
+    }
+    if (!ok) return cleanup_fail(ATMO_ERR_DEVICE, "auxiliary variables on the device failed");
+  }
   SweLaunch& L = h->swe_base;
   L.n = n;
   L.n_elements = E;
@@ -487,13 +573,12 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
   L.fxf = h->d_fxf;
   L.geo32 = h->d_geo32;
   L.fxf32 = h->d_fxf32;
+  L.hs64 = h->d_hs64;
   L.n_nodes = (long long)NN;
   L.nbr = h->d_nbr;
   L.member = h->d_member;
   L.dt_member = nullptr;
-  // without bottom topography (h_s = 0 at every node) the stage skips its row
-  L.topography = 0;
-  for (size_t gq = 0; gq < NN && !L.topography; ++gq) L.topography = geo[3 * NN + gq] != 0.0;
+  L.topography = topography;
   L.variant = 0;
   *out = h;
   return ATMO_OK;
@@ -501,9 +586,15 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
 
 int atmo_autotune(atmo_handle* h, int32_t* variant, double* ms) {
   if (!h) return fail(nullptr, ATMO_ERR_INVALID, "null handle");
+  if (h->eq == ATMO_EQ_EULER_ENERGY_GRAVITY_3D) {
+    const int rc = e3_autotune(h, ms);
+    if (rc != ATMO_OK) return rc;
+    if (variant) *variant = h->e3_epb - 1;
+    return ATMO_OK;
+  }
   if (h->eq != ATMO_EQ_COVARIANT_SWE_2D) {
     if (variant) *variant = 0;
-    return ATMO_OK;  // the other stage kernels have one shape
+    return ATMO_OK;  // the rainy stage kernel has one shape
   }
   const int rc = swe_autotune(h, ms);
   if (rc != ATMO_OK) return rc;
@@ -513,6 +604,14 @@ int atmo_autotune(atmo_handle* h, int32_t* variant, double* ms) {
 
 int atmo_set_kernel_variant(atmo_handle* h, int32_t variant) {
   if (!h) return fail(nullptr, ATMO_ERR_INVALID, "null handle");
+  if (h->eq == ATMO_EQ_EULER_ENERGY_GRAVITY_3D) {
+    if (variant < 0 || variant > 1 ||
+        (variant == 1 && !e3_has_two_element_groups(h->n, h->cfg.precision)))
+      return fail(&h->err, ATMO_ERR_INVALID, "no such kernel variant");
+    h->e3_epb = variant + 1;
+    h->e3_tuned = true;
+    return ATMO_OK;
+  }
   if (h->eq != ATMO_EQ_COVARIANT_SWE_2D)
     return variant == 0 ? ATMO_OK : fail(&h->err, ATMO_ERR_INVALID, "one shape only");
   if (variant < 0 || variant >= kSweVariants)

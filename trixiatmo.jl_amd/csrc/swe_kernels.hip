@@ -296,7 +296,8 @@ void swe_stage(SweLaunch L) {
       s_n[es][9][q] = c11;
       s_n[es][10][q] = c12;
       s_n[es][11][q] = c22;
-      s_n[es][12][q] = L.topography ? (double)gg[3 * NN] : 0.0;
+      if constexpr (sizeof(GT) == 4) s_n[es][12][q] = L.topography ? L.hs64[g] : 0.0;
+      else s_n[es][12][q] = L.topography ? gg[3 * NN] : 0.0;
     }
     __syncthreads();
 

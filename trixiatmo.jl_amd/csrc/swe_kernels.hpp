@@ -34,6 +34,9 @@ struct SweLaunch {
   // operation stays FP64
   const float* geo32;
   const float* fxf32;
+  // FP32 geometry with bottom topography: h_s [n_nodes] in FP64 (the FP32
+  // row 3 is then not read), NULL otherwise
+  const double* hs64;
   long long n_nodes;
   const int* nbr;       // [E, 4, n] neighbour global node of each face node
   int mode;             // 0 rhs, 1 RK stage
@@ -46,6 +49,25 @@ struct SweLaunch {
   int variant;              // workgroup shape: 0..2 = 1, 2, 4 waves; +3 persistent grid
 };
 constexpr int kSweVariants = 6;
+
+// init_auxiliary_node_variables! on the device (swe_geometry.hip): aux
+// [26,n,n,E] from the node coordinates X [3,n,n,E] (LGL nodes x_lgl[n],
+// derivative matrix D[n*n]; topography [n,n,E] or NULL; collocation: the
+// Christoffel symbols from the collocation derivative instead of the exact
+// metric derivatives; the radius is that of the first node of the element's
+// member, member[e] and member_first[m] device arrays, or of the first node
+// of X when member is NULL); false for n > 8
+bool swe_launch_aux(int n, int n_elements, const double* x_lgl, const double* D, const double* X,
+                    const double* topography, int collocation, const int* member,
+                    const int* member_first, double* aux, hipStream_t s);
+// the stage kernel's compact rows geo [12][nn] from aux and X
+void swe_launch_compact_rows(const double* aux, const double* X, long long nn, double* geo,
+                             hipStream_t s);
+// the face maps fxf [E][4][n][2][2] from aux and the neighbour links nbr [E][4][n]
+bool swe_launch_face_maps(int n, int n_elements, const double* aux, const int* nbr, double* fxf,
+                          hipStream_t s);
+// b[q] = (float) a[q] (euler3d_geometry.hip)
+void launch_narrow_to_float(const double* a, float* b, size_t n, hipStream_t s);
 constexpr int kSweGeo = 12;
 
 // J = sqrt(det G) and G^-1 = adj(G) / det G from the covariant metric tensor

@@ -83,6 +83,8 @@ class AtmoGeometry2D(ctypes.Structure):
         ("interface_node_indices", c_int32_p),
         ("n_members", ctypes.c_int32),
         ("member_first_element", c_int32_p),
+        ("christoffel_collocation", ctypes.c_int32),
+        ("bottom_topography", c_double_p),
     ]
 
 
@@ -169,6 +171,9 @@ SIGNATURES = [
     ("atmo_get_state", ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     ("atmo_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]),
     ("atmo_max_dt", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, c_double_p]),
+    ("atmo_covariant_aux_device", ctypes.c_int, [ctypes.c_int, ctypes.c_int32, ctypes.c_void_p,
+                                                 ctypes.c_int32, ctypes.c_void_p,
+                                                 ctypes.c_void_p]),
     ("atmo_max_dt3d_noalloc", ctypes.c_int, [ctypes.POINTER(AtmoConfig), ctypes.c_void_p,
                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                              ctypes.c_void_p, ctypes.c_void_p]),

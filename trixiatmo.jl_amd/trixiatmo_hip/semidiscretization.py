@@ -99,6 +99,9 @@ class SemidiscretizationHyperbolic:
                  initial_condition=None, metric_terms=None, auxiliary_field=None,
                  metric_terms_on_device=True):
         self.mesh, self.equations, self.solver = mesh, equations, solver
+        # covariant form: the auxiliary variables of the handle are computed on
+        # the device (metric_terms_on_device) or uploaded from the host
+        self.aux_on_device = metric_terms_on_device
         self.initial_condition = initial_condition
         self.source_terms = source_terms
         if isinstance(equations, COVARIANT_EQUATIONS):
@@ -152,16 +155,24 @@ class SemidiscretizationHyperbolic:
         mesh, solver = self.mesh, self.solver
         if solver.polydeg != mesh.polydeg:
             raise ValueError("solver and mesh polydeg differ")
+        # the host copy of the auxiliary variables serves the analysis and the
+        # initial conditions; the handle builds its own on the device from the
+        # node coordinates (aux_on_device) or takes this copy
         self.aux_node_vars = covariant_aux(mesh, metric_terms, auxiliary_field)
         cfg = _swe_config(self.equations, solver, mesh.n_elements, source_terms)
+        mt = metric_terms or MetricTermsCovariantSphere()
         k = [np.ascontiguousarray(self.aux_node_vars.ravel(order="F")),
              np.ascontiguousarray(mesh.node_coordinates.ravel(order="F")),
              np.ascontiguousarray(mesh.interface_neighbor_ids.ravel(order="F"), dtype=np.int32),
-             np.ascontiguousarray(mesh.interface_node_indices.ravel(order="F"), dtype=np.int32)]
-        geo = AtmoGeometry2D(aux_node_vars=dptr(k[0]), node_coordinates=dptr(k[1]),
+             np.ascontiguousarray(mesh.interface_node_indices.ravel(order="F"), dtype=np.int32),
+             np.ascontiguousarray(self.aux_node_vars[19].ravel(order="F"))]
+        geo = AtmoGeometry2D(aux_node_vars=None if self.aux_on_device else dptr(k[0]),
+                             node_coordinates=dptr(k[1]),
                              n_interfaces=mesh.n_interfaces, interface_neighbor_ids=iptr(k[2]),
                              interface_node_indices=iptr(k[3]), n_members=1,
-                             member_first_element=None)
+                             member_first_element=None,
+                             christoffel_collocation=1 if mt.collocation else 0,
+                             bottom_topography=dptr(k[4]) if auxiliary_field is not None else None)
         h = ctypes.c_void_p()
         check(lib().atmo_swe_create(ctypes.byref(cfg), ctypes.byref(geo), ctypes.byref(h)))
         self._h = h
@@ -281,7 +292,8 @@ class SemidiscretizationEnsemble(SemidiscretizationHyperbolic):
     time step.  `members` is a list of (mesh, transformed initial condition,
     auxiliary_field or None)."""
 
-    def __init__(self, members, equations, solver, source_terms=None, metric_terms=None):
+    def __init__(self, members, equations, solver, source_terms=None, metric_terms=None,
+                 metric_terms_on_device=True):
         self.equations, self.solver = equations, solver
         self.source_terms = source_terms
         self.mesh = None
@@ -306,15 +318,21 @@ class SemidiscretizationEnsemble(SemidiscretizationHyperbolic):
         self.member_first_element = np.asarray(first, dtype=np.int32)
         self.n_members = len(members)
         cfg = _swe_config(equations, solver, E, source_terms)
+        mt = metric_terms or MetricTermsCovariantSphere()
         k = [np.ascontiguousarray(self.aux_node_vars.ravel(order="F")),
              np.ascontiguousarray(self.node_coordinates.ravel(order="F")),
              np.ascontiguousarray(nbr.ravel(order="F"), dtype=np.int32),
              np.ascontiguousarray(idx.ravel(order="F"), dtype=np.int32),
-             self.member_first_element]
-        geo = AtmoGeometry2D(aux_node_vars=dptr(k[0]), node_coordinates=dptr(k[1]),
+             self.member_first_element,
+             np.ascontiguousarray(self.aux_node_vars[19].ravel(order="F"))]
+        has_topography = any(field is not None for _, _, field in members)
+        geo = AtmoGeometry2D(aux_node_vars=None if metric_terms_on_device else dptr(k[0]),
+                             node_coordinates=dptr(k[1]),
                              n_interfaces=nbr.shape[1], interface_neighbor_ids=iptr(k[2]),
                              interface_node_indices=iptr(k[3]), n_members=self.n_members,
-                             member_first_element=iptr(k[4]))
+                             member_first_element=iptr(k[4]),
+                             christoffel_collocation=1 if mt.collocation else 0,
+                             bottom_topography=dptr(k[5]) if has_topography else None)
         h = ctypes.c_void_p()
         check(lib().atmo_swe_create(ctypes.byref(cfg), ctypes.byref(geo), ctypes.byref(h)))
         self._h = h
