@@ -115,25 +115,25 @@ class Oracle:
         return out
 
     def flux(self, p, u, n):
-        return self._vec("ora_e3_flux", p, _d(np.asarray(u, float)), _d(np.asarray(n, float)))
+        return self._vec("ora_e3_flux", p, _d(np.ascontiguousarray(u, float)), _d(np.ascontiguousarray(n, float)))
 
     def volume_flux(self, p, which, ul, ur, n):
-        return self._vec("ora_e3_volume_flux", p, which, _d(np.asarray(ul, float)),
-                         _d(np.asarray(ur, float)), _d(np.asarray(n, float)))
+        return self._vec("ora_e3_volume_flux", p, which, _d(np.ascontiguousarray(ul, float)),
+                         _d(np.ascontiguousarray(ur, float)), _d(np.ascontiguousarray(n, float)))
 
     def noncons_flux(self, p, which, ul, ur, n):
-        return self._vec("ora_e3_noncons_flux", p, which, _d(np.asarray(ul, float)),
-                         _d(np.asarray(ur, float)), _d(np.asarray(n, float)))
+        return self._vec("ora_e3_noncons_flux", p, which, _d(np.ascontiguousarray(ul, float)),
+                         _d(np.ascontiguousarray(ur, float)), _d(np.ascontiguousarray(n, float)))
 
     def surface_flux(self, p, which, ul, ur, n):
-        return self._vec("ora_e3_surface_flux", p, which, _d(np.asarray(ul, float)),
-                         _d(np.asarray(ur, float)), _d(np.asarray(n, float)))
+        return self._vec("ora_e3_surface_flux", p, which, _d(np.ascontiguousarray(ul, float)),
+                         _d(np.ascontiguousarray(ur, float)), _d(np.ascontiguousarray(n, float)))
 
     def cons2entropy(self, p, u):
-        return self._vec("ora_e3_cons2entropy", p, _d(np.asarray(u, float)))
+        return self._vec("ora_e3_cons2entropy", p, _d(np.ascontiguousarray(u, float)))
 
     def entropy(self, p, u):
-        return self.L.ora_e3_entropy(ctypes.byref(p), _d(np.asarray(u, float)))
+        return self.L.ora_e3_entropy(ctypes.byref(p), _d(np.ascontiguousarray(u, float)))
 
     def _conn(self, mesh, bc_kinds=None):
         keep = [np.ascontiguousarray(mesh.interface_neighbor_ids.ravel(order="F"), dtype=np.int32),

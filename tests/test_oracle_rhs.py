@@ -122,6 +122,49 @@ def test_entropy_conservation_shell_with_gravity(oracle):
     assert abs(ints[2]) < 1e-12 * mag
 
 
+def _isothermal_state(mesh, eq, rng, temperature_noise=0.0):
+    """seeded density noise, random 1 m/s flow, temperature 250 K (times
+    1 + temperature_noise * N(0, 1) per node), phi from the mesh"""
+    _, _, _, _, u = T.workloads.shell(2, 2, polydeg=3)
+    uf = u.reshape(6, -1, order="F").copy()
+    n = uf.shape[1]
+    rho = uf[0] * (1.0 + 1e-3 * rng.standard_normal(n))
+    temp = 250.0 * (1.0 + temperature_noise * rng.standard_normal(n))
+    v = rng.standard_normal((3, n))
+    uf[0] = rho
+    uf[1:4] = rho * v
+    uf[4] = rho * 287.0 * temp / (eq.gamma - 1.0) + 0.5 * rho * (v ** 2).sum(0)
+    return uf.reshape(u.shape, order="F")
+
+
+@pytest.mark.parametrize("temperature_noise", [0.0, 1e-3])
+def test_entropy_balance_of_the_gravity_term_needs_equal_temperatures(oracle, temperature_noise):
+    """flux_ranocha + flux_nonconservative_waruszewski_etal
+    (compressible_euler_energy_with_gravity_3d.jl:135-159) in the volume and
+    on the surface, no dissipation.  Contracted with the entropy variables
+    w = (.., rho v / p, -rho / p) of node i, the gravity pair term leaves
+    ln_mean(rho) [phi] (rho_i / p_i) (v_i - v_avg) . n, which cancels
+    against its partner only when rho / p, the temperature, is the same at
+    both nodes: with noisy density and flow but one temperature the rate is
+    rounding (1e-18 of the terms), with 0.1 % temperature noise it is not
+    (1e-7).  An arithmetic instead of a logarithmic density mean, or a
+    dropped energy entry, breaks the first case."""
+    mesh, eq, solver, bcs, _ = T.workloads.shell(2, 2, polydeg=3, volume_flux="flux_ranocha",
+                                                 surface_flux="flux_ranocha")
+    u = _isothermal_state(mesh, eq, np.random.default_rng(5), temperature_noise)
+    p = params_for((mesh, eq, solver, 0.0))
+    du = oracle.rhs(p, mesh, u)
+    rate = oracle.integrals(p, mesh, u, du)[2]
+    uf = u.reshape(6, -1, order="F")
+    w = np.stack([oracle.cons2entropy(p, uf[:, q]) for q in range(uf.shape[1])], axis=1)
+    mag = integral(mesh, np.abs(w * du.reshape(6, -1, order="F")).sum(0).reshape(u[0].shape,
+                                                                                 order="F"))
+    if temperature_noise == 0.0:
+        assert abs(rate) < 1e-13 * mag
+    else:
+        assert abs(rate) > 1e-9 * mag
+
+
 # test/test_3d_euler_energy.jl:7-29: elixir_energy_baroclinic_instability.jl
 # with tspan = (0, 0.01) and tol = 1e-15 (atol = 1e-8 on the comparison)
 BAROCLINIC_L2 = [6.540827547514395e-7, 0.0001972937313811121, 0.00042458497371703434,
