@@ -7,8 +7,8 @@ node coordinates alone against handles given the host's auxiliary variables.
 Bars: every row but the exact Christoffel symbols bit for bit (the device
 evaluates the same expressions in the same order with contraction off); the
 collocation Christoffel symbols bit for bit; the exact ones (forward-mode
-dual numbers on the device, complex step on the host) to 1e-14 of each
-row's largest magnitude.  The rhs of a handle built from coordinates equals
+dual numbers on the device, complex step on the host) to 1e-14 of the
+largest magnitude of the six symbols.  The rhs of a handle built from coordinates equals
 the host-aux handle's to 1e-13 of max |du| (the Christoffel symbols enter the
 geometric source linearly)."""
 import ctypes
@@ -53,8 +53,10 @@ def test_device_aux_matches_the_host_builder(polydeg, collocation):
     if collocation:
         np.testing.assert_array_equal(dev[20:], host[20:])
     else:
-        err = np.abs(dev[20:] - host[20:]).max(axis=1) / np.abs(host[20:]).max(axis=1)
-        print("  exact Christoffel rel err per row:", err, flush=True)
+        # some symbols vanish analytically and both builders leave rounding
+        # noise there: the error is measured on the scale of all six rows
+        err = np.abs(dev[20:] - host[20:]).max(axis=1) / np.abs(host[20:]).max()
+        print("  exact Christoffel err per row / max |Gamma|:", err, flush=True)
         assert np.all(err <= 1e-14), err
     assert np.all(device_aux(mesh, collocation)[19] == 0.0)
 

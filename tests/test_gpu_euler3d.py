@@ -5,7 +5,8 @@ on the GPU, and size-independent properties at the full shell size.
 Tolerance: FP64 path <= 1e-12 relative per conserved variable, where the
 error max |gpu - oracle| is divided by max over nodes of sum_terms |term| (the
 oracle's magnitude mode; equal to max |du| when no terms cancel), and in
-addition <= 1e-9 relative to max |du| itself."""
+addition <= 1e-10 relative to max |du| itself (the largest such error of
+these cases is 1.8e-11, on the baroclinic state)."""
 import numpy as np
 import pytest
 
@@ -47,7 +48,7 @@ def check_rhs(oracle, mesh, eq, solver, bcs, u0, omega=0.0, source=None):
     print("  rel err per variable (term scale):", err, flush=True)
     print("  rel err per variable (|du| scale):", plain, flush=True)
     assert np.all(err[:5] <= RHS_RTOL), err
-    assert np.all(plain[:5] <= 1e-9), plain
+    assert np.all(plain[:5] <= 1e-10), plain
     assert np.abs(du_gpu.reshape(6, -1, order="F")[5] -
                   du_ref.reshape(6, -1, order="F")[5]).max() <= 1e-12 * max(1.0, np.abs(du_ref).max())
     return semi, du_gpu, du_ref

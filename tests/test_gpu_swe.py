@@ -5,8 +5,9 @@ integrated on the GPU, and ensembles of uneven members against the same
 members run alone.
 
 Tolerance: rhs <= 1e-12 relative per variable, the error divided by max over
-nodes of sum_terms |term| (the oracle's magnitude mode), and <= 1e-9 relative
-to max |du|; ensembles are bit-identical to the members run alone (the same
+nodes of sum_terms |term| (the oracle's magnitude mode), and <= 1e-12 relative
+to max |du| where the tendency is resolved (the largest such error of these
+cases is 2.9e-14); ensembles are bit-identical to the members run alone (the same
 per-element arithmetic)."""
 import numpy as np
 import pytest
@@ -43,7 +44,7 @@ def check_rhs(oracle, case, u):
     assert np.all(err <= RHS_RTOL), err
     # the |du| criterion applies where du is a resolved value, not rounding
     resolved = np.abs(du_ref.reshape(3, -1, order="F")).max(axis=1) > 1e-10 * mag
-    assert np.all(plain[resolved] <= 1e-9), plain
+    assert np.all(plain[resolved] <= 1e-12), plain
     return semi, so, du_gpu, du_ref
 
 
@@ -226,10 +227,18 @@ def test_autotuner_shapes_and_graph_replay_are_bit_identical():
 
 def test_inconsistent_metric_terms_are_refused(monkeypatch):
     """the stage recomputes J = sqrt(det G) and G^-1 from G: auxiliary
-    variables whose J or G^-1 disagree with their G are refused at creation"""
+    variables given by the caller whose J or G^-1 disagree with their G are
+    refused at creation"""
     from trixiatmo_hip import semidiscretization as S
     case = W.zonal_jet(2, 3, seed=0)
     good = S.covariant_aux
+
+    def host_aux_semi():
+        return S.SemidiscretizationHyperbolic(case.mesh, case.equations, case.solver,
+                                              source_terms=case.source_terms,
+                                              initial_condition=case.initial_condition,
+                                              metric_terms=case.metric_terms,
+                                              metric_terms_on_device=False)
     for row in (12, 16, 17, 18):
         def bad(mesh, metric_terms=None, auxiliary_field=None, row=row):
             aux = good(mesh, metric_terms, auxiliary_field)
@@ -237,6 +246,6 @@ def test_inconsistent_metric_terms_are_refused(monkeypatch):
             return aux
         monkeypatch.setattr(S, "covariant_aux", bad)
         with pytest.raises(T.AtmoError, match="sqrt\\(det G\\)"):
-            case.semi()
+            host_aux_semi()
     monkeypatch.setattr(S, "covariant_aux", good)
-    case.semi()
+    host_aux_semi()

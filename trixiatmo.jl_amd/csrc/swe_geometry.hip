@@ -189,6 +189,7 @@ __global__ __launch_bounds__(64) void swe_aux_kernel(NodeTable nt, const double*
 __global__ void swe_compact_rows_kernel(const double* __restrict__ aux,
                                         const double* __restrict__ X, long long nn,
                                         double* __restrict__ geo) {
+#pragma clang fp contract(off)
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nn) return;
   const double* a = aux + 26 * g;
