@@ -48,26 +48,44 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(mesh, eq, solver, u0, seconds=10.0):
+def cpu_baseline(mesh, eq, solver, u0, du_gpu=None, max_seconds=30.0, min_calls=20):
     """the oracle (FP64 C++ OpenMP restatement of the reference rhs!) on the
-    same workload, timed on this host's cores"""
+    same workload, timed on this host's cores: 5 warm-up calls, then the
+    median of at least `min_calls` calls (more while under `max_seconds`).
+    With the GPU's rhs of the same state (du_gpu) it also reports, as the
+    checker, the GPU's error against the oracle per conserved variable, on
+    the scale of the summed terms and on the scale of max |du|."""
+    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle, params_for
     o = Oracle()
     p = params_for((mesh, eq, solver, 0.0))
-    o.rhs(p, mesh, u0)  # warm-up
-    n_calls, t0 = 0, time.perf_counter()
-    while True:
+    for _ in range(5):
+        ref = o.rhs(p, mesh, u0)
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < min_calls or (time.perf_counter() - t_start < max_seconds and
+                                     len(times) < 200):
+        t0 = time.perf_counter()
         o.rhs(p, mesh, u0)
-        n_calls += 1
-        el = time.perf_counter() - t0
-        if (el > seconds and n_calls >= 20) or n_calls >= 400:
-            break
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
     nodes = mesh.n_elements * mesh.nnodes ** 3
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": nodes * n_calls / el / 1e9, "unit": "GDOF*RHS/s", "cores": cores,
-            "kind": "port",
-            "sample": f"{n_calls} full rhs! calls of the same workload ({el:.1f} s)"}
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out = {"value": nodes / med / 1e9, "unit": "GDOF*RHS/s", "cores": threads,
+           "nproc": os.cpu_count(), "kind": "port",
+           "sample": f"median of {len(times)} full FP64 rhs! calls of the same workload (the FP64 "
+                     f"state after the timed steps) after 5 warm-up calls ({med * 1e3:.1f} ms per "
+                     f"call, {threads} OpenMP threads)"}
+    if du_gpu is not None:
+        mag = o.rhs_magnitude(p, mesh, u0)
+        nv = 6
+        d = np.abs((du_gpu - ref).reshape(nv, -1, order="F")).max(axis=1)
+        term = d / np.maximum(np.abs(mag.reshape(nv, -1, order="F")).max(axis=1), 1e-300)
+        plain = d / np.maximum(np.abs(ref.reshape(nv, -1, order="F")).max(axis=1), 1e-300)
+        out["gpu_fp64_rhs_rel_err_vs_oracle"] = {"term_scale": [float(x) for x in term[:5]],
+                                                  "du_scale": [float(x) for x in plain[:5]]}
+    return out
 
 
 def bench_covariant(steps, warmup, trees=32, polydeg=3, n_members=32):
@@ -151,6 +169,11 @@ def bench_covariant(steps, warmup, trees=32, polydeg=3, n_members=32):
             rec["speedup_vs_fp64"] = out["ensemble"]["avg_launch_us"] / rec["avg_launch_us"]
         out["ensemble" + tag] = rec
         del ens
+    out["precision_note"] = ("fp32_geometry stores the stage's metric terms, Christoffel "
+                             "symbols and face maps in FP32 and computes in FP64; FP32 flux "
+                             "arithmetic is refused for the covariant form (no faster than FP64 "
+                             "on this GPU's non-packed vector units, and the stage is bound by "
+                             "bytes)")
     return out
 
 
@@ -340,6 +363,19 @@ def bench_rainy(steps, warmup, train_seeds=4, mlp_epochs=20, saturated_training=
 
     out["unseeded"], dt = run("unseeded")
     u_unseeded = semi.get_state()
+    # algorithmic HBM bytes of one fused stage per node: read u and the RK
+    # register k, write k and the new state (9 FP64 each); the uniform
+    # Cartesian cells need no per-node geometry
+    stage_bytes = 8 * (9 + 9 + 9 + 9)
+    avg_s = out["unseeded"]["avg_stage_launch_us"] * 1e-6
+    out["roofline"] = {"bound": "hbm", "kernel": "rainy_stage<4>", "bytes_per_node": stage_bytes,
+                       "achieved": stage_bytes * nodes / avg_s / 1e9, "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": stage_bytes * nodes / avg_s / 1e9 / HBM_PEAK_GBS,
+                       "note": "the fused Newton solve of the stage limiter makes this stage "
+                               "VALU-heavy (exp and log per update); unseeded launch"}
+    semi.seed_stage_limiter("closed_form")
+    out["closed_form_seeded"], _ = run("closed_form_seeded")
+    semi.seed_stage_limiter(False)
     # training states: further seeds of this configuration (relative humidity
     # below 1); saturated_training adds a supersaturated variant (relative
     # humidity 1.05), which measured worse on this configuration's solves
@@ -369,11 +405,14 @@ def bench_rainy(steps, warmup, train_seeds=4, mlp_epochs=20, saturated_training=
     st0 = semi.newton(seeded=False)
     semi.set_state(cold.reshape(-1))
     st1 = semi.newton(seeded=True)
+    semi.set_state(cold.reshape(-1))
+    st2 = semi.newton(seeded="closed_form")
     out["cold_solve"] = {"guess": "T = 250 K, rho_vapour = rho_cloud = 0",
                          "default_updates_per_node": st0["updates_per_node"],
                          "seeded_updates_per_node": st1["updates_per_node"],
+                         "closed_form_updates_per_node": st2["updates_per_node"],
                          "seeded_fallbacks": st1["fallbacks"],
-                         "failures": [st0["failures"], st1["failures"]]}
+                         "failures": [st0["failures"], st1["failures"], st2["failures"]]}
     # a saturated case: a bubble at relative humidity 1.1 (a seed not used in
     # training), solved from its own guess (all moisture as vapour) and cold
     semi.set_state(W.rainy_bubble(seed=train_seeds + 2, relative_humidity=1.1)[5])
@@ -389,10 +428,13 @@ def bench_rainy(steps, warmup, train_seeds=4, mlp_epochs=20, saturated_training=
         semi.set_state(v.reshape(-1))
         s1 = semi.newton(seeded=True)
         u_seed = semi.get_state()
+        semi.set_state(v.reshape(-1))
+        s2 = semi.newton(seeded="closed_form")
         res[tag] = {"default_updates_per_node": s0["updates_per_node"],
                     "seeded_updates_per_node": s1["updates_per_node"],
+                    "closed_form_updates_per_node": s2["updates_per_node"],
                     "seeded_fallbacks": s1["fallbacks"],
-                    "failures": [s0["failures"], s1["failures"]],
+                    "failures": [s0["failures"], s1["failures"], s2["failures"]],
                     "state_rel_diff": float(np.abs(u_seed - u_def).max() / np.abs(u_def).max())}
     cloud = semi.get_state().reshape(-1, 9)[:, 7]
     res["cloudy_node_fraction"] = float(np.mean(cloud > 0))
@@ -448,11 +490,16 @@ def main():
     log("building workload")
     semis, results = {}, {}
     dt = None
+    fp64_objects = None
     for prec in ("fp64", "fp32_kahan"):
         mesh, eq, solver, bcs, u0 = T.workloads.shell(a.trees, a.layers, polydeg=a.polydeg, seed=0,
                                                       precision=prec)
         semi = T.SemidiscretizationHyperbolic(mesh, eq, solver, bcs)
         nodes = mesh.n_elements * mesh.nnodes ** 3
+        if prec == "fp64":
+            # the CPU baseline and its parity check run on these (with the
+            # evolved FP64 state, below)
+            fp64_objects = (mesh, eq, solver)
         semi.set_state(u0)
         if dt is None:
             dt = semi.max_dt(0.5)
@@ -483,7 +530,8 @@ def main():
     n_total = a.warmup + a.steps + a.timing_steps
     # RHS error of the mixed form against the FP64 form on the same evolved
     # state (the FP64 state after all steps), and the state error after the run
-    d64 = semis["fp64"].rhs(None, f64["u_end"]).reshape(6, -1, order="F")
+    du_end = semis["fp64"].rhs(None, f64["u_end"])
+    d64 = du_end.reshape(6, -1, order="F")
     dmx = semis["fp32_kahan"].rhs(None, f64["u_end"]).reshape(6, -1, order="F")
     rhs_err = (np.abs(dmx - d64).max(axis=1) / np.abs(d64).max(axis=1))[:5]
     u64 = f64["u_end"].reshape(6, -1, order="F")
@@ -493,11 +541,12 @@ def main():
     change_err = (np.abs(umx - u64).max(axis=1) /
                   np.maximum(np.abs(u64 - u_start).max(axis=1), 1e-300))[:5]
     semis.clear()
-    # algorithmic HBM bytes of one fused stage per node: read u, write u_out,
-    # read + write the RK register k (6 FP64 each), read Ja (9) and 1/J (1);
+    # algorithmic HBM bytes of one fused stage per node: read u (6 FP64),
+    # write u_out, read + write the RK register k (5 FP64 each: phi has no
+    # tendency and is the same in both state buffers), read Ja (9) and 1/J (1);
     # the mixed path reads Ja in FP32
-    bytes_per_node = 8 * (6 + 6 + 6 + 6 + 9 + 1)
-    bytes_mixed = 8 * (6 + 6 + 6 + 6 + 1) + 4 * 9
+    bytes_per_node = 8 * (6 + 5 + 5 + 5 + 9 + 1)
+    bytes_mixed = 8 * (6 + 5 + 5 + 5 + 1) + 4 * 9
     achieved = bytes_per_node * nodes / avg_s / 1e9
     fetch_kib, write_kib, fetch_factor = a.traffic_kib or (STAGE_FETCH_KIB, STAGE_WRITE_KIB,
                                                            FETCH_FACTOR)
@@ -531,7 +580,7 @@ def main():
     cpu = None
     if not a.no_cpu_baseline:
         log("cpu baseline")
-        cpu = cpu_baseline(mesh, eq, solver, u0)
+        cpu = cpu_baseline(*fp64_objects, f64["u_end"], du_end)
 
     out = {
         "metric": METRIC,
@@ -547,12 +596,15 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded isothermal shell with 10 temperature bumps)",
         "config": {"workload": f"dry Euler energy+gravity, cubed-sphere shell 6x{a.trees}^2x{a.layers}, "
-                               f"polydeg {a.polydeg}, KG+Waruszewski volume, LLF+Waruszewski surface, "
-                               "CarpenterKennedy2N54",
+                               f"polydeg {a.polydeg}, flux_kennedy_gruber + Waruszewski volume, "
+                               "FluxPlusDissipation(flux_kennedy_gruber, "
+                               "DissipationLocalLaxFriedrichs()) + Waruszewski surface (the "
+                               "reference's flux_lax_friedrichs reads phi as rho_e and diverges "
+                               "on this shell), slip walls, CarpenterKennedy2N54",
                    "elements": mesh.n_elements, "nodes": nodes, "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "e3_stage_direct<5,1,1,double,false>", "avg_launch_us": avg_s * 1e6,
+                     "kernel": "e3_stage_direct<5,1,1,double,false,1>", "avg_launch_us": avg_s * 1e6,
                      "bytes_per_node": bytes_per_node, "traffic_bytes_per_node": traffic / nodes},
         "valu": {"fp64_executed_tflops": valu64,
                  "frac_of_fp64_vector_peak": valu64 / FP64_VECTOR_PEAK_TFLOPS,
