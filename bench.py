@@ -48,7 +48,8 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(mesh, eq, solver, u0, du_gpu=None, max_seconds=30.0, min_calls=20):
+def cpu_baseline(mesh, eq, solver, u0, du_gpu=None, max_seconds=30.0, min_calls=20,
+                 others=None):
     """the oracle (FP64 C++ OpenMP restatement of the reference rhs!) on the
     same workload, timed on this host's cores: 5 warm-up calls, then the
     median of at least `min_calls` calls (more while under `max_seconds`).
@@ -85,6 +86,13 @@ def cpu_baseline(mesh, eq, solver, u0, du_gpu=None, max_seconds=30.0, min_calls=
         plain = d / np.maximum(np.abs(ref.reshape(nv, -1, order="F")).max(axis=1), 1e-300)
         out["gpu_fp64_rhs_rel_err_vs_oracle"] = {"term_scale": [float(x) for x in term[:5]],
                                                   "du_scale": [float(x) for x in plain[:5]]}
+        for name, du_other in (others or {}).items():
+            d = np.abs((du_other - ref).reshape(nv, -1, order="F")).max(axis=1)
+            out[f"gpu_{name}_rhs_rel_err_vs_oracle"] = {
+                "term_scale": [float(x) for x in (d / np.maximum(np.abs(
+                    mag.reshape(nv, -1, order="F")).max(axis=1), 1e-300))[:5]],
+                "du_scale": [float(x) for x in (d / np.maximum(np.abs(
+                    ref.reshape(nv, -1, order="F")).max(axis=1), 1e-300))[:5]]}
     return out
 
 
@@ -491,7 +499,7 @@ def main():
     semis, results = {}, {}
     dt = None
     fp64_objects = None
-    for prec in ("fp64", "fp32_kahan"):
+    for prec in ("fp64", "fp32_kahan", "fp32_geometry"):
         mesh, eq, solver, bcs, u0 = T.workloads.shell(a.trees, a.layers, polydeg=a.polydeg, seed=0,
                                                       precision=prec)
         semi = T.SemidiscretizationHyperbolic(mesh, eq, solver, bcs)
@@ -533,6 +541,11 @@ def main():
     du_end = semis["fp64"].rhs(None, f64["u_end"])
     d64 = du_end.reshape(6, -1, order="F")
     dmx = semis["fp32_kahan"].rhs(None, f64["u_end"]).reshape(6, -1, order="F")
+    du_geo = semis["fp32_geometry"].rhs(None, f64["u_end"])
+    geo = results["fp32_geometry"]
+    geo_err = (np.abs(du_geo.reshape(6, -1, order="F") - d64).max(axis=1) /
+               np.abs(d64).max(axis=1))[:5]
+    ugeo = geo["u_end"].reshape(6, -1, order="F")
     rhs_err = (np.abs(dmx - d64).max(axis=1) / np.abs(d64).max(axis=1))[:5]
     u64 = f64["u_end"].reshape(6, -1, order="F")
     umx = mix["u_end"].reshape(6, -1, order="F")
@@ -547,6 +560,7 @@ def main():
     # the mixed path reads Ja in FP32
     bytes_per_node = 8 * (6 + 5 + 5 + 5 + 9 + 1)
     bytes_mixed = 8 * (6 + 5 + 5 + 5 + 1) + 4 * 9
+    bytes_geo = bytes_mixed
     achieved = bytes_per_node * nodes / avg_s / 1e9
     fetch_kib, write_kib, fetch_factor = a.traffic_kib or (STAGE_FETCH_KIB, STAGE_WRITE_KIB,
                                                            FETCH_FACTOR)
@@ -580,7 +594,8 @@ def main():
     cpu = None
     if not a.no_cpu_baseline:
         log("cpu baseline")
-        cpu = cpu_baseline(*fp64_objects, f64["u_end"], du_end)
+        cpu = cpu_baseline(*fp64_objects, f64["u_end"], du_end,
+                           others={"fp32_geometry": du_geo})
 
     out = {
         "metric": METRIC,
@@ -625,6 +640,21 @@ def main():
                   "state_err_rel_to_change": [float(x) for x in change_err],
                   "steps_compared": n_total,
                   "mass_drift": mix["mass_drift"], "entropy_drift": mix["entropy_drift"]},
+        "fp32_geometry": {"precision": "fp64 arithmetic, contravariant vectors stored in fp32",
+                          "value": geo["gdof"], "speedup_vs_fp64": geo["gdof"] / gdof,
+                          "ms_per_step": geo["el"] / a.steps * 1e3,
+                          "avg_launch_us": geo["avg_s"] * 1e6,
+                          "bytes_per_node": bytes_geo,
+                          "hbm_achieved_gbs": bytes_geo * nodes / geo["avg_s"] / 1e9,
+                          "rhs_rel_err_vs_fp64_per_var": [float(x) for x in geo_err],
+                          "state_rel_err_vs_fp64_after_steps":
+                              [float(x) for x in (np.abs(ugeo - u64).max(axis=1) /
+                                                  np.abs(u64).max(axis=1))[:5]],
+                          "state_err_rel_to_change":
+                              [float(x) for x in (np.abs(ugeo - u64).max(axis=1) /
+                                                  np.maximum(np.abs(u64 - u_start).max(axis=1),
+                                                             1e-300))[:5]],
+                          "mass_drift": geo["mass_drift"], "entropy_drift": geo["entropy_drift"]},
         "fp64_drift": {"mass": f64["mass_drift"], "entropy": f64["entropy_drift"],
                        "steps": n_total},
         "covariant_swe": cov,

@@ -131,18 +131,16 @@ E3Launch base_launch(atmo_handle* h) {
 
 bool launch_stage_only(atmo_handle* h, const E3Launch& L) {
   const int prec = h->cfg.precision;
-  return prec == ATMO_PREC_FP64         ? e3_launch_stage_fp64(L, h->stream)
-         : prec == ATMO_PREC_FP32_KAHAN ? e3_launch_stage_fp32_kahan(L, h->stream)
-                                        : e3_launch_stage_fp32(L, h->stream);
+  return prec == ATMO_PREC_FP64            ? e3_launch_stage_fp64(L, h->stream)
+         : prec == ATMO_PREC_FP32_GEOMETRY ? e3_launch_stage_fp64_geo32(L, h->stream)
+         : prec == ATMO_PREC_FP32_KAHAN    ? e3_launch_stage_fp32_kahan(L, h->stream)
+                                           : e3_launch_stage_fp32(L, h->stream);
 }
 
 int launch_stage(atmo_handle* h, const E3Launch& L) {
   TimerPair tp;
   if (h->timing) timer_begin(h, 0, &tp);
-  const int prec = h->cfg.precision;
-  const bool ok = prec == ATMO_PREC_FP64         ? e3_launch_stage_fp64(L, h->stream)
-                  : prec == ATMO_PREC_FP32_KAHAN ? e3_launch_stage_fp32_kahan(L, h->stream)
-                                                 : e3_launch_stage_fp32(L, h->stream);
+  const bool ok = launch_stage_only(h, L);
   if (h->timing) timer_end(h, &tp);
   if (!ok) return fail(&h->err, ATMO_ERR_INVALID, "no kernel for this flux/polydeg combination");
   HIP_TRY(h, hipGetLastError());
@@ -448,7 +446,7 @@ int atmo_create(const atmo_config* cfg, const atmo_geometry3d* g, atmo_handle** 
     return fail(nullptr, ATMO_ERR_INVALID, "unsupported nonconservative flux");
   if (cfg->surface_flux < 0 || cfg->surface_flux > 5)
     return fail(nullptr, ATMO_ERR_INVALID, "unsupported surface flux");
-  if (cfg->precision < ATMO_PREC_FP64 || cfg->precision > ATMO_PREC_FP32)
+  if (cfg->precision < ATMO_PREC_FP64 || cfg->precision > ATMO_PREC_FP32_GEOMETRY)
     return fail(nullptr, ATMO_ERR_INVALID, "unsupported precision");
   if (!(cfg->c_p > cfg->c_v) || !(cfg->c_v > 0))
     return fail(nullptr, ATMO_ERR_INVALID, "need c_p > c_v > 0");
@@ -524,7 +522,9 @@ int atmo_create(const atmo_config* cfg, const atmo_geometry3d* g, atmo_handle** 
   h->n = n;
   h->n3 = n3;
   h->len = (int64_t)6 * n3 * E;
-  h->bg_on = cfg->precision != ATMO_PREC_FP64 && cfg->background_p0 > 0 && cfg->background_T0 > 0;
+  // the hydrostatic background split serves the FP32 flux forms only
+  h->bg_on = (cfg->precision == ATMO_PREC_FP32_KAHAN || cfg->precision == ATMO_PREC_FP32) &&
+             cfg->background_p0 > 0 && cfg->background_T0 > 0;
   auto cleanup_fail = [&](int code) {
     atmo_destroy(h);
     return code;

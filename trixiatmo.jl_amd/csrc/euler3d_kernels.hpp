@@ -31,7 +31,7 @@ struct E3Launch {
   int mode;  // 0 rhs, 1 RK stage
   const double* u;
   double* out;
-  double* k;
+  double* k;              // RK register [5, nodes] (phi has no tendency)
   const double* cv;       // FP64 contravariant vectors (reductions)
   const void* cv_t;       // contravariant vectors in the stage kernel's compute type
   const double* invJ;
@@ -57,6 +57,8 @@ void e3_launch_background_state(const E3Launch& L, const double* u, double* ubg,
 void e3_launch_extract_momentum(const double* du, float* B, long n_nodes, hipStream_t s);
 
 bool e3_launch_stage_fp64(const E3Launch& L, hipStream_t s);
+// FP64 arithmetic, contravariant vectors read from the FP32 copy (cv_t)
+bool e3_launch_stage_fp64_geo32(const E3Launch& L, hipStream_t s);
 bool e3_launch_stage_fp32_kahan(const E3Launch& L, hipStream_t s);
 bool e3_launch_stage_fp32(const E3Launch& L, hipStream_t s);
 // which 0: per-element max scaled speed -> result[0] = max

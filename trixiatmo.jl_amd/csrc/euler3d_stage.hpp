@@ -28,6 +28,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "euler3d_kernels.hpp"
 #include "euler3d_physics.hpp"
 #include "speed_max.hpp"
@@ -379,9 +381,9 @@ struct FaceNormals {
   T n[FacePrefetch<N, EPB>::PASSES][3];
 };
 
-template <int N, typename T, int EPB = 1>
+template <int N, typename T, int EPB = 1, typename G = T>
 __device__ __forceinline__ void face_normals(FaceNormals<N, T, EPB>& fn, int e0, int n_valid, int t,
-                                             const T* __restrict__ cv, const T* s_cv,
+                                             const G* __restrict__ cv, const T* s_cv,
                                              int cv_stride, const FacePrefetch<N, EPB>& pf,
                                              const int* __restrict__ nbr,
                                              const int* __restrict__ face_info) {
@@ -401,13 +403,13 @@ __device__ __forceinline__ void face_normals(FaceNormals<N, T, EPB>& fn, int e0,
 #pragma unroll
       for (int c = 0; c < 3; ++c)
         fn.n[ps][c] = sgn * (s_cv ? s_cv[es * cv_stride + (3 * d + c) * N3 + node]
-                                  : cv[(e * N3 + node) * 9 + 3 * d + c]);
+                                  : T(cv[(e * N3 + node) * 9 + 3 * d + c]));
     } else {
       const int dp = info & 7;
       const T sgn = (dp % 2 == 0) ? T(1) : T(-1);  // minus the primary's outward normal
-      const T* jp = cv + (size_t)link * 9 + 3 * (dp / 2);
+      const G* jp = cv + (size_t)link * 9 + 3 * (dp / 2);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) fn.n[ps][c] = sgn * jp[c];
+      for (int c = 0; c < 3; ++c) fn.n[ps][c] = sgn * T(jp[c]);
     }
   }
 }
@@ -634,9 +636,10 @@ __device__ __forceinline__ void stage_epilogue(
   // calc_sources!: source_terms_coriolis (-2 Omega e_z x rho v), after the Jacobian
   du[1] += coriolis * uo[2];
   du[2] += -coriolis * uo[1];
-  // phi has no tendency: in an RK stage its register stays 0 and its state
-  // entry is the same in both state buffers (copied when the state is set),
-  // so the update leaves both untouched
+  // phi has no tendency: its state entry is the same in both state buffers
+  // (copied when the state is set), so an RK stage leaves it untouched, and
+  // the RK register holds the other five variables only ([5, nodes]: whole
+  // cache lines, 16 B per node less traffic than a phi row kept at zero)
   un[NV - 1] = uo[NV - 1];
   if (mode == 0) {
 #pragma unroll
@@ -644,8 +647,8 @@ __device__ __forceinline__ void stage_epilogue(
   } else {
 #pragma unroll
     for (int v = 0; v < NV - 1; ++v) {
-      const double kn = rk_a * kreg[gnode * NV + v] + rk_dt * du[v];
-      kreg[gnode * NV + v] = kn;
+      const double kn = rk_a * kreg[gnode * (NV - 1) + v] + rk_dt * du[v];
+      kreg[gnode * (NV - 1) + v] = kn;
       un[v] = uo[v] + rk_b * kn;
       out[gnode * NV + v] = un[v];
     }
@@ -817,11 +820,13 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
 // pair-record form's exactly; in exchange for twice the flux arithmetic the
 // volume phase needs no pair buffer and no barrier, which leaves LDS for more
 // resident workgroups.
-template <int N, int VOL, int NC, typename T, bool KAHAN, int EPB = 1>
+// G: storage type of the contravariant vectors in memory (T, or float for the
+// FP64 form with FP32 geometry storage: widened to T where they are read)
+template <int N, int VOL, int NC, typename T, bool KAHAN, int EPB = 1, typename G = T>
 __global__ __launch_bounds__(EPB * Shape<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 void e3_stage_direct(
     E3Params q, DsplitTable dst, double inv_w, int n_elements, const double* __restrict__ u,
-    double* __restrict__ out, double* __restrict__ kreg, const T* __restrict__ cv,
+    double* __restrict__ out, double* __restrict__ kreg, const G* __restrict__ cv,
     const double* __restrict__ invJ, const int* __restrict__ nbr,
     const int* __restrict__ face_info, int surf, int snc, int mode, double rk_a, double rk_dt,
     ResidentLink res, double rk_b, double coriolis, Background bg) {
@@ -861,7 +866,7 @@ void e3_stage_direct(
   if (active) {
     node_fields_to_lds<N, VOL, NC, T>(q, u, gnode, tl, s_w, s_phi, s_pb, bg);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) s_buf[c * N3 + tl] = cv[gnode * 9 + c];
+    for (int c = 0; c < 9; ++c) s_buf[c * N3 + tl] = T(cv[gnode * 9 + c]);
   }
   FacePrefetch<N, EPB> pf;
   face_prefetch<N, EPB>(pf, e0, n_valid, t, u, nbr, face_info);
@@ -958,7 +963,7 @@ void e3_stage_direct(
     }
   }
   FaceNormals<N, T, EPB> fn;
-  face_normals<N, T, EPB>(fn, e0, n_valid, t, cv, s_buf_all[0], BUF, pf, nbr, face_info);
+  face_normals<N, T, EPB, G>(fn, e0, n_valid, t, cv, s_buf_all[0], BUF, pf, nbr, face_info);
   __syncthreads();  // contravariant vectors consumed; the buffer takes the face fluxes
   face_fluxes_to_lds<N, T, EPB>(q, e0, n_valid, t, inv_w, u, pf, fn, nbr, face_info, surf, snc, bg,
                                 s_w_all[0], NW * N3, s_pb_all[0], s_buf_all[0], BUF);
@@ -992,30 +997,33 @@ constexpr bool has_two_element_groups() {
   return (N == 4 || N == 5) && use_direct_form<N, 1, T>();
 }
 
-template <int N, int VOL, int NC, typename T, bool KAHAN, int EPB>
+template <int N, int VOL, int NC, typename T, bool KAHAN, int EPB, typename G>
 void launch_direct(const E3Launch& L, const DsplitTable& dst, hipStream_t s) {
   const int n_groups = (L.n_elements + EPB - 1) / EPB;
   const int grid = 8 * ((n_groups + 7) / 8);
-  hipLaunchKernelGGL((e3_stage_direct<N, VOL, NC, T, KAHAN, EPB>), dim3(grid),
+  hipLaunchKernelGGL((e3_stage_direct<N, VOL, NC, T, KAHAN, EPB, G>), dim3(grid),
                      dim3(EPB * Shape<N>::BLOCK), 0, s, L.q, dst, L.inv_w, L.n_elements, L.u,
-                     L.out, L.k, reinterpret_cast<const T*>(L.cv_t), L.invJ, L.nbr, L.face_info,
+                     L.out, L.k, reinterpret_cast<const G*>(L.cv_t), L.invJ, L.nbr, L.face_info,
                      L.surface_flux, L.surface_noncons, L.mode, L.rk_a, L.rk_dt, L.res, L.rk_b,
                      L.coriolis, L.bg);
 }
 
-template <int N, int VOL, int NC, typename T, bool KAHAN>
+// G != T (FP32 geometry storage of the FP64 form) exists in the direct form only
+template <int N, int VOL, int NC, typename T, bool KAHAN, typename G = T>
 void launch_stage_inst(const E3Launch& L, hipStream_t s) {
   DsplitTable dst;
   for (int q = 0; q < N * N; ++q) dst.ds[q] = L.dsplit[q];
   const int grid = 8 * ((L.n_elements + 7) / 8);
+  static_assert(std::is_same<G, T>::value || use_direct_form<N, VOL, T>(),
+                "geometry storage other than T needs the direct form");
   if constexpr (use_direct_form<N, VOL, T>()) {
     if constexpr (has_two_element_groups<N, T>()) {
       if (L.epb == 2) {
-        launch_direct<N, VOL, NC, T, KAHAN, 2>(L, dst, s);
+        launch_direct<N, VOL, NC, T, KAHAN, 2, G>(L, dst, s);
         return;
       }
     }
-    launch_direct<N, VOL, NC, T, KAHAN, 1>(L, dst, s);
+    launch_direct<N, VOL, NC, T, KAHAN, 1, G>(L, dst, s);
   } else {
     hipLaunchKernelGGL((e3_stage<N, VOL, NC, T, KAHAN>), dim3(grid), dim3(Shape<N>::BLOCK), 0, s,
                        L.q, dst, L.inv_w, L.n_elements, L.u, L.out, L.k,
@@ -1025,36 +1033,36 @@ void launch_stage_inst(const E3Launch& L, hipStream_t s) {
   }
 }
 
-template <typename T, bool KAHAN, int N, int VOL>
+template <typename T, bool KAHAN, int N, int VOL, typename G = T>
 bool dispatch_nc(const E3Launch& L, hipStream_t s) {
   switch (L.volume_noncons) {
-    case 0: launch_stage_inst<N, VOL, 0, T, KAHAN>(L, s); return true;
-    case 1: launch_stage_inst<N, VOL, 1, T, KAHAN>(L, s); return true;
-    case 2: launch_stage_inst<N, VOL, 2, T, KAHAN>(L, s); return true;
+    case 0: launch_stage_inst<N, VOL, 0, T, KAHAN, G>(L, s); return true;
+    case 1: launch_stage_inst<N, VOL, 1, T, KAHAN, G>(L, s); return true;
+    case 2: launch_stage_inst<N, VOL, 2, T, KAHAN, G>(L, s); return true;
   }
   return false;
 }
 
-template <typename T, bool KAHAN, int N>
+template <typename T, bool KAHAN, int N, typename G = T>
 bool dispatch_vol(const E3Launch& L, hipStream_t s) {
   switch (L.volume_flux) {
-    case 0: return dispatch_nc<T, KAHAN, N, 0>(L, s);
-    case 1: return dispatch_nc<T, KAHAN, N, 1>(L, s);
-    case 2: return dispatch_nc<T, KAHAN, N, 2>(L, s);
+    case 0: return dispatch_nc<T, KAHAN, N, 0, G>(L, s);
+    case 1: return dispatch_nc<T, KAHAN, N, 1, G>(L, s);
+    case 2: return dispatch_nc<T, KAHAN, N, 2, G>(L, s);
   }
   return false;
 }
 
-template <typename T, bool KAHAN>
+template <typename T, bool KAHAN, typename G = T>
 bool dispatch_stage(const E3Launch& L, hipStream_t s) {
   switch (L.n) {
-    case 2: return dispatch_vol<T, KAHAN, 2>(L, s);
-    case 3: return dispatch_vol<T, KAHAN, 3>(L, s);
-    case 4: return dispatch_vol<T, KAHAN, 4>(L, s);
-    case 5: return dispatch_vol<T, KAHAN, 5>(L, s);
-    case 6: return dispatch_vol<T, KAHAN, 6>(L, s);
-    case 7: return dispatch_vol<T, KAHAN, 7>(L, s);
-    case 8: return dispatch_vol<T, KAHAN, 8>(L, s);
+    case 2: return dispatch_vol<T, KAHAN, 2, G>(L, s);
+    case 3: return dispatch_vol<T, KAHAN, 3, G>(L, s);
+    case 4: return dispatch_vol<T, KAHAN, 4, G>(L, s);
+    case 5: return dispatch_vol<T, KAHAN, 5, G>(L, s);
+    case 6: return dispatch_vol<T, KAHAN, 6, G>(L, s);
+    case 7: return dispatch_vol<T, KAHAN, 7, G>(L, s);
+    case 8: return dispatch_vol<T, KAHAN, 8, G>(L, s);
   }
   return false;
 }

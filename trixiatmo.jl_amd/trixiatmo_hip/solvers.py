@@ -24,8 +24,10 @@ class DGSEM:
 
     `precision` selects the arithmetic of the flux evaluations and volume sums
     ("fp64", "fp32_kahan" or "fp32"); RK registers stay FP64 in every form.
-    The covariant form takes "fp64" or "fp32_geometry" (its metric terms,
-    Christoffel symbols and face maps stored in FP32, every operation FP64).
+    "fp32_geometry" keeps every operation in FP64 and stores the metric terms
+    in FP32: the 3D contravariant vectors, or the covariant form's metric
+    terms, Christoffel symbols and face maps. The covariant form takes "fp64"
+    or "fp32_geometry".
     """
     polydeg: int
     surface_flux: Union[Tuple[Union[str, FluxLMARS], str], str]

@@ -19,7 +19,8 @@ def periodic_cube(trees=4, polydeg=3, seed=0, precision="fp64"):
     solver = DGSEM(polydeg=polydeg, surface_flux=("flux_lax_friedrichs", "flux_zero"),
                    volume_integral=VolumeIntegralFluxDifferencing(("flux_ranocha", "flux_zero")),
                    precision=precision,
-                   hydrostatic_background=None if precision == "fp64" else (1.0, 1.0 / eq.R))
+                   hydrostatic_background=None if precision in ("fp64", "fp32_geometry")
+                   else (1.0, 1.0 / eq.R))
     u0 = initial_condition_periodic_modes(mesh, eq, seed)
     return mesh, eq, solver, {}, u0
 
@@ -47,7 +48,8 @@ def shell(trees_per_face=16, layers=8, polydeg=4, seed=0, precision="fp64",
                    volume_integral=VolumeIntegralFluxDifferencing(
                        (volume_flux, "flux_nonconservative_waruszewski_etal")),
                    precision=precision,
-                   hydrostatic_background=None if precision == "fp64" else (eq.p_0, 250.0))
+                   hydrostatic_background=None if precision in ("fp64", "fp32_geometry")
+                   else (eq.p_0, 250.0))
     u0 = initial_condition_isothermal_perturbed(mesh, eq, seed, inner_radius=SHELL_RADIUS)
     bcs = {"inside": "boundary_condition_slip_wall", "outside": "boundary_condition_slip_wall"}
     return mesh, eq, solver, bcs, u0
