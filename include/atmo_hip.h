@@ -57,7 +57,8 @@ extern "C" {
 #define ATMO_PREC_FP64 0          /* everything in FP64 */
 #define ATMO_PREC_FP32_KAHAN 1    /* FP32 fluxes, Kahan-compensated FP32 sums */
 #define ATMO_PREC_FP32 2          /* FP32 fluxes, plain FP32 sums */
-#define ATMO_PREC_FP32_GEOMETRY 3 /* covariant form: metric terms and face maps
+#define ATMO_PREC_FP32_GEOMETRY 3 /* metric terms (3D: contravariant vectors;
+                                     covariant form: metric terms and face maps)
                                      stored in FP32, all arithmetic FP64 */
 
 /* node-local source terms (added after the Jacobian, as calc_sources!) */
@@ -120,6 +121,26 @@ int atmo_mesh3d_arrays(const atmo_mesh3d* mesh, const double** node_coordinates,
                        const int32_t** boundary_kind);
 void atmo_mesh3d_destroy(atmo_mesh3d* mesh);
 
+/* The face connectivity of a P4estMesh{3} built on the device from its node
+ * coordinates alone (p4est's connectivity and Trixi's
+ * init_interfaces! / init_boundaries! [ext];
+ * src/meshes/p4est_cubed_sphere_topography.jl:271-594 for the shell): faces
+ * matched by their quantised centres (radix sort), the primary side the
+ * element face that sorts first, the secondary side's node-index codes the
+ * orientation under which its nodes coincide; lists in the order of
+ * atmo_mesh3d_arrays, equal to them entry for entry.  period[3] gives the
+ * period of each axis (0 or NULL: not periodic).  node_coordinates
+ * [3,n,n,n,E] and the outputs are device pointers sized for the maxima
+ * (interfaces <= 3E: neighbor ids [2,3E], node indices [3,2,3E]; boundaries
+ * <= 6E); the counts come back in host memory.  ATMO_ERR_MESH when a face
+ * matches more than one other or no orientation fits.  Complete on return. */
+int atmo_mesh3d_connectivity_device(int polydeg, int32_t n_elements,
+                                    const double* node_coordinates, const double* period,
+                                    int32_t* interface_neighbor_ids,
+                                    int32_t* interface_node_indices, int32_t* boundary_element,
+                                    int32_t* boundary_node_indices, int32_t* boundary_kind,
+                                    int32_t* n_interfaces, int32_t* n_boundaries);
+
 /* 2D cubed sphere for the covariant form */
 typedef struct atmo_mesh2d atmo_mesh2d;
 
@@ -141,6 +162,17 @@ int atmo_mesh2d_arrays(const atmo_mesh2d* mesh, const double** node_coordinates,
                        const double** aux_node_vars, const int32_t** interface_neighbor_ids,
                        const int32_t** interface_node_indices);
 void atmo_mesh2d_destroy(atmo_mesh2d* mesh);
+
+/* The edge connectivity of P4estMeshCubedSphere2D built on the device from
+ * its node coordinates [3,n,n,E] (src/meshes/p4est_cubed_sphere.jl:82-299
+ * with p4est's connectivity): equal entry for entry to atmo_mesh2d_arrays'
+ * interface lists (interfaces = 2E on the closed sphere: neighbor ids
+ * [2,2E], node indices [2,2,2E], device pointers); ATMO_ERR_MESH when an
+ * edge has no partner. Complete on return. */
+int atmo_mesh2d_connectivity_device(int polydeg, int32_t n_elements,
+                                    const double* node_coordinates,
+                                    int32_t* interface_neighbor_ids,
+                                    int32_t* interface_node_indices, int32_t* n_interfaces);
 
 /* ---------------------------------------------- semidiscretization */
 typedef struct atmo_handle atmo_handle;

@@ -171,6 +171,12 @@ SIGNATURES = [
     ("atmo_get_state", ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     ("atmo_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]),
     ("atmo_max_dt", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, c_double_p]),
+    ("atmo_mesh3d_connectivity_device", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_int32, ctypes.c_void_p, c_double_p, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_int32_p, c_int32_p]),
+    ("atmo_mesh2d_connectivity_device", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+      c_int32_p]),
     ("atmo_covariant_aux_device", ctypes.c_int, [ctypes.c_int, ctypes.c_int32, ctypes.c_void_p,
                                                  ctypes.c_int32, ctypes.c_void_p,
                                                  ctypes.c_void_p]),

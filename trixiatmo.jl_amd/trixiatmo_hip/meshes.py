@@ -101,7 +101,9 @@ def P4estMesh(trees_per_dimension, *, polydeg, coordinates_min, coordinates_max,
     check(lib().atmo_mesh3d_cartesian(polydeg, t.ctypes.data_as(c_int32_p),
                                       lo.ctypes.data_as(c_double_p), hi.ctypes.data_as(c_double_p),
                                       per.ctypes.data_as(c_int32_p), ctypes.byref(h)))
-    return P4estMesh3D(h, polydeg)
+    mesh = P4estMesh3D(h, polydeg)
+    mesh.period = np.where(per != 0, hi - lo, 0.0)
+    return mesh
 
 
 def P4estMeshCubedSphereTopography(trees_per_face_dimension, layers, inner_radius, thickness,
@@ -113,6 +115,40 @@ def P4estMeshCubedSphereTopography(trees_per_face_dimension, layers, inner_radiu
                                                float(inner_radius), float(thickness),
                                                ctypes.byref(h)))
     return P4estMesh3D(h, polydeg)
+
+
+def device_connectivity(mesh):
+    """The face connectivity of `mesh` (P4estMesh3D or P4estMeshCubedSphere2D)
+    built on the device from its node coordinates alone
+    (atmo_mesh3d_connectivity_device / atmo_mesh2d_connectivity_device): a
+    dict of the interface (and, in 3D, boundary) arrays in the layout of the
+    mesh's own attributes."""
+    import torch
+    E, n = mesh.n_elements, mesh.nnodes
+    X = torch.from_numpy(np.ascontiguousarray(mesh.node_coordinates.ravel(order="F"))).cuda()
+    ni, nb = ctypes.c_int32(0), ctypes.c_int32(0)
+    i32 = dict(dtype=torch.int32, device="cuda")
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+    if isinstance(mesh, P4estMesh3D):
+        ids, codes = torch.full((2 * 3 * E,), -1, **i32), torch.full((6 * 3 * E,), -1, **i32)
+        be, bc, bk = (torch.full((6 * E,), -1, **i32), torch.full((3 * 6 * E,), -1, **i32),
+                      torch.full((6 * E,), -1, **i32))
+        period = np.ascontiguousarray(getattr(mesh, "period", np.zeros(3)), dtype=np.float64)
+        check(lib().atmo_mesh3d_connectivity_device(
+            mesh.polydeg, E, ptr(X), period.ctypes.data_as(c_double_p), ptr(ids), ptr(codes),
+            ptr(be), ptr(bc), ptr(bk), ctypes.byref(ni), ctypes.byref(nb)))
+        I, B = ni.value, nb.value
+        return {"interface_neighbor_ids": ids.cpu().numpy()[:2 * I].reshape(2, I, order="F"),
+                "interface_node_indices": codes.cpu().numpy()[:6 * I].reshape(3, 2, I, order="F"),
+                "boundary_element": be.cpu().numpy()[:B],
+                "boundary_node_indices": bc.cpu().numpy()[:3 * B].reshape(3, B, order="F"),
+                "boundary_kind": bk.cpu().numpy()[:B]}
+    ids, codes = torch.full((2 * 2 * E,), -1, **i32), torch.full((4 * 2 * E,), -1, **i32)
+    check(lib().atmo_mesh2d_connectivity_device(mesh.polydeg, E, ptr(X), ptr(ids), ptr(codes),
+                                                ctypes.byref(ni)))
+    I = ni.value
+    return {"interface_neighbor_ids": ids.cpu().numpy()[:2 * I].reshape(2, I, order="F"),
+            "interface_node_indices": codes.cpu().numpy()[:4 * I].reshape(2, 2, I, order="F")}
 
 
 class P4estMeshCubedSphere2DData:
