@@ -207,7 +207,15 @@ typedef struct {
   const int32_t* boundary_element;       /* [B] */
   const int32_t* boundary_node_indices;  /* [3,B] */
   const int32_t* boundary_condition;     /* [B] ATMO_BC_* */
-  const double* node_coordinates;        /* [3,n,n,n,E]; read only when the metric terms are NULL */
+  const double* node_coordinates;        /* [3,n,n,n,E]; read when the metric terms are NULL
+                                          * or n_interfaces < 0 */
+  /* n_interfaces < 0: the interface and boundary lists are built on the
+   * device from node_coordinates (atmo_mesh3d_connectivity_device, the same
+   * lists as atmo_mesh3d_arrays) and the lists above are not read; each
+   * boundary takes the condition of its kind (0 = :inside, 1 = :outside,
+   * 2 = other faces) */
+  const double* period;                  /* [3] period per axis, 0 = not periodic (NULL: none) */
+  int32_t boundary_condition_by_kind[3]; /* ATMO_BC_* */
 } atmo_geometry3d;
 
 /* Metric terms of P4estMesh{3} elements from their node coordinates
@@ -266,6 +274,10 @@ typedef struct {
   /* read only when aux_node_vars is NULL */
   int32_t christoffel_collocation;       /* 0: exact metric derivatives, 1: collocation */
   const double* bottom_topography;       /* [n,n,E] host array (aux row 20), or NULL for zero */
+  /* n_interfaces < 0: the interface lists are built on the device from
+   * node_coordinates (atmo_mesh2d_connectivity_device; one member, or each
+   * ensemble member's elements by themselves) and the lists above are not
+   * read */
 } atmo_geometry2d;
 
 /* init_auxiliary_node_variables! on the device

@@ -432,6 +432,71 @@ const char* atmo_last_error(const atmo_handle* h) {
   return h ? h->err.c_str() : global_error().c_str();
 }
 
+// device buffers of a set-up step, released on every return path
+struct DeviceBuffers {
+  std::vector<void*> ptrs;
+  ~DeviceBuffers() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  void* get(size_t bytes) {
+    void* q = nullptr;
+    if (hipMalloc(&q, bytes + 16) != hipSuccess) return nullptr;
+    ptrs.push_back(q);
+    return q;
+  }
+};
+
+// atmo_create with n_interfaces < 0: the interface and boundary lists are
+// built on the device from the node coordinates, copied to the host (the
+// create path validates them and builds the per-element link table there)
+// and handed to the ordinary path with each boundary's condition by kind
+static int create_with_device_connectivity(const atmo_config* cfg, const atmo_geometry3d* g,
+                                           atmo_handle** out) {
+  if (!g->node_coordinates) return fail(nullptr, ATMO_ERR_INVALID, "missing node coordinates");
+  const int n = cfg->polydeg + 1, E = cfg->n_elements;
+  const size_t n3 = (size_t)n * n * n;
+  DeviceBuffers db;
+  auto* d_xyz = static_cast<double*>(db.get(sizeof(double) * 3 * n3 * E));
+  auto* d_ids = static_cast<int32_t*>(db.get(sizeof(int32_t) * 2 * 3 * (size_t)E));
+  auto* d_codes = static_cast<int32_t*>(db.get(sizeof(int32_t) * 6 * 3 * (size_t)E));
+  auto* d_be = static_cast<int32_t*>(db.get(sizeof(int32_t) * 6 * (size_t)E));
+  auto* d_bcodes = static_cast<int32_t*>(db.get(sizeof(int32_t) * 3 * 6 * (size_t)E));
+  auto* d_kind = static_cast<int32_t*>(db.get(sizeof(int32_t) * 6 * (size_t)E));
+  if (!d_xyz || !d_ids || !d_codes || !d_be || !d_bcodes || !d_kind)
+    return fail(nullptr, ATMO_ERR_DEVICE, "device allocation failed");
+  if (hipMemcpy(d_xyz, g->node_coordinates, sizeof(double) * 3 * n3 * E, hipMemcpyHostToDevice) !=
+      hipSuccess)
+    return fail(nullptr, ATMO_ERR_DEVICE, "device upload failed");
+  int32_t ni = 0, nb = 0;
+  const int rc = atmo_mesh3d_connectivity_device(cfg->polydeg, E, d_xyz, g->period, d_ids,
+                                                 d_codes, d_be, d_bcodes, d_kind, &ni, &nb);
+  if (rc != ATMO_OK) return rc;
+  std::vector<int32_t> ids(2 * (size_t)ni), codes(6 * (size_t)ni), be(nb), bcodes(3 * (size_t)nb),
+      kind(nb), bc(nb);
+  const bool ok =
+      (ni == 0 || (hipMemcpy(ids.data(), d_ids, sizeof(int32_t) * ids.size(),
+                             hipMemcpyDeviceToHost) == hipSuccess &&
+                   hipMemcpy(codes.data(), d_codes, sizeof(int32_t) * codes.size(),
+                             hipMemcpyDeviceToHost) == hipSuccess)) &&
+      (nb == 0 || (hipMemcpy(be.data(), d_be, sizeof(int32_t) * nb, hipMemcpyDeviceToHost) ==
+                       hipSuccess &&
+                   hipMemcpy(bcodes.data(), d_bcodes, sizeof(int32_t) * bcodes.size(),
+                             hipMemcpyDeviceToHost) == hipSuccess &&
+                   hipMemcpy(kind.data(), d_kind, sizeof(int32_t) * nb, hipMemcpyDeviceToHost) ==
+                       hipSuccess));
+  if (!ok) return fail(nullptr, ATMO_ERR_DEVICE, "device download failed");
+  for (int b = 0; b < nb; ++b) bc[b] = g->boundary_condition_by_kind[kind[b]];
+  atmo_geometry3d gg = *g;
+  gg.n_interfaces = ni;
+  gg.interface_neighbor_ids = ids.data();
+  gg.interface_node_indices = codes.data();
+  gg.n_boundaries = nb;
+  gg.boundary_element = be.data();
+  gg.boundary_node_indices = bcodes.data();
+  gg.boundary_condition = bc.data();
+  return atmo_create(cfg, &gg, out);
+}
+
 int atmo_create(const atmo_config* cfg, const atmo_geometry3d* g, atmo_handle** out) {
   if (!cfg || !g || !out) return fail(nullptr, ATMO_ERR_INVALID, "null argument");
   if (cfg->equations != ATMO_EQ_EULER_ENERGY_GRAVITY_3D)
@@ -454,6 +519,7 @@ int atmo_create(const atmo_config* cfg, const atmo_geometry3d* g, atmo_handle** 
     return fail(nullptr, ATMO_ERR_INVALID, "FluxLMARS needs a positive speed of sound");
   if (cfg->source_terms != ATMO_SRC_NONE && cfg->source_terms != ATMO_SRC_CORIOLIS)
     return fail(nullptr, ATMO_ERR_INVALID, "unsupported source terms");
+  if (g->n_interfaces < 0) return create_with_device_connectivity(cfg, g, out);
   const bool metric_on_device = !g->contravariant_vectors && !g->inverse_jacobian;
   if ((!metric_on_device && (!g->contravariant_vectors || !g->inverse_jacobian)) ||
       (metric_on_device && !g->node_coordinates))

@@ -320,6 +320,68 @@ int atmo_covariant_aux_device(int polydeg, int32_t n_elements, const double* nod
   return ATMO_OK;
 }
 
+// atmo_swe_create with n_interfaces < 0: the interface lists of each member
+// (or of the one mesh) built on the device from its node coordinates, with
+// the member's first element added to the ids, then the ordinary path
+static int swe_create_with_device_connectivity(const atmo_swe_config* cfg,
+                                               const atmo_geometry2d* g, atmo_handle** out) {
+  const int n = cfg->polydeg + 1, E = cfg->n_elements;
+  const size_t n2 = (size_t)n * n;
+  const int M = g->member_first_element ? g->n_members : 1;
+  if (M < 1) return fail(nullptr, ATMO_ERR_INVALID, "n_members must be positive");
+  std::vector<int32_t> first(M + 1);
+  for (int m = 0; m <= M; ++m)
+    first[m] = g->member_first_element ? g->member_first_element[m] : (m == 0 ? 0 : E);
+  if (first[0] != 0 || first[M] != E)
+    return fail(nullptr, ATMO_ERR_INVALID, "member ranges must cover all elements");
+  for (int m = 0; m < M; ++m)
+    if (first[m + 1] <= first[m]) return fail(nullptr, ATMO_ERR_INVALID, "empty member range");
+  double* d_xyz = nullptr;
+  int32_t *d_ids = nullptr, *d_codes = nullptr;
+  auto release = [&]() {
+    if (d_xyz) (void)hipFree(d_xyz);
+    if (d_ids) (void)hipFree(d_ids);
+    if (d_codes) (void)hipFree(d_codes);
+  };
+  if (hipMalloc(&d_xyz, sizeof(double) * 3 * n2 * E) != hipSuccess ||
+      hipMalloc(&d_ids, sizeof(int32_t) * 2 * 2 * (size_t)E) != hipSuccess ||
+      hipMalloc(&d_codes, sizeof(int32_t) * 4 * 2 * (size_t)E) != hipSuccess ||
+      hipMemcpy(d_xyz, g->node_coordinates, sizeof(double) * 3 * n2 * E,
+                hipMemcpyHostToDevice) != hipSuccess) {
+    release();
+    return fail(nullptr, ATMO_ERR_DEVICE, "device allocation or upload failed");
+  }
+  std::vector<int32_t> ids, codes;
+  for (int m = 0; m < M; ++m) {
+    const int Em = first[m + 1] - first[m];
+    int32_t ni = 0;
+    const int rc = atmo_mesh2d_connectivity_device(cfg->polydeg, Em, d_xyz + 3 * n2 * first[m],
+                                                   d_ids, d_codes, &ni);
+    if (rc != ATMO_OK) {
+      release();
+      return rc;
+    }
+    const size_t i0 = ids.size() / 2;
+    ids.resize(2 * (i0 + ni));
+    codes.resize(4 * (i0 + ni));
+    if (ni > 0 &&
+        (hipMemcpy(ids.data() + 2 * i0, d_ids, sizeof(int32_t) * 2 * ni, hipMemcpyDeviceToHost) !=
+             hipSuccess ||
+         hipMemcpy(codes.data() + 4 * i0, d_codes, sizeof(int32_t) * 4 * ni,
+                   hipMemcpyDeviceToHost) != hipSuccess)) {
+      release();
+      return fail(nullptr, ATMO_ERR_DEVICE, "device download failed");
+    }
+    for (size_t q = 2 * i0; q < ids.size(); ++q) ids[q] += first[m];
+  }
+  release();
+  atmo_geometry2d gg = *g;
+  gg.n_interfaces = (int32_t)(ids.size() / 2);
+  gg.interface_neighbor_ids = ids.data();
+  gg.interface_node_indices = codes.data();
+  return atmo_swe_create(cfg, &gg, out);
+}
+
 int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_handle** out) {
   if (!cfg || !g || !out) return fail(nullptr, ATMO_ERR_INVALID, "null argument");
   if (cfg->polydeg < 1 || cfg->polydeg > 7)
@@ -339,6 +401,8 @@ int atmo_swe_create(const atmo_swe_config* cfg, const atmo_geometry2d* g, atmo_h
                 "the covariant form runs in FP64, with FP64 or FP32 geometry storage");
   const bool geo32 = cfg->precision == ATMO_PREC_FP32_GEOMETRY;
   if (!(cfg->gravity > 0)) return fail(nullptr, ATMO_ERR_INVALID, "gravity must be positive");
+  if (g->node_coordinates && g->n_interfaces < 0)
+    return swe_create_with_device_connectivity(cfg, g, out);
   if (!g->node_coordinates || g->n_interfaces < 0 ||
       (g->n_interfaces > 0 && (!g->interface_neighbor_ids || !g->interface_node_indices)))
     return fail(nullptr, ATMO_ERR_INVALID, "missing geometry arrays");

@@ -57,6 +57,8 @@ class AtmoGeometry3D(ctypes.Structure):
         ("boundary_node_indices", c_int32_p),
         ("boundary_condition", c_int32_p),
         ("node_coordinates", c_double_p),
+        ("period", c_double_p),
+        ("boundary_condition_by_kind", ctypes.c_int32 * 3),
     ]
 
 

@@ -97,8 +97,11 @@ class SemidiscretizationHyperbolic:
 
     def __init__(self, mesh, equations, solver, boundary_conditions=None, source_terms=None,
                  initial_condition=None, metric_terms=None, auxiliary_field=None,
-                 metric_terms_on_device=True):
+                 metric_terms_on_device=True, connectivity_on_device=False):
         self.mesh, self.equations, self.solver = mesh, equations, solver
+        # the interface (and boundary) lists: the mesh's host arrays, or built
+        # by the handle on the device from the node coordinates
+        self.connectivity_on_device = connectivity_on_device
         # covariant form: the auxiliary variables of the handle are computed on
         # the device (metric_terms_on_device) or uploaded from the host
         self.aux_on_device = metric_terms_on_device
@@ -121,29 +124,37 @@ class SemidiscretizationHyperbolic:
         for b in range(mesh.n_boundaries):
             name = bc.get(kinds[int(mesh.boundary_kind[b])], "boundary_condition_slip_wall")
             bcodes[b] = BOUNDARY_CONDITIONS[name]
+        by_kind = [BOUNDARY_CONDITIONS[bc.get(kinds[q], "boundary_condition_slip_wall")]
+                   for q in range(3)]
         # P4estMesh{3} metric terms: computed on the device from the node
         # coordinates (the same bits as the host's), or uploaded from the host
+        if metric_terms_on_device or connectivity_on_device:
+            xyz = np.ascontiguousarray(mesh.node_coordinates.ravel(order="F"))
+        else:
+            xyz = None
         if metric_terms_on_device:
             geo_arrays = [None, None]
-            xyz = np.ascontiguousarray(mesh.node_coordinates.ravel(order="F"))
         else:
             geo_arrays = [np.ascontiguousarray(mesh.contravariant_vectors.ravel(order="F")),
                           np.ascontiguousarray(mesh.inverse_jacobian.ravel(order="F"))]
-            xyz = None
         self._keep = [*geo_arrays,
                       np.ascontiguousarray(mesh.interface_neighbor_ids.ravel(order="F")),
                       np.ascontiguousarray(mesh.interface_node_indices.ravel(order="F")),
                       np.ascontiguousarray(mesh.boundary_element),
                       np.ascontiguousarray(mesh.boundary_node_indices.ravel(order="F")),
-                      bcodes]
+                      bcodes,
+                      np.ascontiguousarray(getattr(mesh, "period", np.zeros(3)), dtype=np.float64)]
         k = self._keep
         geo = AtmoGeometry3D(contravariant_vectors=dptr(k[0]) if k[0] is not None else None,
                              inverse_jacobian=dptr(k[1]) if k[1] is not None else None,
-                             n_interfaces=mesh.n_interfaces, interface_neighbor_ids=iptr(k[2]),
+                             n_interfaces=-1 if connectivity_on_device else mesh.n_interfaces,
+                             interface_neighbor_ids=iptr(k[2]),
                              interface_node_indices=iptr(k[3]), n_boundaries=mesh.n_boundaries,
                              boundary_element=iptr(k[4]), boundary_node_indices=iptr(k[5]),
                              boundary_condition=iptr(k[6]),
-                             node_coordinates=dptr(xyz) if xyz is not None else None)
+                             node_coordinates=dptr(xyz) if xyz is not None else None,
+                             period=dptr(k[7]),
+                             boundary_condition_by_kind=(ctypes.c_int32 * 3)(*by_kind))
         h = ctypes.c_void_p()
         check(lib().atmo_create(ctypes.byref(cfg), ctypes.byref(geo), ctypes.byref(h)))
         del xyz
@@ -168,7 +179,8 @@ class SemidiscretizationHyperbolic:
              np.ascontiguousarray(self.aux_node_vars[19].ravel(order="F"))]
         geo = AtmoGeometry2D(aux_node_vars=None if self.aux_on_device else dptr(k[0]),
                              node_coordinates=dptr(k[1]),
-                             n_interfaces=mesh.n_interfaces, interface_neighbor_ids=iptr(k[2]),
+                             n_interfaces=-1 if self.connectivity_on_device else mesh.n_interfaces,
+                             interface_neighbor_ids=iptr(k[2]),
                              interface_node_indices=iptr(k[3]), n_members=1,
                              member_first_element=None,
                              christoffel_collocation=1 if mt.collocation else 0,
@@ -293,8 +305,9 @@ class SemidiscretizationEnsemble(SemidiscretizationHyperbolic):
     auxiliary_field or None)."""
 
     def __init__(self, members, equations, solver, source_terms=None, metric_terms=None,
-                 metric_terms_on_device=True):
+                 metric_terms_on_device=True, connectivity_on_device=False):
         self.equations, self.solver = equations, solver
+        self.connectivity_on_device = connectivity_on_device
         self.source_terms = source_terms
         self.mesh = None
         self.initial_condition = None
@@ -328,7 +341,8 @@ class SemidiscretizationEnsemble(SemidiscretizationHyperbolic):
         has_topography = any(field is not None for _, _, field in members)
         geo = AtmoGeometry2D(aux_node_vars=None if metric_terms_on_device else dptr(k[0]),
                              node_coordinates=dptr(k[1]),
-                             n_interfaces=nbr.shape[1], interface_neighbor_ids=iptr(k[2]),
+                             n_interfaces=-1 if connectivity_on_device else nbr.shape[1],
+                             interface_neighbor_ids=iptr(k[2]),
                              interface_node_indices=iptr(k[3]), n_members=self.n_members,
                              member_first_element=iptr(k[4]),
                              christoffel_collocation=1 if mt.collocation else 0,
