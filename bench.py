@@ -27,21 +27,23 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3
 METRIC = "GDOF\u00b7RHS/s, mixed vs FP64 on 1 GPU at stated rel. err; % of VALU roofline"
 
 # Per-launch HBM traffic of the FP64 stage kernel at the default workload,
-# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch); FETCH_SIZE
-# reads half the bytes of a plain device copy of known size on this GPU, hence
-# the factor 2 (profiles/e3_stage_shell16x8_p4_pmc.txt).
-STAGE_FETCH_KIB, STAGE_WRITE_KIB, FETCH_FACTOR = 2.103e5, 1.321e5, 2.0
+# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch, means over
+# the run's dispatches): the stage issues only 128 B read requests
+# (TCC_EA0_RDREQ), which FETCH_SIZE counts as 64 B, as it does for a plain
+# device copy of known size, hence the factor 2
+# (profiles/e3_stage_shell16x8_p4_pmc.txt).
+STAGE_FETCH_KIB, STAGE_WRITE_KIB, FETCH_FACTOR = 2.057e5, 1.258e5, 2.0
 # Executed VALU flops per node and stage (SQ_INSTS_VALU_{ADD,MUL,TRANS}_F64 x 64
 # + SQ_INSTS_VALU_FMA_F64 x 128 per dispatch / nodes, same profile file); the
 # mixed form executes mostly FP32.
-EXEC_FP64_FLOPS_PER_NODE = 1790.0
+EXEC_FP64_FLOPS_PER_NODE = 1523.0
 # Algorithmic FP64 work of one rhs! per node of the shell 6x16^2x8, polydeg 4
 # (KG + Waruszewski volume, KG + LLF + Waruszewski surface, slip walls): the
 # oracle's own code run with an operation-counting scalar (oracle/count_flops.py;
 # + - * / sqrt log exp pow one each), every node pair of a line evaluated once
 # as the reference does
 ALG_FP64_FLOPS_PER_NODE = 1680.4
-EXEC_MIXED_FLOPS_PER_NODE = {"fp32": 1870.0, "fp64": 315.0}
+EXEC_MIXED_FLOPS_PER_NODE = {"fp32": 1857.0, "fp64": 310.0}
 
 
 def log(msg):
