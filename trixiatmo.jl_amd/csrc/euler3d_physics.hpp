@@ -58,22 +58,12 @@ __device__ __forceinline__ double rcp_refined(double x) {
 }
 __device__ __forceinline__ float rcp_refined(float x) { return __builtin_amdgcn_rcpf(x); }
 
-template <typename T>
-__device__ __forceinline__ T ln_mean_logs(T x, T y, T lx, T ly) {
-  const T d = y - x, s = x + y;
-  const bool ser = d * d < T(1.0e-4) * (s * s);
-  const T f2 = ser ? (d * d) * rcp_refined(s * s) : T(0);
-  const T q = T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0)));
-  return div_(ser ? s : d, ser ? q : ly - lx);
-}
-
-// the same value with a branch instead of selects, for callers whose lanes
-// mostly agree on the branch (the pairs of one line direction of an element)
 // a / b from a reciprocal refined by one Newton step and one correction of
 // the quotient by its residual: six VALU operations instead of the eleven of
 // the IEEE division sequence, within one unit in the last place of the
-// IEEE quotient (no special operands reach it: b is a difference of
-// logarithms or a sum of positive terms)
+// IEEE quotient (no special operands reach it: b is a nonzero difference
+// of two values or of their logarithms, a positive series denominator or a
+// sum of positive terms)
 __device__ __forceinline__ double div_nr(double a, double b) {
   const double r = rcp_refined(b);
   const double q = a * r;
@@ -81,6 +71,17 @@ __device__ __forceinline__ double div_nr(double a, double b) {
 }
 __device__ __forceinline__ float div_nr(float a, float b) { return div_(a, b); }
 
+template <typename T>
+__device__ __forceinline__ T ln_mean_logs(T x, T y, T lx, T ly) {
+  const T d = y - x, s = x + y;
+  const bool ser = d * d < T(1.0e-4) * (s * s);
+  const T f2 = ser ? (d * d) * rcp_refined(s * s) : T(0);
+  const T q = T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0)));
+  return div_nr(ser ? s : d, ser ? q : ly - lx);
+}
+
+// the same value with a branch instead of selects, for callers whose lanes
+// mostly agree on the branch (the pairs of one line direction of an element)
 template <typename T>
 __device__ __forceinline__ T ln_mean_logs_branch(T x, T y, T lx, T ly) {
   const T d = y - x, s = x + y;
@@ -111,7 +112,7 @@ __device__ __forceinline__ T inv_ln_mean_logs(T x, T y, T lx_minus_ly) {
   const bool ser = d * d < T(1.0e-4) * (s * s);
   const T f2 = ser ? (d * d) * rcp_refined(s * s) : T(0);
   const T q = T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0)));
-  return div_(ser ? q : -lx_minus_ly, ser ? s : d);
+  return div_nr(ser ? q : -lx_minus_ly, ser ? s : d);
 }
 
 // primitive quantities a two-point flux needs from one node

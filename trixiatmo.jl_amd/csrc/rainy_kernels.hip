@@ -196,7 +196,7 @@ __device__ __forceinline__ void flux_ec_rain(const ECNode& l, const ECNode& r, d
   if (l.rv != 0.0 && r.rv != 0.0) vap_log = ln_mean_logs(l.rv, r.rv, l.lrv, r.lrv);
   const double it_avg = 0.5 * (itl + itr);
   const double it_log = inv_ln_mean_logs(itl, itr, l.lit - r.lit);
-  const double p_int = (1.0 / it_avg) * (R_d * dry_a + R_v * vap_a);
+  const double p_int = div_nr(1.0, it_avg) * (R_d * dry_a + R_v * vap_a);
   const double f_vap = vap_log * vdn;
   const double f_cld = cld_a * vdn;
   const double f_dry = dry_log * vdn;
