@@ -13,45 +13,16 @@ struct E3Params {
   double gamma, gm1, inv_gm1, lmars_c;
 };
 
-// Arithmetic helpers: FP64 keeps IEEE division, sqrt and log (the FP64 path is
-// compared with the oracle to 1e-12); FP32 uses the native single-instruction
-// reciprocal, square root and base-2 logarithm (about 1 ulp).
+// Arithmetic helpers.  FP32 uses the native single-instruction reciprocal,
+// square root and base-2 logarithm (about 1 ulp).  FP64 division stays IEEE
+// in div_; the hot per-node and per-face-node quantities use div_nr, sqrt_ and
+// log_ below, which stay within one unit in the last place for the positive
+// normal operands the fluxes give them (the FP64 path is compared with the
+// oracle to 1e-12 of its term scale).
 __device__ __forceinline__ double div_(double a, double b) { return a / b; }
 __device__ __forceinline__ float div_(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-__device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
-__device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ double log_(double x) { return log(x); }
-__device__ __forceinline__ float log_(float x) {
-  return __builtin_amdgcn_logf(x) * 0.693147180559945309f;
-}
 
-// Logarithmic mean and its inverse with the f^2 < 1e-4 series branch of
-// Ranocha et al. (2023, ACM TOMS), the form of Trixi's ln_mean/inv_ln_mean
-// imported at src/TrixiAtmo.jl:37.
-template <typename T>
-__device__ __forceinline__ T ln_mean(T x, T y) {
-  const T f2 = div_(x * (x - T(2) * y) + y * y, x * (x + T(2) * y) + y * y);
-  if (f2 < T(1.0e-4)) {
-    return div_(x + y, T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0))));
-  }
-  return div_(y - x, log_(div_(y, x)));
-}
-
-template <typename T>
-__device__ __forceinline__ T inv_ln_mean(T x, T y) {
-  const T f2 = div_(x * (x - T(2) * y) + y * y, x * (x + T(2) * y) + y * y);
-  if (f2 < T(1.0e-4)) {
-    return div_(T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0))), x + y);
-  }
-  return div_(log_(div_(y, x)), y - x);
-}
-
-// ln_mean / inv_ln_mean from node logarithms: (y - x) / (log y - log x) away
-// from x = y, the series of Ranocha et al. (2023) where (x-y)^2/(x+y)^2 < 1e-4.
-// Both branches end in one division of a selected numerator and denominator,
-// so lanes of a wave that take different branches do not serialise; f^2 uses
-// a reciprocal refined by one Newton step (it only enters the small series
-// terms).
+// 1 / x from the hardware reciprocal and one Newton step
 __device__ __forceinline__ double rcp_refined(double x) {
   const double r = __builtin_amdgcn_rcp(x);
   return r * fma(-x, r, 2.0);
@@ -71,6 +42,91 @@ __device__ __forceinline__ double div_nr(double a, double b) {
 }
 __device__ __forceinline__ float div_nr(float a, float b) { return div_(a, b); }
 
+// square root of x >= 0 from the hardware reciprocal square root and the
+// coupled Newton iterations for sqrt(x) and 1 / (2 sqrt(x)) of Goldschmidt
+// (Markstein, IA-64 and Elementary Functions, 2000, ch. 9), without the
+// exponent scaling that keeps a library square root exact near the ends of
+// the exponent range (the speeds and normals here are far from them);
+// sqrt(0) = 0
+__device__ __forceinline__ double sqrt_(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  g = fma(d, h, g);
+  return x == 0.0 ? 0.0 : g;
+}
+__device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// natural logarithm: x = 2^k m with m in [sqrt(1/2), sqrt(2)), f = m - 1
+// exact, log(1 + f) = 2 atanh(s) with s = f / (2 + f), |s| < 0.172, written
+// as f - (f^2/2 - s (f^2/2 + R)) with R = sum_{j=1..10} 2 s^(2j) / (2j + 1)
+// (the truncated Taylor series; its remainder is below 2^-57 of the result),
+// and k log 2 added in two parts (the upper part has 21 trailing zero bits,
+// so k times it is exact).  Within 0.85 ulp of the exact value over
+// [1e-300, 1e300] (measured against a long-double logarithm); zero,
+// negative, subnormal and non-finite operands take the library logarithm.
+__device__ __forceinline__ double log_(double x) {
+  if (!(x >= 0x1p-1022 && x <= 0x1.fffffffffffffp+1023)) return log(x);
+  int k = __builtin_amdgcn_frexp_exp(x);
+  double m = __builtin_amdgcn_frexp_mant(x);  // [1/2, 1)
+  const bool lo = m < 0x1.6a09e667f3bcdp-1;   // sqrt(1/2)
+  m = lo ? m + m : m;
+  k = lo ? k - 1 : k;
+  const double f = m - 1.0;
+  const double s = div_nr(f, 2.0 + f);
+  const double z = s * s;
+  double R = 0x1.8618618618618p-4;  // 2/21
+  R = fma(R, z, 0x1.af286bca1af28p-4);  // 2/19
+  R = fma(R, z, 0x1.e1e1e1e1e1e1ep-4);  // 2/17
+  R = fma(R, z, 0x1.1111111111111p-3);  // 2/15
+  R = fma(R, z, 0x1.3b13b13b13b14p-3);  // 2/13
+  R = fma(R, z, 0x1.745d1745d1746p-3);  // 2/11
+  R = fma(R, z, 0x1.c71c71c71c71cp-3);  // 2/9
+  R = fma(R, z, 0x1.2492492492492p-2);  // 2/7
+  R = fma(R, z, 0x1.999999999999ap-2);  // 2/5
+  R = fma(R, z, 0x1.5555555555555p-1);  // 2/3
+  R = R * z;
+  const double hfsq = 0.5 * f * f;
+  const double dk = double(k);
+  return dk * 0x1.62e42fee00000p-1 -
+         ((hfsq - fma(s, hfsq + R, dk * 0x1.a39ef35793c76p-33)) - f);
+}
+__device__ __forceinline__ float log_(float x) {
+  return __builtin_amdgcn_logf(x) * 0.693147180559945309f;
+}
+
+// Logarithmic mean and its inverse with the f^2 < 1e-4 series branch of
+// Ranocha et al. (2023, ACM TOMS), the form of Trixi's ln_mean/inv_ln_mean
+// imported at src/TrixiAtmo.jl:37.
+template <typename T>
+__device__ __forceinline__ T ln_mean(T x, T y) {
+  const T f2 = div_nr(x * (x - T(2) * y) + y * y, x * (x + T(2) * y) + y * y);
+  if (f2 < T(1.0e-4)) {
+    return div_nr(x + y, T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0))));
+  }
+  return div_nr(y - x, log_(div_nr(y, x)));
+}
+
+template <typename T>
+__device__ __forceinline__ T inv_ln_mean(T x, T y) {
+  const T f2 = div_nr(x * (x - T(2) * y) + y * y, x * (x + T(2) * y) + y * y);
+  if (f2 < T(1.0e-4)) {
+    return div_nr(T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0))), x + y);
+  }
+  return div_nr(log_(div_nr(y, x)), y - x);
+}
+
+// ln_mean / inv_ln_mean from node logarithms: (y - x) / (log y - log x) away
+// from x = y, the series of Ranocha et al. (2023) where (x-y)^2/(x+y)^2 < 1e-4.
+// Both branches end in one division of a selected numerator and denominator,
+// so lanes of a wave that take different branches do not serialise; f^2 uses
+// a reciprocal refined by one Newton step (it only enters the small series
+// terms).
 template <typename T>
 __device__ __forceinline__ T ln_mean_logs(T x, T y, T lx, T ly) {
   const T d = y - x, s = x + y;
@@ -101,9 +157,9 @@ __device__ __forceinline__ T ln_mean_face(T x, T y) {
   const T d = y - x, s = x + y;
   if (d * d < T(1.0e-4) * (s * s)) {
     const T f2 = (d * d) * rcp_refined(s * s);
-    return div_(s, T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0))));
+    return div_nr(s, T(2) + f2 * (T(2.0 / 3.0) + f2 * (T(2.0 / 5.0) + f2 * T(2.0 / 7.0))));
   }
-  return div_(d, log_(y) - log_(x));
+  return div_nr(d, log_(y) - log_(x));
 }
 
 template <typename T>
@@ -126,7 +182,7 @@ template <typename T>
 __device__ __forceinline__ Prim<T> cons2prim(const E3Params& q, const T* u) {
   Prim<T> w;
   w.rho = u[0];
-  const T inv = div_(T(1), w.rho);
+  const T inv = div_nr(T(1), w.rho);
   w.v1 = u[1] * inv;
   w.v2 = u[2] * inv;
   w.v3 = u[3] * inv;

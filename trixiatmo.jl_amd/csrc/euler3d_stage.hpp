@@ -478,7 +478,7 @@ __device__ __forceinline__ void face_fluxes_to_lds(
     own.v3 = s_w[3 * N3 + node]; own.p = s_w[4 * N3 + node]; own.e = s_w[5 * N3 + node];
     own.rho_e = T(uo[4]);
     own.phi = T(uo[5]);
-    own.inv_rho = div_(T(1), own.rho);
+    own.inv_rho = div_nr(T(1), own.rho);
     const Prim<T> pr = to_t<T>(cons2prim(q, ur));
     T ut[6], urt[6];
 #pragma unroll
@@ -558,7 +558,7 @@ __device__ __forceinline__ void node_fields_to_lds(const E3Params& q, const doub
   const Prim<double> w = cons2prim(q, uo);
   s_w[0 * N3 + t] = T(w.rho); s_w[1 * N3 + t] = T(w.v1); s_w[2 * N3 + t] = T(w.v2);
   s_w[3 * N3 + t] = T(w.v3); s_w[4 * N3 + t] = T(w.p);
-  s_w[5 * N3 + t] = T(w.rho_e / w.rho);
+  s_w[5 * N3 + t] = T(div_nr(w.rho_e, w.rho));
   s_w[6 * N3 + t] = T(w.phi);
   if constexpr (VOL == 0 || NC == 1) s_w[7 * N3 + t] = log_(T(w.rho));
   if constexpr (VOL == 0) s_w[8 * N3 + t] = log_(T(w.rho / w.p));

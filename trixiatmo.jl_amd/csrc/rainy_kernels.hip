@@ -48,9 +48,9 @@ __device__ __forceinline__ void nonlinear_sol(const double* u, double& rv, doubl
   T = u[8];
 }
 
-// x^(1/8) by three correctly rounded square roots (pow(x, 0.125) of the
-// reference to a few ulp, at a fraction of pow's cost)
-__device__ __forceinline__ double root8(double x) { return sqrt(sqrt(sqrt(x))); }
+// x^(1/8) by three square roots (pow(x, 0.125) of the reference to a few
+// ulp, at a fraction of pow's cost)
+__device__ __forceinline__ double root8(double x) { return sqrt_(sqrt_(sqrt_(x))); }
 
 // terminal_velocity_rain (:336-351)
 __device__ __forceinline__ double terminal_velocity(double rho_moist, double rho_rain) {
@@ -94,7 +94,7 @@ __device__ __forceinline__ RNode rnode(const double* u) {
   const double q_v = a.rv / a.rd;
   const double q_l = (a.rc + a.rr) / a.rd;
   const double gm = 1 + (R_d + R_v * q_v) / (c_vd + c_vv * q_v + c_l * q_l);
-  a.c = sqrt(gm * a.p * a.inv);
+  a.c = sqrt_(gm * a.p * a.inv);
   a.vr_m = terminal_velocity(a.rm, a.rr);
   a.vr_sol = terminal_velocity(a.rv + a.rc, a.rr);
   return a;
@@ -226,9 +226,9 @@ __device__ __forceinline__ ECNode ecnode(const double* u) {
   a.v2 = u[4] * inv;
   a.it = 1.0 / T;
   a.vr = terminal_velocity(a.rv + a.rc, a.rr);
-  a.lrd = log(a.rd);
-  a.lrv = log(a.rv);
-  a.lit = -log(T);
+  a.lrd = log_(a.rd);
+  a.lrv = log_(a.rv);
+  a.lit = -log_(T);
   return a;
 }
 
@@ -242,7 +242,7 @@ __device__ __forceinline__ void flux_llf(const RNode& l, const RNode& r, const d
   vl += fabs(l.vr_m * n2);
   double vr = fabs(r.v1 * n1 + r.v2 * n2);
   vr += fabs(r.vr_m * n2);
-  const double lam = fmax(vl, vr) + fmax(l.c, r.c) * sqrt(n1 * n1 + n2 * n2);
+  const double lam = fmax(vl, vr) + fmax(l.c, r.c) * sqrt_(n1 * n1 + n2 * n2);
 #pragma unroll
   for (int v = 0; v < NV; ++v) f[v] = 0.5 * (fl[v] + fr[v]) - 0.5 * lam * (ur[v] - ul[v]);
 }
@@ -262,7 +262,7 @@ __device__ __forceinline__ void source_rainy(const double* u, double* s) {
   const double rho_vs = svp(T) / (R_v * T);
   const double rr = u[2];
   // rr^(3/16), rr^(1/2), rr^(7/8) from the chain of square roots
-  const double r2 = sqrt(rr), r4 = sqrt(r2), r8 = sqrt(r4), r16 = sqrt(r8);
+  const double r2 = sqrt_(rr), r4 = sqrt_(r2), r8 = sqrt_(r4), r16 = sqrt_(r8);
   double S_ev = (3.86e-3 - 9.41e-5 * (T - ref_T)) * (1 + 9.1 * (r16 * r8));
   S_ev *= (rho_vs - rv) * r2;
   const double S = 0.001 * rc + 1.72 * rc * (r2 * r4 * r8) - S_ev;
