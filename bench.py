@@ -26,17 +26,21 @@ FP64_VECTOR_PEAK_TFLOPS = 78.6
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 METRIC = "GDOF\u00b7RHS/s, mixed vs FP64 on 1 GPU at stated rel. err; % of VALU roofline"
 
-# Per-launch HBM traffic of the FP64 stage kernel at the default workload,
-# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch, means over
-# the run's dispatches): the stage issues only 128 B read requests
-# (TCC_EA0_RDREQ), which FETCH_SIZE counts as 64 B, as it does for a plain
-# device copy of known size, hence the factor 2
-# (profiles/e3_stage_shell16x8_p4_pmc.txt).
+# HBM traffic and executed FP64 flops of the FP64 stage kernel are measured by
+# this script itself (measure_counters below: rocprofv3 --pmc passes over a
+# short child run of the same workload).  Where rocprofv3 is missing or a pass
+# fails, these figures of the committed profile
+# (profiles/e3_stage_shell16x8_p4_pmc.txt) stand in, and the JSON line says so:
+# FETCH_SIZE and WRITE_SIZE in KiB per dispatch, the FETCH_SIZE scale (the
+# stage issues 128 B read requests, counted as 64 B, as for a plain device
+# copy of known size) and the executed flops per node
+# (64 x (ADD + MUL + TRANS) + 128 x FMA wave instructions / nodes).
 STAGE_FETCH_KIB, STAGE_WRITE_KIB, FETCH_FACTOR = 2.057e5, 1.258e5, 2.0
-# Executed VALU flops per node and stage (SQ_INSTS_VALU_{ADD,MUL,TRANS}_F64 x 64
-# + SQ_INSTS_VALU_FMA_F64 x 128 per dispatch / nodes, same profile file); the
-# mixed form executes mostly FP32.
 EXEC_FP64_FLOPS_PER_NODE = 1523.0
+COUNTER_PASSES = (("FETCH_SIZE",), ("WRITE_SIZE",),
+                  ("SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_ADD_F64",
+                   "SQ_INSTS_VALU_TRANS_F64"))
+CALIBRATION_COPY_BYTES = 256 << 20
 # Algorithmic FP64 work of one rhs! per node of the shell 6x16^2x8, polydeg 4
 # (KG + Waruszewski volume, KG + LLF + Waruszewski surface, slip walls): the
 # oracle's own code run with an operation-counting scalar (oracle/count_flops.py;
@@ -48,6 +52,118 @@ EXEC_MIXED_FLOPS_PER_NODE = {"fp32": 1857.0, "fp64": 310.0}
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def counter_probe(a):
+    """the child run the counter passes profile: one device copy of
+    CALIBRATION_COPY_BYTES (the largest copy of the run, which scales the
+    byte counters), then the FP64 shell stage in the given workgroup shape
+    (one warm-up step and two more, fifteen stage dispatches)"""
+    import torch
+    import trixiatmo_hip as T
+    src = torch.ones(CALIBRATION_COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    mesh, eq, solver, bcs, u0 = T.workloads.shell(a.trees, a.layers, polydeg=a.polydeg, seed=0,
+                                                  precision="fp64")
+    semi = T.SemidiscretizationHyperbolic(mesh, eq, solver, bcs)
+    semi.set_kernel_variant(a.counter_probe)
+    semi.set_state(u0)
+    dt = semi.max_dt(0.5)
+    semi.step(dt, 3)
+    semi.synchronize()
+    print("counter probe done", flush=True)
+
+
+def measure_counters(a, variant, nodes, time_limit=240):
+    """rocprofv3 --pmc passes (one per group of COUNTER_PASSES) over
+    counter_probe in a child process; returns per-dispatch means of the stage
+    kernel's counters and the copy's, or None and the reason"""
+    import glob
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None, "rocprofv3 not on PATH"
+    stage = f"e3_stage_direct<{a.polydeg + 1}, 1, 1, double, false, {variant + 1}, double>"
+    values = {}
+    with tempfile.TemporaryDirectory(prefix="atmo_pmc_") as tmp:
+        for i, counters in enumerate(COUNTER_PASSES):
+            out_dir = os.path.join(tmp, f"pass{i}")
+            cmd = [prof, "--pmc", *counters, "-f", "csv",
+                   "--kernel-include-regex", "e3_stage_direct|copyBuffer", "-d", out_dir, "-o", "run",
+                   "--", sys.executable, os.path.abspath(__file__), "--counter-probe", str(variant),
+                   "--trees", str(a.trees), "--layers", str(a.layers), "--polydeg", str(a.polydeg)]
+            log(f"counters: pass {i + 1} of {len(COUNTER_PASSES)} ({' '.join(counters)})")
+            p = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, cwd=tmp,
+                                 start_new_session=True, env=dict(os.environ, TMPDIR=tmp))
+            try:
+                text = p.communicate(timeout=time_limit)[0].decode(errors="replace")
+            except subprocess.TimeoutExpired:
+                os.killpg(p.pid, signal.SIGKILL)
+                p.communicate()
+                return None, f"pass {i + 1} exceeded {time_limit} s"
+            files = glob.glob(os.path.join(out_dir, "**", "*counter_collection.csv"), recursive=True)
+            # rocprofv3 can end with a fault of its own after a complete run:
+            # the probe's last line and the result file decide
+            if "counter probe done" not in text or not files:
+                return None, f"pass {i + 1} failed (exit {p.returncode}): {text.strip()[-300:]}"
+            got, why = counter_means(files, stage, counters)
+            if got is None:
+                return None, f"pass {i + 1}: {why}"
+            values.update(got)
+    return counter_summary(values, stage, nodes), None
+
+
+def counter_means(files, stage, counters):
+    """per counter: the mean over the stage kernel's dispatches (rows of one
+    dispatch summed) and, for the byte counters, the largest copy dispatch's
+    value, from rocprofv3's counter_collection.csv files"""
+    import csv
+    per = {}
+    for f in files:
+        with open(f) as fh:
+            for n, r in enumerate(csv.DictReader(fh)):
+                name, c = r["Kernel_Name"], r["Counter_Name"]
+                key = "stage" if stage in name else ("copy" if "copyBuffer" in name else None)
+                if key is None:
+                    continue
+                d = r.get("Dispatch_Id") or r.get("Correlation_Id") or f"{f}:{n}"
+                per.setdefault((key, c), {}).setdefault(d, 0.0)
+                per[(key, c)][d] += float(r["Counter_Value"])
+    values = {}
+    for c in counters:
+        st = list(per.get(("stage", c), {}).values())
+        if not st:
+            return None, f"no {c} value for {stage}"
+        values[c] = sum(st) / len(st)
+        values[c + "_dispatches"] = len(st)
+        if c in ("FETCH_SIZE", "WRITE_SIZE"):
+            cp = list(per.get(("copy", c), {}).values())
+            if not cp:
+                return None, f"no {c} value for the calibration copy"
+            values["copy_" + c] = max(cp)
+    return values, None
+
+
+def counter_summary(values, stage, nodes):
+    """HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE, each scaled by
+    what the counter reads for the calibration copy of known size, and the
+    executed FP64 flops per node"""
+    read_factor = CALIBRATION_COPY_BYTES / (values["copy_FETCH_SIZE"] * 1024.0)
+    write_factor = CALIBRATION_COPY_BYTES / (values["copy_WRITE_SIZE"] * 1024.0)
+    flops = (64.0 * (values["SQ_INSTS_VALU_ADD_F64"] + values["SQ_INSTS_VALU_MUL_F64"] +
+                     values["SQ_INSTS_VALU_TRANS_F64"]) + 128.0 * values["SQ_INSTS_VALU_FMA_F64"])
+    return {"kernel": stage, "dispatches": values["FETCH_SIZE_dispatches"],
+            "fetch_kib": values["FETCH_SIZE"], "write_kib": values["WRITE_SIZE"],
+            "copy_fetch_kib": values["copy_FETCH_SIZE"], "copy_write_kib": values["copy_WRITE_SIZE"],
+            "read_factor": read_factor, "write_factor": write_factor,
+            "traffic": (values["FETCH_SIZE"] * read_factor + values["WRITE_SIZE"] * write_factor) * 1024.0,
+            "exec_fp64_flops_per_node": flops / nodes}
 
 
 def cpu_baseline(mesh, eq, solver, u0, du_gpu=None, max_seconds=30.0, min_calls=20,
@@ -477,7 +593,14 @@ def main():
                     help="rocprofv3 FETCH_SIZE / WRITE_SIZE of the stage kernel and the FETCH scale")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (to scale HBM counters)")
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the rocprofv3 counter passes (traffic and executed flops then "
+                         "come from the committed profile)")
+    ap.add_argument("--counter-probe", type=int, default=None, help=argparse.SUPPRESS)
     a = ap.parse_args()
+    if a.counter_probe is not None:
+        counter_probe(a)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {a.gpus}); the multi-GPU path "
               "(element partition with a face-halo exchange per stage) is not built.",
@@ -513,6 +636,7 @@ def main():
         semi.set_state(u0)
         if dt is None:
             dt = semi.max_dt(0.5)
+        shape, shape_ms = semi.autotune()
         i0 = semi.integrals(fresh_rhs=True)
         log(f"[{prec}] warmup {a.warmup} steps, dt = {dt:.4g} s")
         semi.step(dt, a.warmup)
@@ -529,7 +653,8 @@ def main():
         semi.enable_timing(False)
         u_end = semi.get_state()
         i1 = semi.integrals(fresh_rhs=True)
-        results[prec] = dict(el=el, gdof=nodes * 5 * a.steps / el / 1e9,
+        results[prec] = dict(el=el, gdof=nodes * 5 * a.steps / el / 1e9, shape=shape,
+                             shape_ms=[float(x) for x in shape_ms[:2]],
                              avg_s=total_ms / launches / 1e3, u_end=u_end,
                              ok=bool(np.all(np.isfinite(u_end))),
                              mass_drift=(i1[0] - i0[0]) / i0[0],
@@ -564,10 +689,20 @@ def main():
     bytes_mixed = 8 * (6 + 5 + 5 + 5 + 1) + 4 * 9
     bytes_geo = bytes_mixed
     achieved = bytes_per_node * nodes / avg_s / 1e9
-    fetch_kib, write_kib, fetch_factor = a.traffic_kib or (STAGE_FETCH_KIB, STAGE_WRITE_KIB,
-                                                           FETCH_FACTOR)
-    traffic = (fetch_kib * fetch_factor + write_kib) * 1024.0
-    valu64 = EXEC_FP64_FLOPS_PER_NODE * nodes / avg_s / 1e12
+    counters, counter_note = None, "skipped (--no-counters)"
+    if not a.no_counters and not a.traffic_kib:
+        counters, counter_note = measure_counters(a, f64["shape"], nodes)
+    if counters is not None:
+        traffic, exec_flops = counters["traffic"], counters["exec_fp64_flops_per_node"]
+        counter_source = "rocprofv3 --pmc passes in this run"
+    else:
+        fetch_kib, write_kib, fetch_factor = a.traffic_kib or (STAGE_FETCH_KIB, STAGE_WRITE_KIB,
+                                                               FETCH_FACTOR)
+        traffic = (fetch_kib * fetch_factor + write_kib) * 1024.0
+        exec_flops = EXEC_FP64_FLOPS_PER_NODE
+        counter_source = ("--traffic-kib" if a.traffic_kib else
+                          f"profiles/e3_stage_shell16x8_p4_pmc.txt ({counter_note})")
+    valu64 = exec_flops * nodes / avg_s / 1e12
     valu_mix = (EXEC_MIXED_FLOPS_PER_NODE["fp32"] / FP32_VECTOR_PEAK_TFLOPS +
                 EXEC_MIXED_FLOPS_PER_NODE["fp64"] / FP64_VECTOR_PEAK_TFLOPS) * nodes / mix["avg_s"] / 1e12
 
@@ -621,16 +756,20 @@ def main():
                    "elements": mesh.n_elements, "nodes": nodes, "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "e3_stage_direct<5,1,1,double,false,1>", "avg_launch_us": avg_s * 1e6,
-                     "bytes_per_node": bytes_per_node, "traffic_bytes_per_node": traffic / nodes},
-        "valu": {"fp64_executed_tflops": valu64,
+                     "kernel": f"e3_stage_direct<{a.polydeg + 1},1,1,double,false,{f64['shape'] + 1}>",
+                     "elements_per_workgroup": f64["shape"] + 1,
+                     "autotuner_ms_per_shape": f64["shape_ms"], "avg_launch_us": avg_s * 1e6,
+                     "bytes_per_node": bytes_per_node, "traffic_bytes_per_node": traffic / nodes,
+                     "traffic_source": counter_source, "counters": counters},
+        "valu": {"fp64_executed_tflops": valu64, "executed_flops_per_node": exec_flops,
                  "frac_of_fp64_vector_peak": valu64 / FP64_VECTOR_PEAK_TFLOPS,
                  "algorithmic_flops_per_node": ALG_FP64_FLOPS_PER_NODE,
                  "fp64_algorithmic_tflops": ALG_FP64_FLOPS_PER_NODE * nodes / avg_s / 1e12,
                  "algorithmic_frac_of_fp64_vector_peak":
                      ALG_FP64_FLOPS_PER_NODE * nodes / avg_s / 1e12 / FP64_VECTOR_PEAK_TFLOPS,
                  "mixed_frac_of_vector_peak": valu_mix,
-                 "note": "executed VALU flops from rocprofv3 SQ_INSTS_VALU_* counters"},
+                 "note": "executed VALU flops from rocprofv3 SQ_INSTS_VALU_*_F64 counters "
+                         f"({counter_source})"},
         "cpu_baseline": cpu,
         "mixed": {"precision": "fp32 fluxes + Kahan sums, fp64 state/pressure/RK",
                   "value": mix["gdof"], "speedup_vs_fp64": mix["gdof"] / gdof,

@@ -283,14 +283,6 @@ int e3_autotune(atmo_handle* h, double* ms_per_variant) {
   return ATMO_OK;
 }
 
-}  // namespace atmo
-
-namespace {
-
-}  // namespace
-
-namespace atmo {
-
 // per-element speeds of the state a resident loop starts from; every later
 // step's come from its own last stage
 int e3_resident_prologue(atmo_handle* h) {
