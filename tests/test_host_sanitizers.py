@@ -1,11 +1,11 @@
 """Host code of libatmo_hip under AddressSanitizer and UndefinedBehaviorSanitizer.
 
 tests/asan/Makefile builds every source of the library with the sanitizers on
-the host side (the device code is compiled as usual) and links the driver
-tests/asan/host_checks.cpp, which runs the mesh builders at every polydeg and
+the host side (the device code is compiled as usual) and links the checking
+program tests/asan/host_checks.cpp, which runs the mesh builders at every polydeg and
 the configuration and connectivity checks of the three create calls on inputs
 they must refuse; nothing in it touches a device, so it runs on the CPU.  The
-sanitizers abort the driver on a memory or undefined-behaviour error.
+sanitizers abort that program on a memory or undefined-behaviour error.
 """
 import os
 import shutil
