@@ -16,9 +16,11 @@
 // either stores du or applies k = a k + dt du, u_out = u + b k (ping-pong
 // state buffers, since neighbours still read the old state's face nodes).
 //
-// Compute type T: double (FP64 path, IEEE ops, compared with the oracle to
-// 1e-12) or float (mixed path: node states, pressure recovery, geopotential
-// jumps, the hydrostatic-background split and the RK registers stay FP64;
+// Compute type T: double (FP64 path, compared with the oracle to 1e-12; its
+// logarithm, square root and hot quotients are the within-one-ulp helpers of
+// euler3d_physics.hpp) or float (mixed path: node states, pressure recovery,
+// geopotential jumps, the dissipation's state jumps, the
+// hydrostatic-background split and the RK registers stay FP64;
 // fluxes, geometry and volume sums are FP32, with Kahan-compensated sums when
 // KAHAN is set).  With the background split the momentum rows carry only
 // p' = p - p_bg and (rho_avg - rho_avg,bg) [phi]; the FP64 tendency of the
