@@ -85,19 +85,20 @@ struct Acc {
   }
 };
 
-// jump = ur - ul formed from the FP64 states before rounding to T (the two
-// states of a face node differ by little, so the difference of the rounded
-// states would be mostly their rounding)
+// the dissipation's jumps are formed from the FP64 states uld, urd and then
+// rounded to T (the two states of a face node differ by little, so the
+// difference of the rounded states would be mostly their rounding)
 template <typename T>
 __device__ __forceinline__ void surface_flux(const E3Params& q, int surf, const T* ul, const T* ur,
-                                             const T* jump, const Prim<T>& l, const Prim<T>& r,
-                                             T n0, T n1, T n2, T f[6]) {
+                                             const double* uld, const double* urd,
+                                             const Prim<T>& l, const Prim<T>& r, T n0, T n1, T n2,
+                                             T f[6]) {
   if (surf == 0) {  // flux_lax_friedrichs
     flux_central(l, r, n0, n1, n2, f);
     const T lam = max_abs_speed(q, l, r, n0, n1, n2);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) f[v] += T(-0.5) * lam * jump[v];
-    f[5] = T(-0.5) * lam * jump[5];
+    for (int v = 0; v < 5; ++v) f[v] += T(-0.5) * lam * T(urd[v] - uld[v]);
+    f[5] = T(-0.5) * lam * T(urd[5] - uld[5]);
   } else if (surf == 1) {
     flux_ranocha(q, l, r, n0, n1, n2, f);
     f[5] = T(0);
@@ -112,8 +113,8 @@ __device__ __forceinline__ void surface_flux(const E3Params& q, int surf, const 
     else flux_ranocha(q, l, r, n0, n1, n2, f);
     const T lam = max_abs_speed(q, l, r, n0, n1, n2);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) f[v] += T(-0.5) * lam * jump[v];
-    f[5] = T(-0.5) * lam * jump[5];
+    for (int v = 0; v < 5; ++v) f[v] += T(-0.5) * lam * T(urd[v] - uld[v]);
+    f[5] = T(-0.5) * lam * T(urd[5] - uld[5]);
   }
 }
 
@@ -241,7 +242,8 @@ __device__ __forceinline__ Prim<T> to_t(const Prim<double>& w) {
 // (p' for the pressure terms; wave speeds and upwinding keep the full state).
 template <typename T>
 __device__ __forceinline__ void surface_momentum_split(const E3Params& q, int surf, const T* ul,
-                                                       const T* ur, const T* jump, const Prim<T>& l,
+                                                       const T* ur, const double* uld,
+                                                       const double* urd, const Prim<T>& l,
                                                        const Prim<T>& r, T ppl, T ppr, T n0, T n1,
                                                        T n2, T m[3]) {
   const T nv[3] = {n0, n1, n2};
@@ -253,7 +255,7 @@ __device__ __forceinline__ void surface_momentum_split(const E3Params& q, int su
 #pragma unroll
     for (int k = 0; k < 3; ++k)
       m[k] = T(0.5) * ((l.rho * vnl * vl[k] + ppl * nv[k]) + (r.rho * vnr * vr[k] + ppr * nv[k])) +
-             T(-0.5) * lam * jump[k + 1];
+             T(-0.5) * lam * T(urd[k + 1] - uld[k + 1]);
   } else if (surf == 3) {  // FluxLMARS
     const T c = T(q.lmars_c);
     const T nn = sqrt_(n0 * n0 + n1 * n1 + n2 * n2);
@@ -274,7 +276,7 @@ __device__ __forceinline__ void surface_momentum_split(const E3Params& q, int su
     if (surf >= 4) {
       const T lam = max_abs_speed(q, l, r, n0, n1, n2);
 #pragma unroll
-      for (int k = 0; k < 3; ++k) m[k] += T(-0.5) * lam * jump[k + 1];
+      for (int k = 0; k < 3; ++k) m[k] += T(-0.5) * lam * T(urd[k + 1] - uld[k + 1]);
     }
   }
 }
@@ -485,15 +487,11 @@ __device__ __forceinline__ void face_fluxes_to_lds(
     own.phi = T(uo[5]);
     own.inv_rho = div_nr(T(1), own.rho);
     const Prim<T> pr = to_t<T>(cons2prim(q, ur));
-    T ut[6], urt[6], jump[6];
+    T ut[6], urt[6];
 #pragma unroll
-    for (int v = 0; v < 6; ++v) {
-      ut[v] = T(uo[v]);
-      urt[v] = T(ur[v]);
-      jump[v] = T(ur[v] - uo[v]);
-    }
+    for (int v = 0; v < 6; ++v) { ut[v] = T(uo[v]); urt[v] = T(ur[v]); }
     T fs[6];
-    surface_flux<T>(q, surf, ut, urt, jump, own, pr, n0, n1, n2, fs);
+    surface_flux<T>(q, surf, ut, urt, uo, ur, own, pr, n0, n1, n2, fs);
     double pb_o = 0.0, pb_n = 0.0;
     if (bg.on) {
       // own background pressure from the prologue; across a face phi is
@@ -505,7 +503,7 @@ __device__ __forceinline__ void face_fluxes_to_lds(
       T m[3];
       const double pp_o = (q.gm1 * (uo[4] - 0.5 * (uo[1] * uo[1] + uo[2] * uo[2] + uo[3] * uo[3]) / uo[0])) - pb_o;
       const double pp_n = (q.gm1 * (ur[4] - 0.5 * (ur[1] * ur[1] + ur[2] * ur[2] + ur[3] * ur[3]) / ur[0])) - pb_n;
-      surface_momentum_split<T>(q, surf, ut, urt, jump, own, pr, T(pp_o), T(pp_n), n0, n1, n2, m);
+      surface_momentum_split<T>(q, surf, ut, urt, uo, ur, own, pr, T(pp_o), T(pp_n), n0, n1, n2, m);
       fs[1] = m[0];
       fs[2] = m[1];
       fs[3] = m[2];
