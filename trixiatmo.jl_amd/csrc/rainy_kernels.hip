@@ -55,7 +55,7 @@ __device__ __forceinline__ double root8(double x) { return sqrt_(sqrt_(sqrt_(x))
 // terminal_velocity_rain (:336-351)
 __device__ __forceinline__ double terminal_velocity(double rho_moist, double rho_rain) {
   if (rho_rain > 0.0)
-    return v_0 * 1.9386213994279082 * root8(rho_rain / (kPi * (rho_moist + rho_rain) * N_0));
+    return v_0 * 1.9386213994279082 * root8(div_nr(rho_rain, kPi * (rho_moist + rho_rain) * N_0));
   return 0.0;
 }
 
@@ -84,16 +84,16 @@ __device__ __forceinline__ RNode rnode(const double* u) {
   a.rm = u[1];
   a.rr = u[2];
   a.rho = u[0] + u[1] + u[2];
-  a.inv = 1.0 / a.rho;
+  a.inv = div_nr(1.0, a.rho);
   a.v1 = u[3] * a.inv;
   a.v2 = u[4] * a.inv;
   a.E = u[5];
   nonlinear_sol(u, a.rv, a.rc, a.T);
   a.p = (R_d * a.rd + R_v * a.rv) * a.T;
   // speed_of_sound (:299-334)
-  const double q_v = a.rv / a.rd;
-  const double q_l = (a.rc + a.rr) / a.rd;
-  const double gm = 1 + (R_d + R_v * q_v) / (c_vd + c_vv * q_v + c_l * q_l);
+  const double q_v = div_nr(a.rv, a.rd);
+  const double q_l = div_nr(a.rc + a.rr, a.rd);
+  const double gm = 1 + div_nr(R_d + R_v * q_v, c_vd + c_vv * q_v + c_l * q_l);
   a.c = sqrt_(gm * a.p * a.inv);
   a.vr_m = terminal_velocity(a.rm, a.rr);
   a.vr_sol = terminal_velocity(a.rv + a.rc, a.rr);
@@ -217,14 +217,14 @@ __device__ __forceinline__ void flux_ec_rain(const ECNode& l, const ECNode& r, d
 __device__ __forceinline__ ECNode ecnode(const double* u) {
   ECNode a;
   const double rho = u[0] + u[1] + u[2];
-  const double inv = 1.0 / rho;
+  const double inv = div_nr(1.0, rho);
   double T;
   nonlinear_sol(u, a.rv, a.rc, T);
   a.rd = u[0];
   a.rr = u[2];
   a.v1 = u[3] * inv;
   a.v2 = u[4] * inv;
-  a.it = 1.0 / T;
+  a.it = div_nr(1.0, T);
   a.vr = terminal_velocity(a.rv + a.rc, a.rr);
   a.lrd = log_(a.rd);
   a.lrv = log_(a.rv);
