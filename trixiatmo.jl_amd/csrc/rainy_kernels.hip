@@ -651,8 +651,14 @@ void rainy_stage(RainyLaunch L) {
       }
     }
   } else {
-    // flux differencing with flux_ec_rain, gathered per node; each pair is
-    // evaluated with the lower node index on the left as the scatter form does
+    // flux differencing with flux_ec_rain, gathered per node.  flux_ec_rain
+    // gives the same bits with its two states exchanged (every operation is
+    // commutative or negates both operands of a quotient), so each lane takes
+    // its partners in rotated order own + 1, ..., own + n - 1 (mod n) with its
+    // own node on the left: every lane has a partner in each of the n - 1
+    // steps and the wave evaluates one flux per step (a loop over the line's
+    // n nodes that skips the own one and orders the pair by index runs both
+    // orders of the call under divergent lanes in every step)
     if (active) {
       double uo[NV];
 #pragma unroll
@@ -672,9 +678,9 @@ void rainy_stage(RainyLaunch L) {
       for (int o = 0; o < 2; ++o) {
         const int own = o == 0 ? ti : tj;
         const double n1 = o == 0 ? L.Ja11 : 0.0, n2 = o == 0 ? 0.0 : L.Ja22;
-#pragma unroll
-        for (int m = 0; m < N; ++m) {
-          if (m == own) continue;
+#pragma unroll 1
+        for (int j = 1; j < N; ++j) {
+          const int m = own + j >= N ? own + j - N : own + j;
           const int pq = o == 0 ? m + N * tj : ti + N * m;
           ECNode b;
           b.rd = s_ec(es, 0, pq); b.rv = s_ec(es, 1, pq); b.rc = s_ec(es, 2, pq);
@@ -682,8 +688,7 @@ void rainy_stage(RainyLaunch L) {
           b.it = s_ec(es, 6, pq); b.vr = s_ec(es, 7, pq);
           b.lrd = s_ec(es, 8, pq); b.lrv = s_ec(es, 9, pq); b.lit = s_ec(es, 10, pq);
           double f[6];
-          if (own < m) flux_ec_rain(a, b, n1, n2, f);
-          else flux_ec_rain(b, a, n1, n2, f);
+          flux_ec_rain(a, b, n1, n2, f);
           const double ds = L.dsplit[own * N + m];
 #pragma unroll
           for (int v = 0; v < 6; ++v) du[v] += ds * f[v];
