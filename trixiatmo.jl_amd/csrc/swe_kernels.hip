@@ -326,9 +326,11 @@ void swe_stage(SweLaunch L) {
 #pragma unroll
       for (int o = 0; o < 2; ++o) {
         const int own = o == 0 ? ti : tj;
-#pragma unroll
-        for (int m = 0; m < N; ++m) {
-          if (m == own) continue;
+        // the partners in rotated order own + 1, ..., own + n - 1 (mod n):
+        // every lane has a partner in each step
+#pragma unroll 1
+        for (int j = 1; j < N; ++j) {
+          const int m = own + j >= N ? own + j - N : own + j;
           const int pq = o == 0 ? m + N * tj : ti + N * m;
           const double hr = s_n[es][0][pq], hvr_o = s_n[es][1 + o][pq];
           const double vr[2] = {s_n[es][3][pq], s_n[es][4][pq]};
