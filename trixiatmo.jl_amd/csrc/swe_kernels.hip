@@ -393,31 +393,36 @@ void swe_stage(SweLaunch L) {
       if constexpr (N == 4) {
         // polydeg 3 keeps everything in registers with the orientation as a
         // run-time value, and runs faster so (measured)
+        // the two states selected per lane and one call: the lanes of a wave
+        // hold faces of both signs, and a branch per sign runs both calls
         const int o = f / 2;
-        if (f % 2 == 0) {  // negative-side face: -(f*(u_nbr, u_own) + g/2)
-          surface_terms(L, tr, ul, ax, o, fs);
+        const bool neg = f % 2 == 0;  // negative-side face: -(f*(u_nbr, u_own) + g/2)
+        double sl[NV], sr[NV];
 #pragma unroll
-          for (int v = 0; v < NV; ++v) fs[v] = -fs[v];
-        } else {
-          surface_terms(L, ul, tr, ax, o, fs);
+        for (int v = 0; v < NV; ++v) {
+          sl[v] = neg ? tr[v] : ul[v];
+          sr[v] = neg ? ul[v] : tr[v];
         }
+        surface_terms(L, sl, sr, ax, o, fs);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) fs[v] = neg ? -fs[v] : fs[v];
       } else {
         // the orientation as a compile-time constant in each branch: every
         // index into the node arrays (u[1 + o], G^{io}) is then fixed and they
-        // stay in registers (with a run-time orientation they went to scratch)
-        if (f == 0) {  // negative-side faces: -(f*(u_nbr, u_own) + g/2)
-          surface_terms(L, tr, ul, ax, 0, fs);
+        // stay in registers (with a run-time orientation they went to scratch);
+        // the sign's two states selected per lane, so a wave runs at most the
+        // two orientations' calls
+        const bool neg = f % 2 == 0;  // negative-side faces: -(f*(u_nbr, u_own) + g/2)
+        double sl[NV], sr[NV];
 #pragma unroll
-          for (int v = 0; v < NV; ++v) fs[v] = -fs[v];
-        } else if (f == 2) {
-          surface_terms(L, tr, ul, ax, 1, fs);
-#pragma unroll
-          for (int v = 0; v < NV; ++v) fs[v] = -fs[v];
-        } else if (f == 1) {
-          surface_terms(L, ul, tr, ax, 0, fs);
-        } else {
-          surface_terms(L, ul, tr, ax, 1, fs);
+        for (int v = 0; v < NV; ++v) {
+          sl[v] = neg ? tr[v] : ul[v];
+          sr[v] = neg ? ul[v] : tr[v];
         }
+        if (f < 2) surface_terms(L, sl, sr, ax, 0, fs);
+        else surface_terms(L, sl, sr, ax, 1, fs);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) fs[v] = neg ? -fs[v] : fs[v];
       }
 #pragma unroll
       for (int v = 0; v < NV; ++v) s_sf[fe][v][r] = fs[v] * inv_w;
