@@ -578,10 +578,13 @@ __device__ __forceinline__ void rainy_node_speeds(const RainyLaunch& L, const do
 // SPEED: the RK stage also writes L.speed_partial (the last stage of a
 // resident step); a variant of its own, so the other stages keep their
 // register budget
-// polydeg 3: a four-wave register budget (128 VGPRs, 64 B of scratch per
-// lane) measured 67.0 against 68.2 us per stage at 136 VGPRs and three waves
+// polydeg 3 without the network: a four-wave register budget (128 VGPRs,
+// 64 B of scratch per lane) measured 67.0 against 68.2 us per stage at 136
+// VGPRs and three waves; the network-seeded variant keeps the compiler's
+// choice (it measured about 1 us slower with that budget)
 template <int N, bool SEEDED, bool SPEED>
-__global__ __launch_bounds__(RShape<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(N == 4 ? 4 : 1)))
+__global__ __launch_bounds__(RShape<N>::BLOCK)
+__attribute__((amdgpu_waves_per_eu(N == 4 && !SEEDED ? 4 : 1)))
 void rainy_stage(RainyLaunch L) {
   using S = RShape<N>;
   constexpr int N2 = S::N2, EPB = S::EPB, B = S::BLOCK;
