@@ -580,8 +580,8 @@ __device__ __forceinline__ void rainy_node_speeds(const RainyLaunch& L, const do
 // register budget
 // polydeg 3 without the network: a four-wave register budget (128 VGPRs,
 // 64 B of scratch per lane) measured 67.0 against 68.2 us per stage at 136
-// VGPRs and three waves; the network-seeded variant keeps the compiler's
-// choice (it measured about 1 us slower with that budget)
+// VGPRs and three waves; the network-seeded variant, whose inference holds
+// more values, keeps the compiler's choice
 template <int N, bool SEEDED, bool SPEED>
 __global__ __launch_bounds__(RShape<N>::BLOCK)
 __attribute__((amdgpu_waves_per_eu(N == 4 && !SEEDED ? 4 : 1)))
