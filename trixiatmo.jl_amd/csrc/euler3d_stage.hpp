@@ -783,9 +783,11 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
           jo[1] = s_ja[1][t];
           jo[2] = s_ja[2][t];
         }
-#pragma unroll
-        for (int m = 0; m < N; ++m) {
-          if (m == own_d) continue;
+        // the partners in rotated order own_d + 1, ..., own_d + n - 1 (mod n):
+        // every lane gathers in each step
+#pragma unroll 1
+        for (int j = 1; j < N; ++j) {
+          const int m = own_d + j >= N ? own_d + j - N : own_d + j;
           const int a = m < own_d ? m : own_d, b = m < own_d ? own_d : m;
           const int pid = (own_line - line0) * P + a * (2 * N - a - 1) / 2 + (b - a - 1);
           const T ds = s_ds[own_d * N + m];
