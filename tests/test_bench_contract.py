@@ -70,3 +70,18 @@ def test_counter_files_give_the_stage_means_and_the_copy_scale(tmp_path):
     assert s["exec_fp64_flops_per_node"] == (64.0 * 6.0 + 128.0 * 10.0) / 4
     missing, why = b.counter_means([str(f)], stage.replace("5,", "6,"), ("FETCH_SIZE",))
     assert missing is None and "no FETCH_SIZE" in why
+
+
+def test_counter_files_without_the_expected_columns_are_an_error_not_a_crash(tmp_path):
+    """a counter file of another layout makes counter_means raise, which
+    bench.py's main catches (the line then falls back to the committed
+    profile and says so)"""
+    import pytest
+    b = _bench_module()
+    f = tmp_path / "run_counter_collection.csv"
+    f.write_text("Kernel,Counter,Value\nx,FETCH_SIZE,1\n")
+    with pytest.raises(KeyError):
+        b.counter_means([str(f)], "e3_stage_direct<5, 1, 1, double, false, 1, double>",
+                        ("FETCH_SIZE",))
+    src = open(os.path.join(ROOT, "bench.py")).read()
+    assert "except Exception as exc" in src and "counter passes failed" in src
