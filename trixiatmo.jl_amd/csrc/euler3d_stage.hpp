@@ -845,8 +845,10 @@ void e3_stage_direct(
   constexpr int NW = NodeFields<VOL, NC, T>::NW;
   constexpr int BUF = 9 * N3 > 36 * N2 ? 9 * N3 : 36 * N2;
   // flux_kennedy_gruber (with or without a gravity term) takes the sum form
-  // below
+  // below, and so does the FP64 flux_ranocha (alone or with the Waruszewski
+  // term)
   constexpr bool KG_SUMS = VOL == 1;
+  constexpr bool RANOCHA_SUMS = VOL == 0 && !MIXED && NC != 2;
   // EPB element slots of BE lanes each; slot es owns s_w_all[es], s_buf_all[es]
   __shared__ T s_w_all[EPB][NW * N3];
   __shared__ double s_phi_all[EPB][MIXED ? N3 : 1];
@@ -945,6 +947,34 @@ void e3_stage_direct(
             acc_nc[2].add(ds * (n2s * sm));
             acc_nc[3].add(ds * (sj * vns));
           }
+        } else if constexpr (RANOCHA_SUMS) {
+          // flux_ranocha and the Waruszewski term with every factor 1/2 of
+          // the averages (normals, velocities, pressure) left out and applied
+          // once at the end: 4 f_rho, 8 f_m, 8 f_E, 2 g_m and 4 g_E, each the
+          // averaged form's value times a power of two, so the sums round as
+          // the averaged form's do
+          const T n0s = jo0 + ja[pn], n1s = jo1 + ja[N3 + pn], n2s = jo2 + ja[2 * N3 + pn];
+          const T vnls = l.v1 * n0s + l.v2 * n1s + l.v3 * n2s;
+          const T vnrs = r.v1 * n0s + r.v2 * n1s + r.v3 * n2s;
+          const T rho_mean = ln_mean_logs(l.rho, r.rho, l.lrho, r.lrho);
+          const T inv_rp = l.p * r.p * inv_ln_mean_logs(l.rho * r.p, r.rho * l.p, l.lrp - r.lrp);
+          const T v1s = l.v1 + r.v1, v2s = l.v2 + r.v2, v3s = l.v3 + r.v3;
+          const T ps2 = T(2) * (l.p + r.p);
+          const T vvs = l.v1 * r.v1 + l.v2 * r.v2 + l.v3 * r.v3;
+          const T F = rho_mean * (vnls + vnrs);
+          acc[0].add(ds * F);
+          acc[1].add(ds * (F * v1s + ps2 * n0s));
+          acc[2].add(ds * (F * v2s + ps2 * n1s));
+          acc[3].add(ds * (F * v3s + ps2 * n2s));
+          acc[4].add(ds * (F * (vvs + inv_rp * T(2.0 * q.inv_gm1)) +
+                           T(2) * (l.p * vnrs + r.p * vnls)));
+          if constexpr (NC == 1) {
+            const T sj = rho_mean * dphi;
+            acc_nc[0].add(ds * (n0s * sj));
+            acc_nc[1].add(ds * (n1s * sj));
+            acc_nc[2].add(ds * (n2s * sj));
+            acc_nc[3].add(ds * (n0s * sj * v1s + n1s * sj * v2s + n2s * sj * v3s));
+          }
         } else {
           const T n0 = T(0.5) * (jo0 + ja[pn]);
           const T n1 = T(0.5) * (jo1 + ja[N3 + pn]);
@@ -964,6 +994,13 @@ void e3_stage_direct(
     }
     if constexpr (KG_SUMS) {
       constexpr T sc[5] = {T(0.125), T(0.0625), T(0.0625), T(0.0625), T(0.0625)};
+      constexpr T scn[4] = {T(0.5), T(0.5), T(0.5), T(0.25)};
+#pragma unroll
+      for (int v = 0; v < 5; ++v) acc[v].scale(sc[v]);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc_nc[v].scale(scn[v]);
+    } else if constexpr (RANOCHA_SUMS) {
+      constexpr T sc[5] = {T(0.25), T(0.125), T(0.125), T(0.125), T(0.125)};
       constexpr T scn[4] = {T(0.5), T(0.5), T(0.5), T(0.25)};
 #pragma unroll
       for (int v = 0; v < 5; ++v) acc[v].scale(sc[v]);
