@@ -893,20 +893,24 @@ void e3_stage_direct(
       const int own_d = d == 0 ? ti : (d == 1 ? tj : tk);
       const T* ja = s_buf + 3 * d * N3;
       const T jo0 = ja[tl], jo1 = ja[N3 + tl], jo2 = ja[2 * N3 + tl];
+      // per lane: the step from which the partner index wraps, and the
+      // Dsplit row of the own node
+      const int lim = N - own_d;
+      const T* ds_row = s_ds + own_d * N + own_d;
       // the partners in cyclic order own_d + 1, ..., own_d + n - 1 (mod n):
       // every lane has a partner in each of the n - 1 iterations (a loop over
       // the line's n nodes that skips the own one leaves a lane in n idle in
       // every iteration)
 #pragma unroll 1
       for (int j = 1; j < N; ++j) {
-        const bool wrap = own_d + j >= N;
-        const int m = wrap ? own_d + j - N : own_d + j;
-        const int pn = wrap ? tl + (j - N) * stride : tl + j * stride;  // the partner node
+        const bool wrap = j >= lim;
+        const int jw = wrap ? j - N : j;  // partner index minus own index
+        const int pn = tl + jw * stride;  // the partner node
         const FastPrim<T> r = load_fast_prim<VOL, NC, N3, T>(s_w, pn, bg);
         T dphi;
         if constexpr (MIXED) dphi = T(s_phi[pn] - phi_own);
         else dphi = r.phi - l.phi;
-        const T ds = s_ds[own_d * N + m];
+        const T ds = ds_row[jw];
         if constexpr (KG_SUMS) {
           // flux_kennedy_gruber and the gravity term from sums of the two
           // nodes' values: every average's factor 1/2 is a power of two, so
