@@ -128,3 +128,53 @@ def test_log_means_from_node_logarithms(probe, op):
     print(f"  op {op}: series branch max relative error {rel[series].max():.3e}; log branch max "
           f"error / bound {(rel[~series] / bound[~series]).max():.3f}", flush=True)
     assert np.all(rel <= bound)
+
+
+@pytest.mark.parametrize("direction", [0, 1])
+def test_flux_ec_rain_gives_the_same_bits_with_its_states_exchanged(direction):
+    """the rainy stage takes each node's partners in rotated order with the
+    own node on the left, so every pair is evaluated once in each order; the
+    flux must give the same bits either way (else the two nodes of a pair
+    would not see the same flux)"""
+    if not os.path.exists(LIB):
+        raise FileNotFoundError(f"{LIB} is missing: run __graft_entry__.build()")
+    lib = ctypes.CDLL(LIB)
+    f = lib.atmo_probe_flux_ec_rain
+    dp = ctypes.POINTER(ctypes.c_double)
+    f.argtypes = [dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, dp]
+    f.restype = ctypes.c_int
+    rng = np.random.default_rng(11 + direction)
+    n = 100000
+
+    def nodes(size):
+        rd = rng.uniform(0.4, 1.3, size)
+        rv = np.where(rng.random(size) < 0.05, 0.0, rng.uniform(1e-4, 2e-2, size))
+        rc = np.where(rng.random(size) < 0.5, 0.0, rng.uniform(0.0, 2e-3, size))
+        rr = np.where(rng.random(size) < 0.5, 0.0, rng.uniform(0.0, 2e-3, size))
+        v = rng.uniform(-40.0, 40.0, (2, size))
+        T = rng.uniform(230.0, 310.0, size)
+        vr = rng.uniform(0.0, 8.0, size)
+        with np.errstate(divide="ignore"):
+            return np.stack([rd, rv, rc, rr, v[0], v[1], 1.0 / T, vr, np.log(rd), np.log(rv),
+                             -np.log(T)], axis=1)
+
+    l = nodes(n)
+    r = nodes(n)
+    # a third of the pairs close together (the series branch of the log means)
+    close = slice(0, n // 3)
+    r[close] = l[close] * (1.0 + rng.uniform(-1e-3, 1e-3, (n // 3, 11)))
+    r[close, 8] = np.log(r[close, 0])
+    with np.errstate(divide="ignore"):
+        r[close, 9] = np.log(r[close, 1])
+    r[close, 10] = np.log(r[close, 6])
+    l, r = np.ascontiguousarray(l), np.ascontiguousarray(r)
+    n1, n2 = (64.0, 0.0) if direction == 0 else (0.0, 51.2)
+    f_lr = np.empty((n, 6))
+    f_rl = np.empty((n, 6))
+    rc = f(l.ctypes.data_as(dp), r.ctypes.data_as(dp), n, n1, n2,
+           f_lr.ctypes.data_as(dp), f_rl.ctypes.data_as(dp))
+    assert rc == 0, f"HIP error {rc}"
+    assert np.all(np.isfinite(f_lr))
+    differ = np.count_nonzero(f_lr.view(np.int64) != f_rl.view(np.int64))
+    print(f"  direction {direction}: {differ} of {f_lr.size} entries differ", flush=True)
+    assert differ == 0
