@@ -542,11 +542,17 @@ __attribute__((amdgpu_waves_per_eu(N == 4 && !SEEDED ? 4 : 1)))
 void rainy_stage(RainyLaunch L) {
   using S = RShape<N>;
   constexpr int N2 = S::N2, EPB = S::EPB, B = S::BLOCK;
-  constexpr int VOLW = 12 * N2 > NV * S::NF ? 12 * N2 : NV * S::NF;
+  // one buffer per element, used in turn for the volume terms (flux
+  // differencing: the per-node quantities of flux_ec_rain [11][N2]; weak form:
+  // the contravariant fluxes of the 6 flux rows, [2][6][N2] where they fit,
+  // else one direction [6][N2] at a time) and for the face fluxes [9][4 n].
+  // Sized by flux differencing and the faces: at polydeg 3 the workgroup then
+  // takes 10,240 B of LDS and sixteen fit on a CU (160 KiB) where 10,752 B
+  // let fifteen, which put the 8,192 workgroups of a 256 x 128 mesh in three
+  // rounds instead of two
+  constexpr int VOLW = 11 * N2 > NV * S::NF ? 11 * N2 : NV * S::NF;
+  constexpr bool WEAK_SPLIT = 12 * N2 > VOLW;
   __shared__ double s_u[EPB][NV][N2];
-  // one buffer per element, used in turn for the volume terms (weak form: the
-  // contravariant fluxes of the 6 flux rows [2][6][N2]; flux differencing: the
-  // per-node quantities of flux_ec_rain [11][N2]) and for the face fluxes [9][4 n]
   __shared__ double s_buf[EPB][VOLW];
   auto s_f = [&](int es, int o, int v, int q) -> double& { return s_buf[es][(o * 6 + v) * N2 + q]; };
   auto s_ec = [&](int es, int k, int q) -> double& { return s_buf[es][k * N2 + q]; };
@@ -585,6 +591,7 @@ void rainy_stage(RainyLaunch L) {
 
   if (L.volume_integral == 0) {
     // weak form: node fluxes into LDS, then du_i = sum_m dhat[i][m] F_m per line
+    double g2[6];  // the second direction's contravariant flux (split form)
     if (active) {
       double uo[NV];
 #pragma unroll
@@ -596,17 +603,44 @@ void rainy_stage(RainyLaunch L) {
 #pragma unroll
       for (int v = 0; v < 6; ++v) {
         s_f(es, 0, v, q) = L.Ja11 * f1[v] + 0.0 * f2[v];
-        s_f(es, 1, v, q) = 0.0 * f1[v] + L.Ja22 * f2[v];
+        g2[v] = 0.0 * f1[v] + L.Ja22 * f2[v];
+        if constexpr (!WEAK_SPLIT) s_f(es, 1, v, q) = g2[v];
       }
     }
     __syncthreads();
-    if (active) {
+    if constexpr (!WEAK_SPLIT) {
+      if (active) {
 #pragma unroll
-      for (int m = 0; m < N; ++m) {
-        const double d1 = L.dhat[ti * N + m], d2 = L.dhat[tj * N + m];
+        for (int m = 0; m < N; ++m) {
+          const double d1 = L.dhat[ti * N + m], d2 = L.dhat[tj * N + m];
 #pragma unroll
-        for (int v = 0; v < 6; ++v)
-          du[v] += d1 * s_f(es, 0, v, m + N * tj) + d2 * s_f(es, 1, v, ti + N * m);
+          for (int v = 0; v < 6; ++v)
+            du[v] += d1 * s_f(es, 0, v, m + N * tj) + d2 * s_f(es, 1, v, ti + N * m);
+        }
+      }
+    } else {
+      // one direction at a time through the same rows of the buffer
+      if (active) {
+#pragma unroll
+        for (int m = 0; m < N; ++m) {
+          const double d1 = L.dhat[ti * N + m];
+#pragma unroll
+          for (int v = 0; v < 6; ++v) du[v] += d1 * s_f(es, 0, v, m + N * tj);
+        }
+      }
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int v = 0; v < 6; ++v) s_f(es, 0, v, q) = g2[v];
+      }
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int m = 0; m < N; ++m) {
+          const double d2 = L.dhat[tj * N + m];
+#pragma unroll
+          for (int v = 0; v < 6; ++v) du[v] += d2 * s_f(es, 0, v, ti + N * m);
+        }
       }
     }
   } else {
