@@ -10,7 +10,7 @@ t = 2); test/test_2d_rainy_euler.jl:79-114 (the rainy bubble of
 examples/euler/precipitation/buoyancy/elixir_hoeck_bubble.jl, 16 x 16 cells,
 10 s: flux_ec_rain, LLF, source_terms_rainy, the simple slip wall and the
 stage limiter together; with the reference's adaptive SSPRK43 restated, and
-with this build's fixed-step integrator within the measured integrator gap);
+with this package's fixed-step integrator within the measured integrator gap);
 test/test_unit.jl:526-544 (flux_ec_rain against flux on the reference's
 state and normals).  Closed forms: saturated states of that test and unsaturated states
 solve the saturation system exactly; the Jacobian and the derivative of the
@@ -405,7 +405,7 @@ def test_bubble_regression_with_the_reference_integrator(oracle):
 
 def bubble_ck2n54_bars(oracle):
     """the oracle's fixed-step CarpenterKennedy2N54 run (CFL 0.25, the
-    integrator of this build) and the bars of its regression values: three
+    integrator of this package) and the bars of its regression values: three
     times the measured gap between that run and the restated reference
     integrator, plus 1e-6 relative; returns (u, steps, l2 bar, linf bar)"""
     B, mesh, ro, u0 = _bubble_oracle(oracle)
@@ -422,7 +422,7 @@ def bubble_ck2n54_bars(oracle):
 
 
 def test_bubble_regression_fixed_steps(oracle):
-    """the oracle with this build's integrator (fixed-step CK2N54 at CFL
+    """the oracle with this package's integrator (fixed-step CK2N54 at CFL
     0.25): the regression values within the derived integrator gap"""
     import rainy_bubble_pin as B
     u, steps, bar_l2, bar_linf = bubble_ck2n54_bars(oracle)
