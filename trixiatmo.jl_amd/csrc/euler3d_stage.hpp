@@ -381,14 +381,16 @@ __device__ __forceinline__ bool face_node_of(int t, int ps, int n_valid, int& es
 // normal of every face pass: the own outward contravariant vector on a primary
 // or boundary face, minus the primary's on a secondary face (Trixi evaluates
 // an interface with the primary element's normal).  Read before the face
-// fluxes overwrite the LDS copy of the own vectors (s_cv = [9][n^3] per slot,
-// slot stride cv_stride, or null to read them from memory).
+// fluxes overwrite the LDS copy of the own vectors (LDS_CV: s_cv = [9][n^3]
+// per slot, slot stride cv_stride; else read from memory).  A compile-time
+// choice: a run-time null test leaves the compiler a pointer that may be LDS
+// or global memory, which it reads with flat loads.
 template <int N, typename T, int EPB = 1>
 struct FaceNormals {
   T n[FacePrefetch<N, EPB>::PASSES][3];
 };
 
-template <int N, typename T, int EPB = 1, typename G = T>
+template <int N, typename T, int EPB = 1, typename G = T, bool LDS_CV = true>
 __device__ __forceinline__ void face_normals(FaceNormals<N, T, EPB>& fn, int e0, int n_valid, int t,
                                              const G* __restrict__ cv, const T* s_cv,
                                              int cv_stride, const FacePrefetch<N, EPB>& pf,
@@ -409,8 +411,8 @@ __device__ __forceinline__ void face_normals(FaceNormals<N, T, EPB>& fn, int e0,
       const size_t e = (size_t)(e0 + es);
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        fn.n[ps][c] = sgn * (s_cv ? s_cv[es * cv_stride + (3 * d + c) * N3 + node]
-                                  : T(cv[(e * N3 + node) * 9 + 3 * d + c]));
+        if constexpr (LDS_CV) fn.n[ps][c] = sgn * s_cv[es * cv_stride + (3 * d + c) * N3 + node];
+        else fn.n[ps][c] = sgn * T(cv[(e * N3 + node) * 9 + 3 * d + c]);
     } else {
       const int dp = info & 7;
       const T sgn = (dp % 2 == 0) ? T(1) : T(-1);  // minus the primary's outward normal
@@ -807,7 +809,7 @@ __global__ __launch_bounds__(Shape<N>::BLOCK) void e3_stage(
     }
   }
   FaceNormals<N, T> fn;
-  face_normals<N, T>(fn, e, 1, t, cv, nullptr, 0, pf, nbr, face_info);
+  face_normals<N, T, 1, T, false>(fn, e, 1, t, cv, nullptr, 0, pf, nbr, face_info);
   __syncthreads();  // pair terms consumed; the buffer now takes the face fluxes
   face_fluxes_to_lds<N, T>(q, e, 1, t, inv_w, u, pf, fn, nbr, face_info, surf, snc, bg, s_w, 0,
                            s_pb, s_fl, 0);

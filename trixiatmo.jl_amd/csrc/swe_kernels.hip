@@ -463,7 +463,11 @@ void swe_stage(SweLaunch L) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) L.out[g * NV + v] = du[v];
       } else {
-        const double dt = L.dt_member ? L.dt_member[L.member[e]] : L.rk_dt;
+        // the member's step size, else the launch's; selected as values (one
+        // conditional form with both reads merges them into a read through a
+        // pointer to global or argument memory, a flat load)
+        const double dt_m = L.dt_member ? L.dt_member[L.member[e]] : 0.0;
+        const double dt = L.dt_member ? dt_m : L.rk_dt;
         double un[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
