@@ -79,13 +79,17 @@ def test_rhs_bubble(oracle, vol, surf, periodic_y):
     check_rhs(oracle, mesh, u.reshape(9, 4, 4, -1, order="F"), vol, surf, "source_terms_rainy")
 
 
+@pytest.mark.parametrize("vol", [1, 0])
 @pytest.mark.parametrize("polydeg", [1, 2, 4, 5, 6, 7])
-def test_rhs_polydeg(oracle, polydeg):
+def test_rhs_polydeg(oracle, polydeg, vol):
+    """every polydeg with both volume integrals; the weak form takes both
+    directions' fluxes through LDS at once up to polydeg 2 and one direction
+    at a time from polydeg 3 on"""
     mesh, u = bubble_state(nx=4, ny=3, polydeg=polydeg)
     n = polydeg + 1
     ro = RainyOracle(oracle, mesh)
     u, _, _ = ro.newton(u)
-    check_rhs(oracle, mesh, u.reshape(9, n, n, -1, order="F"), 1, "flux_lax_friedrichs",
+    check_rhs(oracle, mesh, u.reshape(9, n, n, -1, order="F"), vol, "flux_lax_friedrichs",
               "source_terms_rainy")
 
 
